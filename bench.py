@@ -1,0 +1,177 @@
+"""Benchmark: single-source BFS with exact path counts on R-MAT scale 20.
+
+Workload (one step = one BFS from one source, graph resident in HBM):
+  R-MAT scale 20 (n = 1,048,576), edge factor 16, a=.57 b=.19 c=.19 d=.05,
+  symmetrized, deduplicated, randomly relabelled (seed 2), int32 CSC;
+  64 random sources of nonzero degree (seed 2); steps cycle through them.
+value = GTEPS = sum over timed BFS of (m_s / 2) / total time, where m_s is
+the number of in-list entries of the vertices reached (undirected edges
+counted once).  Every timed BFS computes depths S and exact uint64 path
+counts sigma.
+
+Prints ONE JSON line.  Also reports the roofline of the dominant kernel and a
+CPU baseline (the reference's own sequential bfs_seq_td_csc, compiled from
+its sources into oracle/_ref, else this project's C restatement).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "GTEPS on 1 MI355X + batched sources/s & p99 latency vs per-source loop, exact"
+
+
+def _quiet_call(fn, *a):
+    """Run fn with the process's stdout fd sent to /dev/null (the reference's
+    sequential code prints its timings with printf)."""
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    libc.fflush(None)
+    saved = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        return fn(*a)
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(devnull)
+
+
+def cpu_baseline(g, sources, budget_s: float):
+    """Time the reference's sequential BFS (or the restatement) on host cores,
+    one thread, on a bounded sample of the same sources."""
+    import oracle
+
+    kind = "reference" if oracle.ref_available("td") else "port"
+    edges = 0
+    t_total = 0.0
+    done = 0
+    for r in sources:
+        t0 = time.perf_counter()
+        if kind == "reference":
+            S, _ = _quiet_call(oracle.ref_seq_td_csc, g.CP, g.IC, int(r), "td")
+            reached = (S > 0)
+            reached[int(r)] = True
+        else:
+            S, _, _, _ = oracle.seq_td_csc(g.CP, g.IC, int(r), s_unreached=-1)
+            reached = S >= 0
+        t_total += time.perf_counter() - t0
+        edges += int(np.diff(g.CP)[reached].sum())
+        done += 1
+        if t_total > budget_s:
+            break
+    gteps = (edges / 2) / t_total / 1e9
+    return {
+        "value": gteps, "unit": "GTEPS", "cores": 1, "kind": kind,
+        "sample": f"{done} of {len(sources)} sources, bfs_seq_td_csc "
+                  f"(O(levels*(n+nz)) sweep), single thread, "
+                  f"{t_total:.1f} s on {os.cpu_count()} host cores",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--sources", type=int, default=64)
+    ap.add_argument("--mode", default="auto", choices=["auto", "auto_wa", "pull_sc", "pull_wa", "push", "cooc"])
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the single-source BFS is measured on one MI355X", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import turbobfs_amd as tb
+    from turbobfs_amd import _lib, graphs
+
+    torch.cuda.set_device(0)
+    g = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)
+    sources = graphs.sample_sources(graphs.nonzero_degree_vertices(g), args.sources, seed=2)
+    modes = {
+        "auto": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_SC),
+        "auto_wa": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_WA),
+        "pull_sc": dict(mode=_lib.MODE_PULL_SC),
+        "pull_wa": dict(mode=_lib.MODE_PULL_WA),
+        "push": dict(mode=_lib.MODE_PUSH),
+        "cooc": dict(mode=_lib.MODE_COOC),
+    }
+    opts = tb.default_opts(**modes[args.mode])
+    gr = tb.Graph(g.CP, g.IC, symmetric=True)
+    ctx = tb.Context(g.n)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+
+    for i in range(args.warmup):
+        ctx.run(gr, int(sources[i % len(sources)]), opts)
+    torch.cuda.synchronize()
+
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    edges = 0
+    kern_ms = 0.0
+    levels = []
+    ev0.record(stream)
+    for i in range(args.steps):
+        st = ctx.run(gr, int(sources[i % len(sources)]), opts)
+        edges += st["edges"]
+        kern_ms += st["ms"]
+        levels.append(st["levels"])
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    total_ms = ev0.elapsed_time(ev1)
+    gteps = (edges / 2) / (total_ms * 1e-3) / 1e9
+
+    # roofline: algorithmic bytes of one BFS = 4(n+1) + 4 m_s + 12 n
+    n = g.n
+    b_alg = args.steps * (4 * (n + 1) + 12 * n) + 4 * edges
+    achieved = b_alg / (kern_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": gteps,
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 depth + u64 path count",
+        "data": "synthetic",
+        "config": {
+            "workload": f"single-source BFS with exact sigma, R-MAT scale {args.scale} "
+                        f"ef16 a.57 b.19 c.19 symmetrized relabelled seed 2",
+            "n": n, "nz": g.nz, "sources": len(sources), "source_seed": 2,
+            "mode": args.mode, "mean_levels": float(np.mean(levels)),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "whole BFS (all level kernels of one source)",
+        },
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(g, sources, args.cpu_budget)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

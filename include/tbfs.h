@@ -1,0 +1,135 @@
+/*
+ * tbfs.h -- extended C API of libturbobfs (MI355X / gfx950).
+ *
+ * The reference (pcdslab/TurboBFS) has no library API beyond its per-variant
+ * GPU entry points (bfstdcsc.h, bfsbucsc.h, bfstdbucsc.h, bfstdcooc.h, also
+ * exported by this library).  Those re-upload the graph on every call.  This
+ * header adds what a resident-graph caller needs: graphs kept in HBM, a
+ * reusable BFS context bound to one HIP stream, a choice of level kernel and
+ * level loop, and exact 64-bit path counts with an overflow flag.
+ *
+ * Conventions
+ *   - CSC in-lists: column v of (CP, IC) lists every u with an edge u -> v,
+ *     as in the reference (TurboBFS_CSC_TD/bfstdcsc.h:30-34).
+ *   - S[v] = depth of v, -1 when v is not reached, S[root] = 0.
+ *   - sigma[v] = number of shortest root->v paths modulo 2^64; ovf[v] = 1 when
+ *     the exact count is >= 2^64 (sigma then holds it modulo 2^64).
+ *   - Every function returns TBFS_OK (0) or a nonzero status; nothing exits
+ *     the process.
+ */
+#ifndef TBFS_H
+#define TBFS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TBFS_OK 0
+#define TBFS_ERR_INVALID 1     /* bad argument or malformed graph */
+#define TBFS_ERR_HIP 2         /* a HIP runtime call failed */
+#define TBFS_ERR_NOMEM 3       /* device or host allocation failed */
+#define TBFS_ERR_UNSUPPORTED 4 /* mode not available for this graph */
+#define TBFS_SIGMA_INEXACT 5   /* (reference entry points only) a path count
+                                  exceeds what the reference's output type
+                                  holds exactly: float above 2^24, int32 above
+                                  2^31-1; the outputs are still written */
+
+/* graph flags */
+#define TBFS_GRAPH_SYMMETRIC 1u /* A == A^T: out-lists alias the in-lists */
+#define TBFS_GRAPH_DEVICE 2u    /* input arrays are device pointers */
+
+/* level kernel selection (tbfs_opts.mode) */
+#define TBFS_MODE_AUTO 0    /* direction-optimizing: push <-> pull */
+#define TBFS_MODE_PULL_SC 1 /* pull, one lane per column, every level */
+#define TBFS_MODE_PULL_WA 2 /* pull, one wavefront per column, every level */
+#define TBFS_MODE_PUSH 3    /* top-down push from a queue, every level */
+#define TBFS_MODE_COOC 4    /* edge-parallel over the COOC nonzeros */
+
+/* pull kernel used by TBFS_MODE_AUTO for its bottom-up levels (tbfs_opts.pull) */
+#define TBFS_PULL_SC 0 /* one lane per column */
+#define TBFS_PULL_WA 1 /* one wavefront per column */
+
+/* level loop (tbfs_opts.loop) */
+#define TBFS_LOOP_HOST 0    /* host launches each level, reads its counters */
+
+typedef struct tbfs_graph tbfs_graph;
+typedef struct tbfs_ctx tbfs_ctx;
+
+typedef struct tbfs_opts {
+  int32_t mode;   /* TBFS_MODE_* */
+  int32_t loop;   /* TBFS_LOOP_* */
+  int32_t sigma;  /* 1: compute path counts, 0: depths only */
+  int32_t bbreak; /* depths only: stop a column scan at its first parent */
+  float alpha;    /* AUTO: switch push->pull when m_frontier*alpha > m_unexplored */
+  float beta;     /* AUTO: switch pull->push when n_frontier*beta < n */
+  int32_t block;  /* threads per workgroup, multiple of 64 (0: default) */
+  int32_t grid;   /* workgroups (0: default) */
+  int32_t pull;   /* TBFS_PULL_*: AUTO's bottom-up kernel */
+  int32_t reserved[7];
+} tbfs_opts;
+
+typedef struct tbfs_stats {
+  int32_t levels;      /* number of distinct depths (1 + eccentricity of root) */
+  int32_t push_levels; /* levels expanded top-down (push) */
+  int32_t pull_levels; /* levels expanded bottom-up (pull) or edge-parallel */
+  int32_t pad;
+  int64_t reached;     /* vertices with S >= 0 */
+  int64_t edges;       /* m_s: in-list entries of the reached vertices */
+  int64_t ovf64;       /* reached vertices whose count is >= 2^64 */
+  double ms;           /* device time of the BFS, first to last kernel */
+} tbfs_stats;
+
+/* Fill *o with the defaults: AUTO, host loop, sigma on, alpha 4, beta 24. */
+void tbfs_opts_default(tbfs_opts *o);
+
+/* Human-readable name of a status code. */
+const char *tbfs_status_str(int status);
+
+/* Upload a CSC graph (host arrays unless TBFS_GRAPH_DEVICE).  The CSC is
+ * validated (CP[0] = 0, CP non-decreasing, CP[n] = nz, 0 <= IC < n).  Unless
+ * TBFS_GRAPH_SYMMETRIC is set, the out-list transpose is built on the device
+ * (stable radix sort, so each out-list is in increasing column order). */
+int tbfs_graph_create(const int32_t *CP, const int32_t *IC, int32_t n,
+                      int32_t nz, uint32_t flags, tbfs_graph **out);
+
+/* Upload a COOC graph: edge I[k] -> J[k], k < nz (TurboBFS_COOC_TD/bfstdcooc.h:29-32).
+ * When J is non-decreasing the CSC column pointers are derived too, so every
+ * mode is available; otherwise only TBFS_MODE_COOC is. */
+int tbfs_graph_create_cooc(const int32_t *I, const int32_t *J, int32_t n,
+                           int32_t nz, uint32_t flags, tbfs_graph **out);
+
+int tbfs_graph_destroy(tbfs_graph *g);
+
+/* n, nz and flags of a resident graph (any pointer may be NULL). */
+int tbfs_graph_info(const tbfs_graph *g, int32_t *n, int32_t *nz,
+                    uint32_t *flags);
+
+/* A BFS context: one HIP stream plus per-BFS state for graphs of up to
+ * n_cap vertices.  One context runs one BFS at a time. */
+int tbfs_ctx_create(int32_t n_cap, tbfs_ctx **out);
+int tbfs_ctx_destroy(tbfs_ctx *c);
+
+/* The context's hipStream_t (as void*), for callers that time or order work. */
+void *tbfs_ctx_stream(tbfs_ctx *c);
+
+/* Run one BFS from root on g.  Results stay in the context's device buffers
+ * until the next BFS; st (may be NULL) receives the counters. */
+int tbfs_bfs(tbfs_ctx *c, const tbfs_graph *g, int32_t root,
+             const tbfs_opts *o, tbfs_stats *st);
+
+/* Copy the last BFS's results to host memory (any pointer may be NULL;
+ * n must equal that graph's vertex count). */
+int tbfs_ctx_read(tbfs_ctx *c, int32_t n, int32_t *S, uint64_t *sigma,
+                  uint8_t *ovf);
+
+/* Device pointers of the context's S / sigma / ovf buffers. */
+int tbfs_ctx_device_ptrs(tbfs_ctx *c, int32_t **S, uint64_t **sigma,
+                         uint8_t **ovf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TBFS_H */

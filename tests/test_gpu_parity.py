@@ -1,0 +1,179 @@
+"""GPU parity: every entry point through the C-ABI against the oracle.
+
+S must match bit-exactly; sigma must match exactly (uint64 value and overflow
+flag through the extended API; at the drop-in boundary, the reference's
+float/int output type with the inexact flag raised exactly when a count
+leaves that type's exact range).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import turbobfs_amd as tb
+from turbobfs_amd import graphs, mtx
+from tests.conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+F32_EXACT = 1 << 24
+I32_MAX = 2**31 - 1
+
+
+def _er4096():
+    return graphs.erdos_renyi(4096, 8.0 / 4095.0, seed=1)
+
+
+def _dropin_all(g, r, directed_ok=True):
+    """Run the seven drop-in entry points; return dict name -> outputs."""
+    n, nz = g.n, g.nz
+    out = {}
+    for name in ("bfs_gpu_td_csc_sc", "bfs_gpu_td_csc_wa"):
+        S = np.full(n, 7, np.int32)
+        sig = np.full(n, -1, np.float32)
+        rc = getattr(tb, name)(g.IC, g.CP, S, sig, r, nz, n, 1)
+        out[name] = (rc, S, sig)
+    for name in ("bfs_gpu_bu_csc_sc", "bfs_gpu_bu_csc_wa"):
+        for bb in (0, 1):
+            S = np.full(n, 7, np.int32)
+            rc = getattr(tb, name)(g.IC, g.CP, S, r, nz, n, 1, bb)
+            out[f"{name}[bbreak={bb}]"] = (rc, S, None)
+    for name in ("bfs_gpu_tdbu_csc_sc", "bfs_gpu_tdbu_csc_wa"):
+        S = np.full(n, 7, np.int32)
+        sig = np.full(n, 7, np.int32)
+        rc = getattr(tb, name)(g.IC, g.CP, S, sig, r, nz, n, 1, 1)
+        out[name] = (rc, S, sig)
+    I, J = graphs.csc_to_cooc(g)
+    sig = np.full(n, -1, np.float32)
+    rc = tb.bfs_gpu_td_cooc_sc(I, J, sig, r, nz, n, 1)
+    out["bfs_gpu_td_cooc_sc"] = (rc, None, sig)
+    return out
+
+
+def _check_dropin(g, r):
+    S0, sig0, ovf0, _ = oracle.seq_td_csc(g.CP, g.IC, r, s_unreached=0)
+    S1, sig1, ovf1, _ = oracle.seq_td_csc(g.CP, g.IC, r, s_unreached=-1)
+    exp_f32 = np.where(ovf1 == 1, np.float32(np.inf), sig1.astype(np.float32))
+    f32_inexact = bool((ovf1 == 1).any() or (sig1 > F32_EXACT).any())
+    i32_inexact = bool((ovf1 == 1).any() or (sig1 > I32_MAX).any())
+    exp_i32 = (sig1 & np.uint64(0xFFFFFFFF)).astype(np.uint32).view(np.int32)
+    res = _dropin_all(g, r)
+    for name, (rc, S, sig) in res.items():
+        if name.startswith("bfs_gpu_td_csc"):
+            assert rc == (5 if f32_inexact else 0), name
+            np.testing.assert_array_equal(S, S0, err_msg=name)
+            np.testing.assert_array_equal(sig, exp_f32, err_msg=name)
+        elif name.startswith("bfs_gpu_bu_csc"):
+            assert rc == 0, name
+            np.testing.assert_array_equal(S, S1, err_msg=name)
+        elif name.startswith("bfs_gpu_tdbu_csc"):
+            assert rc == (5 if i32_inexact else 0), name
+            np.testing.assert_array_equal(S, S1, err_msg=name)
+            np.testing.assert_array_equal(sig, exp_i32, err_msg=name)
+        else:  # cooc
+            assert rc == (5 if f32_inexact else 0), name
+            np.testing.assert_array_equal(sig, exp_f32, err_msg=name)
+
+
+def test_er4096_dropin(gpu):
+    """Config: undirected G(4096, 8/4095) seed 1, source 0, all seven entry points."""
+    _check_dropin(_er4096(), 0)
+
+
+@pytest.mark.parametrize("fname,r", [
+    ("mycielskian4.mtx", 0), ("Tina_AskCal.mtx", 10), ("cage4.mtx", 0),
+    ("Ragusa18.mtx", 0), ("luxembourg_osm.mtx.gz", 0),
+])
+def test_reference_fixtures_dropin(gpu, fname, r):
+    g = mtx.read_csc(golden(fname))
+    _check_dropin(g, r)
+
+
+MODES = [
+    ("auto_sc", dict(mode=tb._lib.MODE_AUTO, pull=tb._lib.PULL_SC)),
+    ("auto_wa", dict(mode=tb._lib.MODE_AUTO, pull=tb._lib.PULL_WA)),
+    ("pull_sc", dict(mode=tb._lib.MODE_PULL_SC)),
+    ("pull_wa", dict(mode=tb._lib.MODE_PULL_WA)),
+    ("push", dict(mode=tb._lib.MODE_PUSH)),
+    ("cooc", dict(mode=tb._lib.MODE_COOC)),
+]
+
+
+def _check_modes(g, roots, symmetric, modes=MODES):
+    gr = tb.Graph(g.CP, g.IC, symmetric=symmetric)
+    ctx = tb.Context(g.n)
+    for r in roots:
+        S, sig, ovf, lv = oracle.textbook(g.CP, g.IC, int(r))
+        for name, kw in modes:
+            res = ctx.bfs(gr, int(r), tb.default_opts(**kw))
+            np.testing.assert_array_equal(res.S, S, err_msg=f"{name} r={r} S")
+            np.testing.assert_array_equal(res.sigma, sig, err_msg=f"{name} r={r} sigma")
+            np.testing.assert_array_equal(res.ovf, ovf, err_msg=f"{name} r={r} ovf")
+            assert res.stats["levels"] == lv, name
+            assert res.stats["reached"] == int((S >= 0).sum()), name
+            assert res.stats["edges"] == int(np.diff(g.CP)[S >= 0].sum()), name
+
+
+def test_rmat14_all_modes(gpu):
+    g = graphs.rmat(14, 16, seed=2)
+    roots = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 4, seed=2)
+    _check_modes(g, roots, symmetric=True)
+    _check_modes(g, roots[:1], symmetric=False)
+
+
+def test_grid_overflow_all_modes(gpu):
+    """Unrelabelled 48x48 grid from a corner: sigma = C(x+y, x) passes 2^64
+    (x+y >= 68), so the overflow flag and the wrapped value are exercised."""
+    g = graphs.grid(48, 48, 0.0, seed=0, relabel=False)
+    S, sig, ovf, _ = oracle.textbook(g.CP, g.IC, 0)
+    assert ovf.any() and not ovf.all()
+    _check_modes(g, [0, 48 * 48 - 1], symmetric=True)
+
+
+def test_perturbed_grid_modes(gpu):
+    g = graphs.grid(96, 64, 0.1, seed=3, relabel=True)
+    _check_modes(g, [0, 17, 5000], symmetric=True)
+
+
+def test_directed_reference_fixture_modes(gpu):
+    g = mtx.read_csc(golden("Tina_AskCal.mtx"))
+    _check_modes(g, range(g.n), symmetric=False)
+
+
+def test_edge_cases(gpu):
+    # single vertex, no edges
+    g = graphs.CSC(np.array([0, 0], np.int32), np.zeros(0, np.int32))
+    _check_dropin(g, 0)
+    # isolated root in a larger graph, plus a duplicate entry and a self-loop
+    CP = np.array([0, 0, 3, 5, 6], np.int32)
+    IC = np.array([2, 2, 1, 1, 3, 2], np.int32)  # col1: 2,2,1 ; col2: 1,3 ; col3: 2
+    g = graphs.CSC(CP, IC)
+    for r in range(4):
+        _check_dropin(g, r)
+        _check_modes(g, [r], symmetric=False)
+
+
+def test_invalid_inputs_rejected(gpu):
+    g = _er4096()
+    S = np.zeros(g.n, np.int32)
+    sig = np.zeros(g.n, np.float32)
+    assert tb.bfs_gpu_td_csc_sc(g.IC, g.CP, S, sig, g.n, g.nz, g.n, 1) == 1
+    assert tb.bfs_gpu_td_csc_sc(g.IC, g.CP, S, sig, -1, g.nz, g.n, 1) == 1
+    assert tb.bfs_gpu_td_csc_sc(g.IC, g.CP, S, sig, 0, g.nz, g.n, 0) == 1
+    bad = g.CP.copy()
+    bad[5] = bad[6] + 1  # not monotone
+    assert tb.bfs_gpu_td_csc_sc(g.IC, bad, S, sig, 0, g.nz, g.n, 1) == 1
+    badic = g.IC.copy()
+    badic[3] = g.n
+    assert tb.bfs_gpu_bu_csc_sc(badic, g.CP, S, 0, g.nz, g.n, 1, 1) == 1
+
+
+def test_repetition_same_result(gpu):
+    g = _er4096()
+    S1 = np.zeros(g.n, np.int32)
+    s1 = np.zeros(g.n, np.float32)
+    S3 = np.zeros(g.n, np.int32)
+    s3 = np.zeros(g.n, np.float32)
+    assert tb.bfs_gpu_td_csc_wa(g.IC, g.CP, S1, s1, 5, g.nz, g.n, 1) == 0
+    assert tb.bfs_gpu_td_csc_wa(g.IC, g.CP, S3, s3, 5, g.nz, g.n, 3) == 0
+    np.testing.assert_array_equal(S1, S3)
+    np.testing.assert_array_equal(s1, s3)

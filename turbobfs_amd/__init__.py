@@ -1,0 +1,30 @@
+"""turbobfs_amd -- MI355X-native (gfx950) GPU BFS behind TurboBFS's C host API.
+
+The compute lives in libturbobfs.so (hand-written HIP kernels + a C-ABI,
+include/*.h).  This package is the Python side: a ctypes binding that mirrors
+the reference's entry points, the resident-graph API, and the synthetic
+workload generators.
+"""
+from . import _lib, graphs  # noqa: F401
+from ._lib import LIB_PATH, TbfsError, load  # noqa: F401
+from .bfs import (  # noqa: F401
+    Context,
+    Graph,
+    Result,
+    bfs_gpu_bu_csc_sc,
+    bfs_gpu_bu_csc_wa,
+    bfs_gpu_td_cooc_sc,
+    bfs_gpu_td_csc_sc,
+    bfs_gpu_td_csc_wa,
+    bfs_gpu_tdbu_csc_sc,
+    bfs_gpu_tdbu_csc_wa,
+    default_opts,
+)
+
+__all__ = [
+    "Context", "Graph", "Result", "default_opts", "graphs", "load", "LIB_PATH",
+    "TbfsError",
+    "bfs_gpu_td_csc_sc", "bfs_gpu_td_csc_wa", "bfs_gpu_bu_csc_sc",
+    "bfs_gpu_bu_csc_wa", "bfs_gpu_tdbu_csc_sc", "bfs_gpu_tdbu_csc_wa",
+    "bfs_gpu_td_cooc_sc",
+]

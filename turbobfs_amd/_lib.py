@@ -1,0 +1,138 @@
+"""ctypes binding of libturbobfs.so (the C-ABI declared in include/*.h).
+
+The library is loaded from this package directory only.  There is no CPU
+fallback: if the shared object is missing, importing the compute entry points
+raises, so a run without the HIP library fails loudly instead of silently
+computing something else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libturbobfs.so")
+
+# status codes (include/tbfs.h)
+TBFS_OK = 0
+TBFS_ERR_INVALID = 1
+TBFS_ERR_HIP = 2
+TBFS_ERR_NOMEM = 3
+TBFS_ERR_UNSUPPORTED = 4
+TBFS_SIGMA_INEXACT = 5
+
+TBFS_GRAPH_SYMMETRIC = 1
+TBFS_GRAPH_DEVICE = 2
+
+MODE_AUTO = 0
+MODE_PULL_SC = 1
+MODE_PULL_WA = 2
+MODE_PUSH = 3
+MODE_COOC = 4
+
+PULL_SC = 0
+PULL_WA = 1
+
+LOOP_HOST = 0
+
+_i32p = C.POINTER(C.c_int32)
+_u64p = C.POINTER(C.c_uint64)
+_u8p = C.POINTER(C.c_uint8)
+_f32p = C.POINTER(C.c_float)
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("loop", C.c_int32),
+        ("sigma", C.c_int32),
+        ("bbreak", C.c_int32),
+        ("alpha", C.c_float),
+        ("beta", C.c_float),
+        ("block", C.c_int32),
+        ("grid", C.c_int32),
+        ("pull", C.c_int32),
+        ("reserved", C.c_int32 * 7),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int32),
+        ("push_levels", C.c_int32),
+        ("pull_levels", C.c_int32),
+        ("pad", C.c_int32),
+        ("reached", C.c_int64),
+        ("edges", C.c_int64),
+        ("ovf64", C.c_int64),
+        ("ms", C.c_double),
+    ]
+
+
+# name -> (restype, argtypes); every symbol the public headers declare
+SIGNATURES = {
+    # drop-in surface (bfstdcsc.h, bfsbucsc.h, bfstdbucsc.h, bfstdcooc.h)
+    "bfs_gpu_td_csc_sc": (C.c_int, [_i32p, _i32p, _i32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bfs_gpu_td_csc_wa": (C.c_int, [_i32p, _i32p, _i32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bfs_gpu_bu_csc_sc": (C.c_int, [_i32p, _i32p, _i32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bfs_gpu_bu_csc_wa": (C.c_int, [_i32p, _i32p, _i32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bfs_gpu_tdbu_csc_sc": (C.c_int, [_i32p, _i32p, _i32p, _i32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bfs_gpu_tdbu_csc_wa": (C.c_int, [_i32p, _i32p, _i32p, _i32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bfs_gpu_td_cooc_sc": (C.c_int, [_i32p, _i32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    # extended API (tbfs.h)
+    "tbfs_opts_default": (None, [C.POINTER(Opts)]),
+    "tbfs_status_str": (C.c_char_p, [C.c_int]),
+    "tbfs_graph_create": (C.c_int, [_i32p, _i32p, C.c_int32, C.c_int32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "tbfs_graph_create_cooc": (C.c_int, [_i32p, _i32p, C.c_int32, C.c_int32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "tbfs_graph_destroy": (C.c_int, [C.c_void_p]),
+    "tbfs_graph_info": (C.c_int, [C.c_void_p, _i32p, _i32p, C.POINTER(C.c_uint32)]),
+    "tbfs_ctx_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
+    "tbfs_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "tbfs_ctx_stream": (C.c_void_p, [C.c_void_p]),
+    "tbfs_bfs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(Opts), C.POINTER(Stats)]),
+    "tbfs_ctx_read": (C.c_int, [C.c_void_p, C.c_int32, _i32p, _u64p, _u8p]),
+    "tbfs_ctx_device_ptrs": (C.c_int, [C.c_void_p, C.POINTER(_i32p), C.POINTER(_u64p), C.POINTER(_u8p)]),
+    # generators (tbfs_gen.h)
+    "tbfs_gen_er": (C.c_int, [C.c_int32, C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "tbfs_gen_rmat": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "tbfs_gen_grid": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "tbfs_gen_ws": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "tbfs_csc_host_info": (C.c_int, [C.c_void_p, _i32p, _i32p]),
+    "tbfs_csc_host_copy": (C.c_int, [C.c_void_p, _i32p, _i32p]),
+    "tbfs_csc_host_free": (None, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libturbobfs.so (once) and attach the C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+            "(or make -C turbobfs_amd/csrc); there is no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def status_str(rc: int) -> str:
+    return load().tbfs_status_str(rc).decode()
+
+
+class TbfsError(RuntimeError):
+    def __init__(self, what: str, rc: int):
+        super().__init__(f"{what}: {status_str(rc)} ({rc})")
+        self.rc = rc
+
+
+def check(rc: int, what: str) -> None:
+    if rc != TBFS_OK:
+        raise TbfsError(what, rc)

@@ -1,0 +1,195 @@
+"""Python mirror of the library's two surfaces.
+
+1. The reference's GPU entry points, same names and argument meaning
+   (TurboBFS_CSC_TD/bfstdcsc.h:54-66, TurboBFS_CSC_BU/bfsbucsc.h:55-67,
+   TurboBFS_CSC_TDBU/bfstdbucsc.h:56-68, TurboBFS_COOC_TD/bfstdcooc.h:48-49):
+   caller-provided numpy output arrays are filled in place and the C status
+   is returned.  They call straight into libturbobfs.so.
+2. The resident-graph API of include/tbfs.h: Graph (CSC/COOC in HBM) and
+   Context (stream + per-BFS buffers).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import Opts, Stats, check
+
+_i32p = C.POINTER(C.c_int32)
+_u64p = C.POINTER(C.c_uint64)
+_u8p = C.POINTER(C.c_uint8)
+_f32p = C.POINTER(C.c_float)
+
+
+def _ptr(a: np.ndarray, ptype):
+    return a.ctypes.data_as(ptype)
+
+
+def _req(a, dtype, n=None, name="array"):
+    if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags.c_contiguous:
+        raise TypeError(f"{name} must be a C-contiguous numpy {np.dtype(dtype).name} array")
+    if n is not None and a.shape[0] < n:
+        raise ValueError(f"{name} has {a.shape[0]} entries, needs {n}")
+    return a
+
+
+# ---------------------------------------------------------------- drop-in ---
+def bfs_gpu_td_csc_sc(IC_h, CP_h, S_h, sigma_h, r, nz, n, repetition) -> int:
+    lib = _lib.load()
+    return lib.bfs_gpu_td_csc_sc(
+        _ptr(_req(IC_h, np.int32, nz, "IC_h"), _i32p), _ptr(_req(CP_h, np.int32, n + 1, "CP_h"), _i32p),
+        _ptr(_req(S_h, np.int32, n, "S_h"), _i32p), _ptr(_req(sigma_h, np.float32, n, "sigma_h"), _f32p),
+        r, nz, n, repetition)
+
+
+def bfs_gpu_td_csc_wa(IC_h, CP_h, S_h, sigma_h, r, nz, n, repetition) -> int:
+    lib = _lib.load()
+    return lib.bfs_gpu_td_csc_wa(
+        _ptr(_req(IC_h, np.int32, nz, "IC_h"), _i32p), _ptr(_req(CP_h, np.int32, n + 1, "CP_h"), _i32p),
+        _ptr(_req(S_h, np.int32, n, "S_h"), _i32p), _ptr(_req(sigma_h, np.float32, n, "sigma_h"), _f32p),
+        r, nz, n, repetition)
+
+
+def bfs_gpu_bu_csc_sc(I_h, CP_h, S_h, r, nz, n, repetition, bbreak) -> int:
+    lib = _lib.load()
+    return lib.bfs_gpu_bu_csc_sc(
+        _ptr(_req(I_h, np.int32, nz, "I_h"), _i32p), _ptr(_req(CP_h, np.int32, n + 1, "CP_h"), _i32p),
+        _ptr(_req(S_h, np.int32, n, "S_h"), _i32p), r, nz, n, repetition, bbreak)
+
+
+def bfs_gpu_bu_csc_wa(I_h, CP_h, S_h, r, nz, n, repetition, bbreak) -> int:
+    lib = _lib.load()
+    return lib.bfs_gpu_bu_csc_wa(
+        _ptr(_req(I_h, np.int32, nz, "I_h"), _i32p), _ptr(_req(CP_h, np.int32, n + 1, "CP_h"), _i32p),
+        _ptr(_req(S_h, np.int32, n, "S_h"), _i32p), r, nz, n, repetition, bbreak)
+
+
+def bfs_gpu_tdbu_csc_sc(I_h, CP_h, S_h, sigma_h, r, nz, n, repetition, bbreak) -> int:
+    lib = _lib.load()
+    return lib.bfs_gpu_tdbu_csc_sc(
+        _ptr(_req(I_h, np.int32, nz, "I_h"), _i32p), _ptr(_req(CP_h, np.int32, n + 1, "CP_h"), _i32p),
+        _ptr(_req(S_h, np.int32, n, "S_h"), _i32p), _ptr(_req(sigma_h, np.int32, n, "sigma_h"), _i32p),
+        r, nz, n, repetition, bbreak)
+
+
+def bfs_gpu_tdbu_csc_wa(IC_h, CP_h, S_h, sigma_h, r, nz, n, repetition, bbreak) -> int:
+    lib = _lib.load()
+    return lib.bfs_gpu_tdbu_csc_wa(
+        _ptr(_req(IC_h, np.int32, nz, "IC_h"), _i32p), _ptr(_req(CP_h, np.int32, n + 1, "CP_h"), _i32p),
+        _ptr(_req(S_h, np.int32, n, "S_h"), _i32p), _ptr(_req(sigma_h, np.int32, n, "sigma_h"), _i32p),
+        r, nz, n, repetition, bbreak)
+
+
+def bfs_gpu_td_cooc_sc(I_h, J_h, sigma_h, r, nz, n, repetition) -> int:
+    lib = _lib.load()
+    return lib.bfs_gpu_td_cooc_sc(
+        _ptr(_req(I_h, np.int32, nz, "I_h"), _i32p), _ptr(_req(J_h, np.int32, nz, "J_h"), _i32p),
+        _ptr(_req(sigma_h, np.float32, n, "sigma_h"), _f32p), r, nz, n, repetition)
+
+
+# ------------------------------------------------------- resident graphs ---
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    _lib.load().tbfs_opts_default(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(f"tbfs_opts has no field {k}")
+        setattr(o, k, v)
+    return o
+
+
+class Graph:
+    """A CSC (or COOC) graph resident in HBM."""
+
+    def __init__(self, CP=None, IC=None, *, I=None, J=None, n=None,
+                 symmetric: bool = False):
+        lib = _lib.load()
+        self._h = C.c_void_p()
+        flags = _lib.TBFS_GRAPH_SYMMETRIC if symmetric else 0
+        if CP is not None:
+            CP = np.ascontiguousarray(CP, dtype=np.int32)
+            IC = np.ascontiguousarray(IC, dtype=np.int32)
+            self.n = int(CP.shape[0] - 1)
+            self.nz = int(IC.shape[0])
+            check(lib.tbfs_graph_create(_ptr(CP, _i32p), _ptr(IC, _i32p), self.n,
+                                        self.nz, flags, C.byref(self._h)),
+                  "tbfs_graph_create")
+        else:
+            I = np.ascontiguousarray(I, dtype=np.int32)
+            J = np.ascontiguousarray(J, dtype=np.int32)
+            self.n = int(n)
+            self.nz = int(I.shape[0])
+            check(lib.tbfs_graph_create_cooc(_ptr(I, _i32p), _ptr(J, _i32p), self.n,
+                                             self.nz, flags, C.byref(self._h)),
+                  "tbfs_graph_create_cooc")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _lib.load().tbfs_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Result:
+    S: np.ndarray        # int32, -1 = unreached
+    sigma: np.ndarray    # uint64, count mod 2^64
+    ovf: np.ndarray      # uint8, count >= 2^64
+    stats: dict
+
+
+class Context:
+    """Stream + per-BFS device buffers for graphs of up to n_cap vertices."""
+
+    def __init__(self, n_cap: int):
+        self._h = C.c_void_p()
+        check(_lib.load().tbfs_ctx_create(int(n_cap), C.byref(self._h)), "tbfs_ctx_create")
+        self.n_cap = int(n_cap)
+
+    @property
+    def stream(self) -> int:
+        return int(_lib.load().tbfs_ctx_stream(self._h) or 0)
+
+    def run(self, g: Graph, root: int, opts: Opts | None = None) -> dict:
+        st = Stats()
+        o = opts if opts is not None else default_opts()
+        check(_lib.load().tbfs_bfs(self._h, g.handle, int(root), C.byref(o), C.byref(st)),
+              "tbfs_bfs")
+        return {f: getattr(st, f) for f, _ in Stats._fields_ if f != "pad"}
+
+    def read(self, n: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        S = np.empty(n, dtype=np.int32)
+        sigma = np.empty(n, dtype=np.uint64)
+        ovf = np.empty(n, dtype=np.uint8)
+        check(_lib.load().tbfs_ctx_read(self._h, n, _ptr(S, _i32p), _ptr(sigma, _u64p),
+                                        _ptr(ovf, _u8p)), "tbfs_ctx_read")
+        return S, sigma, ovf
+
+    def bfs(self, g: Graph, root: int, opts: Opts | None = None) -> Result:
+        stats = self.run(g, root, opts)
+        S, sigma, ovf = self.read(g.n)
+        stats["ovf64"] = int(ovf.sum())
+        return Result(S, sigma, ovf, stats)
+
+    def close(self):
+        if self._h:
+            _lib.load().tbfs_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,231 @@
+// tbfs_dropin.hip -- the reference's seven GPU entry points, same C
+// signatures, on top of the resident-graph engine.
+//
+// Reference contract (paths relative to the reference root): caller-owned
+// host arrays; the callee uploads the graph, runs the BFS `repetition` times,
+// copies the last S / sigma back and frees every device buffer
+// (TurboBFS_CSC_TD/bfsgputdcsc_sc.cu:47-149).  Kept here, with exact 64-bit
+// counts converted once to the reference's output type, validated input, and
+// nonzero status codes instead of exit().
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "bfsbucsc.h"
+#include "bfstdbucsc.h"
+#include "bfstdcooc.h"
+#include "bfstdcsc.h"
+#include "tbfs.h"
+#include "tbfs_engine.h"
+
+using namespace tbfs;
+
+namespace {
+
+bool verbose() {
+  const char* v = std::getenv("TBFS_VERBOSE");
+  return v && v[0] && v[0] != '0';
+}
+
+struct DropinResult {
+  std::vector<int32_t> S;
+  std::vector<uint64_t> sigma;
+  std::vector<uint8_t> ovf;
+  tbfs_stats st{};
+  double total_ms = 0.0;
+};
+
+// Upload, run `repetition` BFS from r, read back.  `cooc` selects the COOC
+// upload (A = I, B = J) instead of CSC (A = CP, B = IC).
+int run_dropin(const char* name, const int* A, const int* B, int nz, int n,
+               int r, int repetition, const tbfs_opts& o, bool cooc,
+               bool want_sigma, DropinResult* res) {
+  if (n <= 0 || nz < 0 || r < 0 || r >= n || repetition < 1 || !A ||
+      (nz > 0 && !B)) {
+    std::fprintf(stderr,
+                 "turbobfs: %s: invalid arguments (n=%d nz=%d r=%d "
+                 "repetition=%d)\n",
+                 name, n, nz, r, repetition);
+    return TBFS_ERR_INVALID;
+  }
+  tbfs_graph* g = nullptr;
+  tbfs_ctx* c = nullptr;
+  int rc = cooc ? graph_create_cooc(A, B, n, nz, 0u, &g)
+                : graph_create_csc(A, B, n, nz, 0u, &g);
+  if (rc) {
+    std::fprintf(stderr, "turbobfs: %s: graph upload failed: %s\n", name,
+                 tbfs_status_str(rc));
+    return rc;
+  }
+  rc = ctx_create(n, &c);
+  if (rc) {
+    graph_free(g);
+    return rc;
+  }
+  for (int i = 0; i < repetition && !rc; ++i) {
+    rc = run_bfs(c, g, r, o, &res->st);
+    res->total_ms += res->st.ms;
+  }
+  if (!rc) {
+    res->S.resize((size_t)n);
+    if (want_sigma) {
+      res->sigma.resize((size_t)n);
+      res->ovf.resize((size_t)n);
+    }
+    rc = ctx_read(c, n, res->S.data(), want_sigma ? res->sigma.data() : nullptr,
+                  want_sigma ? res->ovf.data() : nullptr);
+  }
+  ctx_free(c);
+  graph_free(g);
+  if (rc) {
+    std::fprintf(stderr, "turbobfs: %s: BFS failed: %s\n", name,
+                 tbfs_status_str(rc));
+    return rc;
+  }
+  if (verbose())
+    std::printf("\n%s::d = %d,r = %d,t_sum=%lfms \n%s::average time BFS = %lfms \n",
+                name, res->st.levels, r, res->total_ms, name,
+                res->total_ms / repetition);
+  return TBFS_OK;
+}
+
+// Exact counts -> float; counts above 2^24 (or >= 2^64) are inexact in float.
+int to_float(const char* name, const DropinResult& res, float* out, int n) {
+  long long inexact = 0;
+  for (int i = 0; i < n; ++i) {
+    if (res.ovf[i]) {
+      out[i] = INFINITY;
+      ++inexact;
+    } else {
+      out[i] = (float)res.sigma[i];
+      if (res.sigma[i] > (1ull << 24)) ++inexact;
+    }
+  }
+  if (inexact) {
+    std::fprintf(stderr,
+                 "turbobfs: %s: %lld path counts exceed 2^24 and are not "
+                 "exact in float\n",
+                 name, inexact);
+    return TBFS_SIGMA_INEXACT;
+  }
+  return TBFS_OK;
+}
+
+// Exact counts -> int32 modulo 2^32 (the reference's int arithmetic).
+int to_int(const char* name, const DropinResult& res, int* out, int n) {
+  long long inexact = 0;
+  for (int i = 0; i < n; ++i) {
+    out[i] = (int)(uint32_t)res.sigma[i];
+    if (res.ovf[i] || res.sigma[i] > 0x7fffffffull) ++inexact;
+  }
+  if (inexact) {
+    std::fprintf(stderr,
+                 "turbobfs: %s: %lld path counts exceed 2^31-1; written "
+                 "modulo 2^32\n",
+                 name, inexact);
+    return TBFS_SIGMA_INEXACT;
+  }
+  return TBFS_OK;
+}
+
+int td_csc(const char* name, int* IC_h, int* CP_h, int* S_h, float* sigma_h,
+           int r, int nz, int n, int repetition, int mode) {
+  if (!S_h || !sigma_h) return TBFS_ERR_INVALID;
+  tbfs_opts o;
+  tbfs_opts_default(&o);
+  o.mode = mode;
+  DropinResult res;
+  int rc = run_dropin(name, CP_h, IC_h, nz, n, r, repetition, o, false, true, &res);
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) S_h[i] = res.S[i] < 0 ? 0 : res.S[i];
+  return to_float(name, res, sigma_h, n);
+}
+
+int bu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int r, int nz,
+           int n, int repetition, int bbreak, int mode) {
+  if (!S_h) return TBFS_ERR_INVALID;
+  tbfs_opts o;
+  tbfs_opts_default(&o);
+  o.mode = mode;
+  o.sigma = 0;
+  o.bbreak = bbreak ? 1 : 0;
+  DropinResult res;
+  int rc = run_dropin(name, CP_h, I_h, nz, n, r, repetition, o, false, false, &res);
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) S_h[i] = res.S[i];
+  return TBFS_OK;
+}
+
+int tdbu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int* sigma_h,
+             int r, int nz, int n, int repetition, int pull) {
+  if (!S_h || !sigma_h) return TBFS_ERR_INVALID;
+  tbfs_opts o;
+  tbfs_opts_default(&o);
+  o.mode = TBFS_MODE_AUTO;
+  o.pull = pull;
+  DropinResult res;
+  int rc = run_dropin(name, CP_h, I_h, nz, n, r, repetition, o, false, true, &res);
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) S_h[i] = res.S[i];
+  return to_int(name, res, sigma_h, n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bfs_gpu_td_csc_sc(int* IC_h, int* CP_h, int* S_h, float* sigma_h, int r,
+                      int nz, int n, int repetition) {
+  return td_csc("bfsgputdcsc_sc", IC_h, CP_h, S_h, sigma_h, r, nz, n,
+                repetition, TBFS_MODE_PULL_SC);
+}
+
+int bfs_gpu_td_csc_wa(int* IC_h, int* CP_h, int* S_h, float* sigma_h, int r,
+                      int nz, int n, int repetition) {
+  return td_csc("bfsgputdcsc_wa", IC_h, CP_h, S_h, sigma_h, r, nz, n,
+                repetition, TBFS_MODE_PULL_WA);
+}
+
+int bfs_gpu_bu_csc_sc(int* I_h, int* CP_h, int* S_h, int r, int nz, int n,
+                      int repetition, int bbreak) {
+  return bu_csc("bfsgpubucsc_sc", I_h, CP_h, S_h, r, nz, n, repetition, bbreak,
+                TBFS_MODE_PULL_SC);
+}
+
+int bfs_gpu_bu_csc_wa(int* I_h, int* CP_h, int* S_h, int r, int nz, int n,
+                      int repetition, int bbreak) {
+  return bu_csc("bfsgpubucsc_wa", I_h, CP_h, S_h, r, nz, n, repetition, bbreak,
+                TBFS_MODE_PULL_WA);
+}
+
+int bfs_gpu_tdbu_csc_sc(int* I_h, int* CP_h, int* S_h, int* sigma_h, int r,
+                        int nz, int n, int repetition, int bbreak) {
+  (void)bbreak;  // counts need every parent: no early exit with sigma on
+  return tdbu_csc("bfsgputdbucsc_sc", I_h, CP_h, S_h, sigma_h, r, nz, n,
+                  repetition, TBFS_PULL_SC);
+}
+
+int bfs_gpu_tdbu_csc_wa(int* IC_h, int* CP_h, int* S_h, int* sigma_h, int r,
+                        int nz, int n, int repetition, int bbreak) {
+  (void)bbreak;
+  return tdbu_csc("bfsgputdbucsc_wa", IC_h, CP_h, S_h, sigma_h, r, nz, n,
+                  repetition, TBFS_PULL_WA);
+}
+
+int bfs_gpu_td_cooc_sc(int* I_h, int* J_h, float* sigma_h, int r, int nz,
+                       int n, int repetition) {
+  if (!sigma_h) return TBFS_ERR_INVALID;
+  tbfs_opts o;
+  tbfs_opts_default(&o);
+  o.mode = TBFS_MODE_COOC;
+  DropinResult res;
+  int rc = run_dropin("bfsgputdcooc_sc", I_h, J_h, nz, n, r, repetition, o,
+                      true, true, &res);
+  if (rc) return rc;
+  return to_float("bfsgputdcooc_sc", res, sigma_h, n);
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// tbfs_engine.h -- host-side objects behind the C API (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tbfs.h"
+#include "tbfs_internal.h"
+
+// A graph resident in HBM.
+struct tbfs_graph {
+  int32_t n = 0;
+  int32_t nz = 0;
+  uint32_t flags = 0;
+  int32_t* CP = nullptr;  // in-lists (CSC); null for an unsorted COOC graph
+  int32_t* IC = nullptr;
+  int32_t* RP = nullptr;  // out-lists; alias CP/IC when symmetric
+  int32_t* JR = nullptr;
+  int32_t* I = nullptr;   // COOC source of each entry (aliases IC)
+  int32_t* J = nullptr;   // COOC destination of each entry
+  bool J_owned = false;
+  bool has_csc = true;
+};
+
+// Per-BFS device state bound to one stream.
+struct tbfs_ctx {
+  int32_t n_cap = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int32_t* S = nullptr;
+  uint64_t* sigma = nullptr;
+  uint8_t* ovf = nullptr;
+  uint64_t* fb[3] = {nullptr, nullptr, nullptr};
+  int32_t* q[2] = {nullptr, nullptr};
+  tbfs::LevelCtr* ctr = nullptr;    // device
+  tbfs::LevelCtr* ctr_h = nullptr;  // pinned host mirror
+  int32_t last_n = 0;
+  uint32_t last_ovf_any = 0;
+};
+
+namespace tbfs {
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
+                     int32_t nz, uint32_t flags, tbfs_graph** out);
+int graph_create_cooc(const int32_t* I, const int32_t* J, int32_t n,
+                      int32_t nz, uint32_t flags, tbfs_graph** out);
+void graph_free(tbfs_graph* g);
+int ctx_create(int32_t n_cap, tbfs_ctx** out);
+void ctx_free(tbfs_ctx* c);
+int run_bfs(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
+            const tbfs_opts& o, tbfs_stats* st);
+int ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma,
+             uint8_t* ovf);
+}  // namespace tbfs

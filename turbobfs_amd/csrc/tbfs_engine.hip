@@ -1,0 +1,569 @@
+// tbfs_engine.hip -- host side of libturbobfs: resident graphs, BFS contexts,
+// the level loop and the extended C API declared in include/tbfs.h.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "tbfs.h"
+#include "tbfs_engine.h"
+#include "tbfs_internal.h"
+
+using namespace tbfs;
+
+// ----------------------------------------------------------------------------
+// error plumbing
+namespace tbfs {
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  std::fprintf(stderr, "turbobfs: %s failed: %s (%s:%d)\n", what,
+               hipGetErrorString(e), file, line);
+  return e == hipErrorOutOfMemory ? TBFS_ERR_NOMEM : TBFS_ERR_HIP;
+}
+
+}  // namespace tbfs
+
+#define HIPCHK(call)                                               \
+  do {                                                             \
+    hipError_t e_ = (call);                                        \
+    if (e_ != hipSuccess) return hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+namespace {
+
+template <typename T>
+int dev_alloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+  }
+  return TBFS_OK;
+}
+
+template <typename T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+int grid_for(int64_t items, int block, int cap) {
+  int64_t g = (items + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+// Host-side CSC validation (O(n + nz)).
+int validate_csc_host(const int32_t* CP, const int32_t* IC, int32_t n,
+                      int32_t nz) {
+  if (n <= 0 || nz < 0 || !CP || (nz > 0 && !IC)) return TBFS_ERR_INVALID;
+  if (CP[0] != 0 || CP[n] != nz) return TBFS_ERR_INVALID;
+  for (int32_t i = 0; i < n; ++i)
+    if (CP[i + 1] < CP[i]) return TBFS_ERR_INVALID;
+  for (int32_t k = 0; k < nz; ++k)
+    if ((uint32_t)IC[k] >= (uint32_t)n) return TBFS_ERR_INVALID;
+  return TBFS_OK;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// graphs
+namespace tbfs {
+
+void graph_free(tbfs_graph* g) {
+  if (!g) return;
+  if (g->RP != g->CP) dev_free(g->RP);
+  if (g->JR != g->IC) dev_free(g->JR);
+  if (g->J_owned) dev_free(g->J);
+  if (g->I != g->IC) dev_free(g->I);
+  dev_free(g->CP);
+  dev_free(g->IC);
+  delete g;
+}
+
+// Build RP/JR (out-lists) from CP/IC with a stable radix sort of the entries
+// by row: JR's rows come out in increasing column order, deterministically.
+int graph_build_transpose(tbfs_graph* g, hipStream_t s) {
+  const int32_t n = g->n, nz = g->nz;
+  int rc = dev_alloc(&g->RP, (size_t)n + 1);
+  if (rc) return rc;
+  rc = dev_alloc(&g->JR, (size_t)nz);
+  if (rc) return rc;
+  int32_t *col = nullptr, *keys_out = nullptr, *cnt = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_sort = 0, tmp_scan = 0;
+  int bits = 1;
+  while (bits < 31 && (1ll << bits) < (int64_t)n) ++bits;
+  rc = dev_alloc(&col, (size_t)nz);
+  if (!rc) rc = dev_alloc(&keys_out, (size_t)nz);
+  if (!rc) rc = dev_alloc(&cnt, (size_t)n + 1);
+  if (rc) goto out;
+  {
+    hipError_t e;
+    e = kern::expand_cols(s, g->CP, n, col);
+    if (e != hipSuccess) { rc = hip_fail(e, "expand_cols", __FILE__, __LINE__); goto out; }
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_sort, g->IC, keys_out,
+                                           col, g->JR, nz, 0, bits, s);
+    if (e != hipSuccess) { rc = hip_fail(e, "SortPairs(size)", __FILE__, __LINE__); goto out; }
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_scan, cnt, g->RP, n + 1, s);
+    if (e != hipSuccess) { rc = hip_fail(e, "ExclusiveSum(size)", __FILE__, __LINE__); goto out; }
+    rc = dev_alloc((char**)&tmp, std::max(tmp_sort, tmp_scan));
+    if (rc) goto out;
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_sort, g->IC, keys_out, col,
+                                           g->JR, nz, 0, bits, s);
+    if (e != hipSuccess) { rc = hip_fail(e, "SortPairs", __FILE__, __LINE__); goto out; }
+    e = hipMemsetAsync(cnt, 0, ((size_t)n + 1) * sizeof(int32_t), s);
+    if (e != hipSuccess) { rc = hip_fail(e, "hipMemsetAsync", __FILE__, __LINE__); goto out; }
+    e = kern::histogram(s, g->IC, nz, cnt);
+    if (e != hipSuccess) { rc = hip_fail(e, "histogram", __FILE__, __LINE__); goto out; }
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_scan, cnt, g->RP, n + 1, s);
+    if (e != hipSuccess) { rc = hip_fail(e, "ExclusiveSum", __FILE__, __LINE__); goto out; }
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__); goto out; }
+  }
+out:
+  dev_free(col);
+  dev_free(keys_out);
+  dev_free(cnt);
+  if (tmp) (void)hipFree(tmp);
+  return rc;
+}
+
+// Derive CP from a non-decreasing column array J (COOC sorted by column).
+int graph_cp_from_sorted(tbfs_graph* g, const int32_t* Jdev, hipStream_t s) {
+  const int32_t n = g->n, nz = g->nz;
+  int32_t* cnt = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_scan = 0;
+  int rc = dev_alloc(&g->CP, (size_t)n + 1);
+  if (!rc) rc = dev_alloc(&cnt, (size_t)n + 1);
+  if (rc) goto out;
+  {
+    hipError_t e = hipMemsetAsync(cnt, 0, ((size_t)n + 1) * sizeof(int32_t), s);
+    if (e == hipSuccess) e = kern::histogram(s, Jdev, nz, cnt);
+    if (e == hipSuccess)
+      e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_scan, cnt, g->CP, n + 1, s);
+    if (e != hipSuccess) { rc = hip_fail(e, "cp_from_sorted", __FILE__, __LINE__); goto out; }
+    rc = dev_alloc((char**)&tmp, tmp_scan);
+    if (rc) goto out;
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_scan, cnt, g->CP, n + 1, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { rc = hip_fail(e, "cp_from_sorted", __FILE__, __LINE__); goto out; }
+  }
+out:
+  dev_free(cnt);
+  if (tmp) (void)hipFree(tmp);
+  return rc;
+}
+
+int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
+                     int32_t nz, uint32_t flags, tbfs_graph** out) {
+  if (!out) return TBFS_ERR_INVALID;
+  *out = nullptr;
+  if (n <= 0 || nz < 0 || !CP || (nz > 0 && !IC)) return TBFS_ERR_INVALID;
+  const bool on_dev = (flags & TBFS_GRAPH_DEVICE) != 0;
+  if (!on_dev) {
+    int rc = validate_csc_host(CP, IC, n, nz);
+    if (rc) return rc;
+  }
+  tbfs_graph* g = new (std::nothrow) tbfs_graph();
+  if (!g) return TBFS_ERR_NOMEM;
+  g->n = n;
+  g->nz = nz;
+  g->flags = flags & (TBFS_GRAPH_SYMMETRIC);
+  hipStream_t s = nullptr;
+  int rc = TBFS_OK;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) { rc = hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); goto fail; }
+  rc = dev_alloc(&g->CP, (size_t)n + 1);
+  if (!rc) rc = dev_alloc(&g->IC, (size_t)nz);
+  if (rc) goto fail;
+  {
+    const hipMemcpyKind kind = on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    e = hipMemcpyAsync(g->CP, CP, ((size_t)n + 1) * sizeof(int32_t), kind, s);
+    if (e == hipSuccess && nz > 0)
+      e = hipMemcpyAsync(g->IC, IC, (size_t)nz * sizeof(int32_t), kind, s);
+    if (e != hipSuccess) { rc = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__); goto fail; }
+  }
+  if (on_dev) {
+    int32_t* bad = nullptr;
+    rc = dev_alloc(&bad, 1);
+    if (rc) goto fail;
+    int32_t hbad = 0;
+    e = hipMemsetAsync(bad, 0, sizeof(int32_t), s);
+    if (e == hipSuccess) e = kern::check_csc(s, g->CP, g->IC, n, nz, bad);
+    if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dev_free(bad);
+    if (e != hipSuccess) { rc = hip_fail(e, "check_csc", __FILE__, __LINE__); goto fail; }
+    if (hbad) { rc = TBFS_ERR_INVALID; goto fail; }
+  }
+  if (g->flags & TBFS_GRAPH_SYMMETRIC) {
+    g->RP = g->CP;
+    g->JR = g->IC;
+  } else {
+    rc = graph_build_transpose(g, s);
+    if (rc) goto fail;
+  }
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__); goto fail; }
+  (void)hipStreamDestroy(s);
+  *out = g;
+  return TBFS_OK;
+fail:
+  if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+  graph_free(g);
+  return rc;
+}
+
+int graph_create_cooc(const int32_t* I, const int32_t* J, int32_t n, int32_t nz,
+                      uint32_t flags, tbfs_graph** out) {
+  if (!out) return TBFS_ERR_INVALID;
+  *out = nullptr;
+  if (n <= 0 || nz < 0 || (nz > 0 && (!I || !J))) return TBFS_ERR_INVALID;
+  if (flags & TBFS_GRAPH_DEVICE) return TBFS_ERR_UNSUPPORTED;
+  bool sorted = true;
+  for (int32_t k = 0; k < nz; ++k) {
+    if ((uint32_t)I[k] >= (uint32_t)n || (uint32_t)J[k] >= (uint32_t)n)
+      return TBFS_ERR_INVALID;
+    if (k > 0 && J[k] < J[k - 1]) sorted = false;
+  }
+  tbfs_graph* g = new (std::nothrow) tbfs_graph();
+  if (!g) return TBFS_ERR_NOMEM;
+  g->n = n;
+  g->nz = nz;
+  g->flags = flags & TBFS_GRAPH_SYMMETRIC;
+  hipStream_t s = nullptr;
+  int rc = TBFS_OK;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) { rc = hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); goto fail; }
+  // I doubles as IC when J is sorted (CSC of the same matrix)
+  rc = dev_alloc(&g->IC, (size_t)nz);
+  if (!rc) rc = dev_alloc(&g->J, (size_t)nz);
+  if (rc) goto fail;
+  g->I = g->IC;
+  g->J_owned = true;
+  e = hipMemcpyAsync(g->IC, I, (size_t)nz * sizeof(int32_t), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(g->J, J, (size_t)nz * sizeof(int32_t), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) { rc = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__); goto fail; }
+  if (sorted) {
+    rc = graph_cp_from_sorted(g, g->J, s);
+    if (rc) goto fail;
+    g->has_csc = true;
+    if (g->flags & TBFS_GRAPH_SYMMETRIC) {
+      g->RP = g->CP;
+      g->JR = g->IC;
+    } else {
+      rc = graph_build_transpose(g, s);
+      if (rc) goto fail;
+    }
+  } else {
+    g->has_csc = false;
+  }
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__); goto fail; }
+  (void)hipStreamDestroy(s);
+  *out = g;
+  return TBFS_OK;
+fail:
+  if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+  graph_free(g);
+  return rc;
+}
+
+// Column-index array for the COOC kernel on a CSC graph (built on first use).
+int graph_ensure_cooc(tbfs_graph* g, hipStream_t s) {
+  if (g->J) return TBFS_OK;
+  int rc = dev_alloc(&g->J, (size_t)g->nz);
+  if (rc) return rc;
+  g->J_owned = true;
+  g->I = g->IC;
+  HIPCHK(kern::expand_cols(s, g->CP, g->n, g->J));
+  return TBFS_OK;
+}
+
+// ----------------------------------------------------------------------------
+// contexts
+void ctx_free(tbfs_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dev_free(c->S);
+  dev_free(c->sigma);
+  dev_free(c->ovf);
+  for (int i = 0; i < 3; ++i) dev_free(c->fb[i]);
+  for (int i = 0; i < 2; ++i) dev_free(c->q[i]);
+  dev_free(c->ctr);
+  if (c->ctr_h) (void)hipHostFree(c->ctr_h);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int ctx_create(int32_t n_cap, tbfs_ctx** out) {
+  if (!out || n_cap <= 0) return TBFS_ERR_INVALID;
+  *out = nullptr;
+  tbfs_ctx* c = new (std::nothrow) tbfs_ctx();
+  if (!c) return TBFS_ERR_NOMEM;
+  c->n_cap = n_cap;
+  const size_t nw = ((size_t)n_cap + 63) / 64;
+  int rc = TBFS_OK;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) { rc = hip_fail(e, "ctx stream/events", __FILE__, __LINE__); goto fail; }
+  rc = dev_alloc(&c->S, (size_t)n_cap);
+  if (!rc) rc = dev_alloc(&c->sigma, (size_t)n_cap);
+  if (!rc) rc = dev_alloc(&c->ovf, (size_t)n_cap);
+  for (int i = 0; i < 3 && !rc; ++i) rc = dev_alloc(&c->fb[i], nw);
+  for (int i = 0; i < 2 && !rc; ++i) rc = dev_alloc(&c->q[i], (size_t)n_cap);
+  if (!rc) rc = dev_alloc(&c->ctr, 4);
+  if (rc) goto fail;
+  e = hipHostMalloc((void**)&c->ctr_h, 4 * sizeof(LevelCtr), hipHostMallocDefault);
+  if (e != hipSuccess) { rc = hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); goto fail; }
+  *out = c;
+  return TBFS_OK;
+fail:
+  ctx_free(c);
+  return rc;
+}
+
+// ----------------------------------------------------------------------------
+// the BFS: host-driven level loop
+int run_bfs(tbfs_ctx* c, const tbfs_graph* gc, int32_t root, const tbfs_opts& o,
+            tbfs_stats* st) {
+  tbfs_graph* g = const_cast<tbfs_graph*>(gc);
+  if (!c || !g) return TBFS_ERR_INVALID;
+  if (g->n > c->n_cap) return TBFS_ERR_INVALID;
+  if (root < 0 || root >= g->n) return TBFS_ERR_INVALID;
+  if (o.mode < TBFS_MODE_AUTO || o.mode > TBFS_MODE_COOC) return TBFS_ERR_INVALID;
+  if (o.loop != TBFS_LOOP_HOST) return TBFS_ERR_UNSUPPORTED;
+  const bool csc_ok = (g->CP != nullptr);
+  if (o.mode != TBFS_MODE_COOC && !csc_ok) return TBFS_ERR_UNSUPPORTED;
+  if (o.mode == TBFS_MODE_COOC) {
+    if (!o.sigma) return TBFS_ERR_UNSUPPORTED;  // COOC reaches vertices through counts
+    int rc = graph_ensure_cooc(g, c->stream);
+    if (rc) return rc;
+  }
+  if (o.pull != TBFS_PULL_SC && o.pull != TBFS_PULL_WA) return TBFS_ERR_INVALID;
+  int block = o.block > 0 ? o.block : 256;
+  if (block % 64 != 0 || block > 1024) return TBFS_ERR_INVALID;
+
+  const int32_t n = g->n;
+  const int32_t nwords = (int32_t)(((int64_t)n + 63) / 64);
+  hipStream_t s = c->stream;
+  GraphView gv{n, g->nz, g->CP, g->IC, g->RP, g->JR};
+  const int fill_grid = grid_for(n, 256, 8192);
+
+  HIPCHK(hipEventRecord(c->ev0, s));
+  HIPCHK(kern::init(s, fill_grid, n, root, c->S, c->sigma, c->ovf, c->fb[0],
+                    c->fb[1], c->fb[2], c->q[0]));
+
+  // counters of the root's level (depth 0)
+  int64_t qsize = 1;
+  uint64_t m_frontier = 0, m_unexplored = (uint64_t)g->nz;
+  int64_t reached = 1;
+  uint64_t edges = 0;
+  if (csc_ok) {
+    int32_t cp2[2] = {0, 0}, rp2[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(cp2, g->CP + root, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(rp2, g->RP + root, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    edges = (uint64_t)(cp2[1] - cp2[0]);
+    m_frontier = (uint64_t)(rp2[1] - rp2[0]);
+    m_unexplored -= edges;
+  }
+  uint32_t ovf_any = 0;
+  int cur = 0;  // fb[cur] / q[cur&1] hold depth d
+  int32_t d = 0;
+  int push_levels = 0, pull_levels = 0;
+  bool pushing = (o.mode == TBFS_MODE_PUSH || o.mode == TBFS_MODE_AUTO);
+  const float alpha = o.alpha > 0 ? o.alpha : 4.0f;
+  const float beta = o.beta > 0 ? o.beta : 24.0f;
+
+  for (;;) {
+    if (o.mode == TBFS_MODE_AUTO) {
+      if (pushing && (double)m_frontier * alpha > (double)m_unexplored) pushing = false;
+      else if (!pushing && (double)qsize * beta < (double)n) pushing = true;
+    }
+    const int nxt = (cur + 1) % 3;
+    StateView sv{c->S, c->sigma, c->ovf, c->fb[cur], c->fb[nxt],
+                 c->q[d & 1], c->q[(d + 1) & 1], c->ctr};
+    HIPCHK(kern::level_prologue(s, grid_for(nwords, 256, 4096), c->fb[nxt],
+                                nwords, c->ctr));
+    const int sig = o.sigma ? 1 : 0;
+    switch (o.mode) {
+      case TBFS_MODE_PULL_SC:
+        HIPCHK(kern::pull_sc(s, grid_for(n, block, 65536), block, gv, sv, d,
+                             sig, o.bbreak, (int)ovf_any));
+        ++pull_levels;
+        break;
+      case TBFS_MODE_PULL_WA:
+        HIPCHK(kern::pull_wa(s, grid_for((int64_t)nwords * 64, block, 65536),
+                             block, gv, sv, d, sig, o.bbreak, (int)ovf_any));
+        ++pull_levels;
+        break;
+      case TBFS_MODE_COOC:
+        HIPCHK(kern::cooc_scatter(s, grid_for(g->nz, block, 65536), block,
+                                  g->I, g->J, g->nz, sv, (int)ovf_any));
+        HIPCHK(kern::cooc_finish(s, grid_for(n, block, 65536), block, gv, sv, d));
+        ++pull_levels;
+        break;
+      default:  // PUSH, or AUTO on its current side
+        if (pushing) {
+          HIPCHK(kern::push(s, grid_for(qsize * 64, block, 65536), block, gv,
+                            sv, d, qsize, (int)ovf_any));
+          ++push_levels;
+        } else if (o.pull == TBFS_PULL_WA) {
+          HIPCHK(kern::pull_wa(s, grid_for((int64_t)nwords * 64, block, 65536),
+                               block, gv, sv, d, sig, o.bbreak, (int)ovf_any));
+          ++pull_levels;
+        } else {
+          HIPCHK(kern::pull_sc(s, grid_for(n, block, 65536), block, gv, sv, d,
+                               sig, o.bbreak, (int)ovf_any));
+          ++pull_levels;
+        }
+        break;
+    }
+    HIPCHK(hipMemcpyAsync(c->ctr_h, c->ctr, sizeof(LevelCtr),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const LevelCtr lc = c->ctr_h[0];
+    if (lc.q == 0) break;
+    qsize = (int64_t)lc.q;
+    reached += (int64_t)lc.q;
+    edges += lc.min;
+    m_frontier = lc.mout;
+    m_unexplored = (m_unexplored > lc.min) ? m_unexplored - lc.min : 0;
+    ovf_any |= lc.ovf;
+    cur = nxt;
+    ++d;
+  }
+  HIPCHK(hipEventRecord(c->ev1, s));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_n = n;
+  c->last_ovf_any = ovf_any;
+  if (st) {
+    st->levels = d + 1;
+    st->push_levels = push_levels;
+    st->pull_levels = pull_levels;
+    st->pad = 0;
+    st->reached = reached;
+    st->edges = (int64_t)edges;
+    st->ovf64 = -1;  // filled by ctx_read when the flags are copied back
+    st->ms = (double)ms;
+  }
+  return TBFS_OK;
+}
+
+int ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma, uint8_t* ovf) {
+  if (!c || n != c->last_n || n <= 0) return TBFS_ERR_INVALID;
+  hipStream_t s = c->stream;
+  if (S) HIPCHK(hipMemcpyAsync(S, c->S, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (sigma) HIPCHK(hipMemcpyAsync(sigma, c->sigma, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  if (ovf) HIPCHK(hipMemcpyAsync(ovf, c->ovf, (size_t)n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return TBFS_OK;
+}
+
+}  // namespace tbfs
+
+// ----------------------------------------------------------------------------
+// extended C API
+extern "C" {
+
+void tbfs_opts_default(tbfs_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->mode = TBFS_MODE_AUTO;
+  o->loop = TBFS_LOOP_HOST;
+  o->sigma = 1;
+  o->bbreak = 0;
+  o->alpha = 4.0f;
+  o->beta = 24.0f;
+  o->pull = TBFS_PULL_SC;
+}
+
+const char* tbfs_status_str(int s) {
+  switch (s) {
+    case TBFS_OK: return "ok";
+    case TBFS_ERR_INVALID: return "invalid argument";
+    case TBFS_ERR_HIP: return "HIP runtime error";
+    case TBFS_ERR_NOMEM: return "out of memory";
+    case TBFS_ERR_UNSUPPORTED: return "unsupported";
+    case TBFS_SIGMA_INEXACT: return "path count exceeds the output type";
+    default: return "unknown status";
+  }
+}
+
+int tbfs_graph_create(const int32_t* CP, const int32_t* IC, int32_t n,
+                      int32_t nz, uint32_t flags, tbfs_graph** out) {
+  return graph_create_csc(CP, IC, n, nz, flags, out);
+}
+
+int tbfs_graph_create_cooc(const int32_t* I, const int32_t* J, int32_t n,
+                           int32_t nz, uint32_t flags, tbfs_graph** out) {
+  return graph_create_cooc(I, J, n, nz, flags, out);
+}
+
+int tbfs_graph_destroy(tbfs_graph* g) {
+  if (!g) return TBFS_ERR_INVALID;
+  graph_free(g);
+  return TBFS_OK;
+}
+
+int tbfs_graph_info(const tbfs_graph* g, int32_t* n, int32_t* nz,
+                    uint32_t* flags) {
+  if (!g) return TBFS_ERR_INVALID;
+  if (n) *n = g->n;
+  if (nz) *nz = g->nz;
+  if (flags) *flags = g->flags;
+  return TBFS_OK;
+}
+
+int tbfs_ctx_create(int32_t n_cap, tbfs_ctx** out) {
+  return ctx_create(n_cap, out);
+}
+
+int tbfs_ctx_destroy(tbfs_ctx* c) {
+  if (!c) return TBFS_ERR_INVALID;
+  ctx_free(c);
+  return TBFS_OK;
+}
+
+void* tbfs_ctx_stream(tbfs_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int tbfs_bfs(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
+             const tbfs_opts* o, tbfs_stats* st) {
+  tbfs_opts def;
+  tbfs_opts_default(&def);
+  return run_bfs(c, g, root, o ? *o : def, st);
+}
+
+int tbfs_ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma,
+                  uint8_t* ovf) {
+  return ctx_read(c, n, S, sigma, ovf);
+}
+
+int tbfs_ctx_device_ptrs(tbfs_ctx* c, int32_t** S, uint64_t** sigma,
+                         uint8_t** ovf) {
+  if (!c) return TBFS_ERR_INVALID;
+  if (S) *S = c->S;
+  if (sigma) *sigma = c->sigma;
+  if (ovf) *ovf = c->ovf;
+  return TBFS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,465 @@
+// tbfs_kernels.hip -- gfx950 level kernels of the single-source BFS.
+//
+// All kernels are integer work (no MFMA): depth S, path count sigma
+// (uint64 mod 2^64 + sticky overflow flag), frontier bitmaps and queues.
+// Wavefronts are 64 lanes; ballots are 64-bit; every per-level counter is
+// aggregated per wavefront before its single device-scope atomic.
+//
+// Correspondence with the reference kernels (paths relative to the reference
+// root):
+//   pull_sc   spMvUgCscScKernel + bfsFunctionsKernel fused
+//             (TurboBFS_CSC_TD/bfsgputdcsc_sc.cu:159-208), and with sigma
+//             off, bfsBuScKernel (TurboBFS_CSC_BU/bfsgpubucsc_sc.cu:122-148)
+//   pull_wa   spMvUgCscWaKernel (TurboBFS_CSC_TD/bfsgputdcsc_wa.cu:159-219)
+//             and bfsBuWaKernel (TurboBFS_CSC_BU/bfsgpubucsc_wa.cu:121-180):
+//             one wavefront per column, ballot early exit, DPP/shuffle sum
+//   cooc_*    spMvUgCoocScKernel + bfsFunctionsKernel
+//             (TurboBFS_COOC_TD/bfsgputdcooc_sc.cu:152-188)
+//   push      top-down expansion from a frontier queue (the TD half of the
+//             direction-optimizing switch, TurboBFS_CSC_TDBU/bfsgputdbucsc_sc.cu:119-153)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tbfs_internal.h"
+
+namespace tbfs {
+namespace {
+
+__device__ __forceinline__ int lane() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  return (1ull << lane()) - 1ull;
+}
+
+__device__ __forceinline__ bool fbit(const uint64_t* __restrict__ fb, int u) {
+  return (fb[u >> 6] >> (u & 63)) & 1ull;
+}
+
+// s += x with the carry folded into the sticky flag o.
+__device__ __forceinline__ void add_carry(uint64_t& s, uint32_t& o, uint64_t x) {
+  const uint64_t t = s + x;
+  o |= (t < s) ? 1u : 0u;
+  s = t;
+}
+
+// Butterfly sum of (value, flag) pairs over the 64 lanes; every lane ends with
+// the total.  All lanes must be converged.
+__device__ __forceinline__ void wave_sum_carry(uint64_t& s, uint32_t& o) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t t = __shfl_xor(s, off);
+    const uint32_t to = __shfl_xor(o, off);
+    const uint64_t r = s + t;
+    o = o | to | ((r < s) ? 1u : 0u);
+    s = r;
+  }
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+// Reserve one slot per lane with pred set in the queue counted by *ctr.
+// Returns this lane's slot (undefined for lanes without pred).
+__device__ __forceinline__ unsigned long long wave_append(
+    unsigned long long* ctr, bool pred) {
+  const uint64_t m = __ballot(pred);
+  unsigned long long base = 0;
+  if (m) {
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    if (lane() == leader)
+      base = atomicAdd(ctr, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+  }
+  return base + (unsigned long long)__popcll(m & lanemask_lt());
+}
+
+// Publish the vertices this wavefront discovered: queue slots, level counters.
+// `found` marks a discovered vertex v on this lane; od/id are its out/in
+// degrees, ov its overflow flag.  All lanes must be converged.
+__device__ __forceinline__ void emit(const StateView& st, int v, bool found,
+                                     uint64_t od, uint64_t id, uint32_t ov) {
+  const uint64_t m = __ballot(found);
+  if (m == 0) return;
+  const unsigned long long pos = wave_append(&st.ctr->q, found);
+  if (found) st.qnext[pos] = v;
+  const uint64_t sod = wave_sum_u64(found ? od : 0);
+  const uint64_t sid = wave_sum_u64(found ? id : 0);
+  const bool anyov = __ballot(found && ov) != 0;
+  if (lane() == 0) {
+    atomicAdd(&st.ctr->mout, (unsigned long long)sod);
+    atomicAdd(&st.ctr->min, (unsigned long long)sid);
+    if (anyov) atomicOr(&st.ctr->ovf, 1u);
+  }
+}
+
+__device__ __forceinline__ uint64_t outdeg(const GraphView& g, int v) {
+  return (uint64_t)(g.RP[v + 1] - g.RP[v]);
+}
+
+// ----------------------------------------------------------------------------
+__global__ void k_init(int32_t n, int32_t root, int32_t* S, uint64_t* sigma,
+                       uint8_t* ovf, uint64_t* fb0, uint64_t* fb1,
+                       uint64_t* fb2, int32_t* q0) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = tid; i < n; i += stride) {
+    S[i] = (i == root) ? 0 : -1;
+    sigma[i] = (i == root) ? 1ull : 0ull;
+    ovf[i] = 0;
+  }
+  const int64_t nw = ((int64_t)n + 63) >> 6;
+  for (int64_t w = tid; w < nw; w += stride) {
+    fb0[w] = (w == (root >> 6)) ? (1ull << (root & 63)) : 0ull;
+    fb1[w] = 0ull;
+    fb2[w] = 0ull;
+  }
+  if (tid == 0) q0[0] = root;
+}
+
+__global__ void k_level_prologue(uint64_t* fnext, int32_t nwords,
+                                 LevelCtr* ctr) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t w = tid; w < nwords; w += stride) fnext[w] = 0ull;
+  if (tid == 0) {
+    ctr->q = 0;
+    ctr->mout = 0;
+    ctr->min = 0;
+    ctr->ovf = 0;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Pull over unvisited columns, one lane per column.  Lanes of a wavefront own
+// 64 consecutive columns, so the ballot of "discovered" is exactly the next
+// frontier word for those columns.
+template <bool SIGMA>
+__global__ __launch_bounds__(256) void k_pull_sc(GraphView g, StateView st,
+                                                 int32_t d, int bbreak,
+                                                 int ovf_any) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+       base < g.n; base += stride) {
+    const int v = (int)base + lane();
+    bool found = false;
+    uint64_t sum = 0;
+    uint32_t ov = 0;
+    uint64_t id = 0;
+    if (v < g.n && st.S[v] == -1) {
+      const int beg = g.CP[v], end = g.CP[v + 1];
+      id = (uint64_t)(end - beg);
+      for (int k = beg; k < end; ++k) {
+        const int u = g.IC[k];
+        if (fbit(st.fcur, u)) {
+          found = true;
+          if (SIGMA) {
+            add_carry(sum, ov, st.sigma[u]);
+            if (ovf_any) ov |= st.ovf[u];
+          } else if (bbreak) {
+            break;
+          }
+        }
+      }
+      if (found) {
+        st.S[v] = d + 1;
+        if (SIGMA) {
+          st.sigma[v] = sum;
+          if (ov) st.ovf[v] = 1;
+        }
+      }
+    }
+    const uint64_t m = __ballot(found);
+    if (m) {
+      if (lane() == 0) st.fnext[base >> 6] = m;
+      emit(st, v, found, found ? outdeg(g, v) : 0, id, ov);
+    }
+  }
+}
+
+// Pull over unvisited columns, one wavefront per column.  A wavefront walks
+// the 64 columns of one frontier word in turn; lanes stride the column's
+// in-list, the path counts are summed with a 64-lane butterfly, and with
+// sigma off and bbreak set the whole wavefront leaves the column at the
+// first 64-entry chunk that holds a parent (ballot).
+template <bool SIGMA>
+__global__ __launch_bounds__(256) void k_pull_wa(GraphView g, StateView st,
+                                                 int32_t d, int bbreak,
+                                                 int ovf_any) {
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t base = wave * 64; base < g.n; base += nwaves * 64) {
+    const int cnt = (int)((g.n - base) < 64 ? (g.n - base) : 64);
+    uint64_t word = 0;
+    uint32_t myov = 0;
+    for (int j = 0; j < cnt; ++j) {
+      const int v = (int)base + j;
+      if (st.S[v] != -1) continue;
+      const int beg = g.CP[v], end = g.CP[v + 1];
+      bool f = false;
+      uint64_t sum = 0;
+      uint32_t ov = 0;
+      for (int k0 = beg; k0 < end; k0 += 64) {
+        const int k = k0 + lane();
+        if (k < end) {
+          const int u = g.IC[k];
+          if (fbit(st.fcur, u)) {
+            f = true;
+            if (SIGMA) {
+              add_carry(sum, ov, st.sigma[u]);
+              if (ovf_any) ov |= st.ovf[u];
+            }
+          }
+        }
+        if (!SIGMA && bbreak) {
+          if (__ballot(f)) break;
+        }
+      }
+      if (__ballot(f)) {
+        if (SIGMA) {
+          wave_sum_carry(sum, ov);
+          if (lane() == 0) {
+            st.sigma[v] = sum;
+            if (ov) st.ovf[v] = 1;
+          }
+        }
+        if (lane() == 0) st.S[v] = d + 1;
+        word |= 1ull << j;
+        if (lane() == j) myov = ov;
+      }
+    }
+    if (word) {
+      if (lane() == 0) st.fnext[base >> 6] = word;
+      const int v = (int)base + lane();
+      const bool found = (word >> lane()) & 1ull;
+      emit(st, v, found, found ? outdeg(g, v) : 0,
+           found ? (uint64_t)(g.CP[v + 1] - g.CP[v]) : 0, myov);
+    }
+  }
+}
+
+// Top-down expansion from the frontier queue: one wavefront per frontier
+// vertex u, lanes stride u's out-list.  The first lane to move S[v] from -1
+// to d+1 enqueues v; every edge into a vertex of depth d+1 adds sigma[u]
+// with a 64-bit atomic whose returned old value exposes the carry.
+__global__ __launch_bounds__(256) void k_push(GraphView g, StateView st,
+                                              int32_t d, int64_t qsize,
+                                              int ovf_any) {
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int dn = d + 1;
+  for (int64_t i = wave; i < qsize; i += nwaves) {
+    const int u = st.qcur[i];
+    const uint64_t x = st.sigma[u];
+    const uint32_t ou = ovf_any ? (uint32_t)st.ovf[u] : 0u;
+    const int beg = g.RP[u], end = g.RP[u + 1];
+    for (int k0 = beg; k0 < end; k0 += 64) {
+      const int k = k0 + lane();
+      bool nw = false;
+      int v = 0;
+      uint32_t ov = 0;
+      if (k < end) {
+        v = g.JR[k];
+        int s = st.S[v];
+        if (s == -1) {
+          const int old = atomicCAS(&st.S[v], -1, dn);
+          nw = (old == -1);
+          s = dn;  // old is -1 or dn: only this level writes S now
+        }
+        if (s == dn) {
+          const unsigned long long old =
+              atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)x);
+          if (old + x < old || ou) {
+            st.ovf[v] = 1;
+            ov = 1;
+          }
+        }
+        if (nw) atomicOr((unsigned long long*)&st.fnext[v >> 6], 1ull << (v & 63));
+      }
+      const uint64_t m = __ballot(nw || ov);
+      if (m) {
+        emit(st, v, nw, nw ? outdeg(g, v) : 0,
+             nw ? (uint64_t)(g.CP[v + 1] - g.CP[v]) : 0, ov);
+        // an overflow on an edge into an already-claimed vertex
+        if (__ballot(ov && !nw) && lane() == 0) atomicOr(&st.ctr->ovf, 1u);
+      }
+    }
+  }
+}
+
+// COOC, edge-parallel: every nonzero whose source is in the frontier adds the
+// source's count into its (unvisited) destination.
+__global__ __launch_bounds__(256) void k_cooc_scatter(const int32_t* __restrict__ I,
+                                                      const int32_t* __restrict__ J,
+                                                      int32_t nz, StateView st,
+                                                      int ovf_any) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nz;
+       k += stride) {
+    const int u = I[k];
+    if (!fbit(st.fcur, u)) continue;
+    const int v = J[k];
+    if (st.S[v] != -1) continue;
+    const uint64_t x = st.sigma[u];
+    const unsigned long long old =
+        atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)x);
+    if (old + x < old || (ovf_any && st.ovf[u])) st.ovf[v] = 1;
+  }
+}
+
+// COOC level finish: an unvisited vertex with a nonzero accumulated count was
+// reached this level (the reference's bfsFunctionsKernel).
+__global__ __launch_bounds__(256) void k_cooc_finish(GraphView g, StateView st,
+                                                     int32_t d) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+       base < g.n; base += stride) {
+    const int v = (int)base + lane();
+    bool found = false;
+    uint32_t ov = 0;
+    if (v < g.n && st.S[v] == -1) {
+      ov = st.ovf[v];
+      found = (st.sigma[v] != 0) || ov;
+      if (found) st.S[v] = d + 1;
+    }
+    const uint64_t m = __ballot(found);
+    if (m) {
+      if (lane() == 0) st.fnext[base >> 6] = m;
+      uint64_t od = 0, id = 0;
+      if (found && g.RP) od = outdeg(g, v);
+      if (found && g.CP) id = (uint64_t)(g.CP[v + 1] - g.CP[v]);
+      emit(st, v, found, od, id, ov);
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// setup kernels
+__global__ void k_expand_cols(const int32_t* CP, int32_t n, int32_t* col) {
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v = wave; v < n; v += nwaves) {
+    const int beg = CP[v], end = CP[v + 1];
+    for (int k = beg + lane(); k < end; k += 64) col[k] = (int32_t)v;
+  }
+}
+
+__global__ void k_histogram(const int32_t* keys, int32_t nz, int32_t* cnt) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nz;
+       k += stride)
+    atomicAdd(&cnt[keys[k]], 1);
+}
+
+// bad[0] |= 1: CP not monotone / CP[0] != 0 / CP[n] != nz; 2: IC out of range
+__global__ void k_check_csc(const int32_t* CP, const int32_t* IC, int32_t n,
+                            int32_t nz, int32_t* bad) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int b = 0;
+  for (int64_t i = tid; i < n; i += stride)
+    if (CP[i + 1] < CP[i]) b |= 1;
+  if (tid == 0 && (CP[0] != 0 || CP[n] != nz)) b |= 1;
+  for (int64_t k = tid; k < nz; k += stride)
+    if ((uint32_t)IC[k] >= (uint32_t)n) b |= 2;
+  if (b) atomicOr(bad, b);
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+namespace kern {
+
+hipError_t init(hipStream_t s, int grid, int32_t n, int32_t root, int32_t* S,
+                uint64_t* sigma, uint8_t* ovf, uint64_t* fb0, uint64_t* fb1,
+                uint64_t* fb2, int32_t* q0) {
+  hipLaunchKernelGGL(k_init, dim3(grid), dim3(256), 0, s, n, root, S, sigma,
+                     ovf, fb0, fb1, fb2, q0);
+  return hipGetLastError();
+}
+
+hipError_t level_prologue(hipStream_t s, int grid, uint64_t* fnext,
+                          int32_t nwords, LevelCtr* ctr) {
+  hipLaunchKernelGGL(k_level_prologue, dim3(grid), dim3(256), 0, s, fnext,
+                     nwords, ctr);
+  return hipGetLastError();
+}
+
+hipError_t pull_sc(hipStream_t s, int grid, int block, const GraphView& g,
+                   const StateView& st, int32_t d, int sigma, int bbreak,
+                   int ovf_any) {
+  if (sigma)
+    hipLaunchKernelGGL(k_pull_sc<true>, dim3(grid), dim3(block), 0, s, g, st,
+                       d, bbreak, ovf_any);
+  else
+    hipLaunchKernelGGL(k_pull_sc<false>, dim3(grid), dim3(block), 0, s, g, st,
+                       d, bbreak, ovf_any);
+  return hipGetLastError();
+}
+
+hipError_t pull_wa(hipStream_t s, int grid, int block, const GraphView& g,
+                   const StateView& st, int32_t d, int sigma, int bbreak,
+                   int ovf_any) {
+  if (sigma)
+    hipLaunchKernelGGL(k_pull_wa<true>, dim3(grid), dim3(block), 0, s, g, st,
+                       d, bbreak, ovf_any);
+  else
+    hipLaunchKernelGGL(k_pull_wa<false>, dim3(grid), dim3(block), 0, s, g, st,
+                       d, bbreak, ovf_any);
+  return hipGetLastError();
+}
+
+hipError_t push(hipStream_t s, int grid, int block, const GraphView& g,
+                const StateView& st, int32_t d, int64_t qsize, int ovf_any) {
+  hipLaunchKernelGGL(k_push, dim3(grid), dim3(block), 0, s, g, st, d, qsize,
+                     ovf_any);
+  return hipGetLastError();
+}
+
+hipError_t cooc_scatter(hipStream_t s, int grid, int block, const int32_t* I,
+                        const int32_t* J, int32_t nz, const StateView& st,
+                        int ovf_any) {
+  hipLaunchKernelGGL(k_cooc_scatter, dim3(grid), dim3(block), 0, s, I, J, nz,
+                     st, ovf_any);
+  return hipGetLastError();
+}
+
+hipError_t cooc_finish(hipStream_t s, int grid, int block, const GraphView& g,
+                       const StateView& st, int32_t d) {
+  hipLaunchKernelGGL(k_cooc_finish, dim3(grid), dim3(block), 0, s, g, st, d);
+  return hipGetLastError();
+}
+
+hipError_t expand_cols(hipStream_t s, const int32_t* CP, int32_t n,
+                       int32_t* col) {
+  int grid = (int)(((int64_t)n + 3) / 4);
+  if (grid > 65536) grid = 65536;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_expand_cols, dim3(grid), dim3(256), 0, s, CP, n, col);
+  return hipGetLastError();
+}
+
+hipError_t histogram(hipStream_t s, const int32_t* keys, int32_t nz,
+                     int32_t* cnt) {
+  int grid = (int)(((int64_t)nz + 255) / 256);
+  if (grid > 65536) grid = 65536;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_histogram, dim3(grid), dim3(256), 0, s, keys, nz, cnt);
+  return hipGetLastError();
+}
+
+hipError_t check_csc(hipStream_t s, const int32_t* CP, const int32_t* IC,
+                     int32_t n, int32_t nz, int32_t* bad) {
+  int64_t m = (int64_t)(n > nz ? n : nz);
+  int grid = (int)((m + 255) / 256);
+  if (grid > 65536) grid = 65536;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_check_csc, dim3(grid), dim3(256), 0, s, CP, IC, n, nz,
+                     bad);
+  return hipGetLastError();
+}
+
+}  // namespace kern
+}  // namespace tbfs
