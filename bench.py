@@ -89,7 +89,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--sources", type=int, default=64)
-    ap.add_argument("--mode", default="auto", choices=["auto", "auto_wa", "pull_sc", "pull_wa", "push", "cooc"])
+    ap.add_argument("--mode", default="auto",
+                    choices=["auto", "auto_host", "auto_sc", "auto_wa", "pull_sc", "pull_wa",
+                             "push", "cooc"])
+    ap.add_argument("--grid", type=int, default=0, help="workgroups of the persistent kernel (0: all co-resident)")
+    ap.add_argument("--alpha", type=float, default=0.0)
+    ap.add_argument("--beta", type=float, default=0.0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -106,37 +111,45 @@ def main():
     torch.cuda.set_device(0)
     g = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)
     sources = graphs.sample_sources(graphs.nonzero_degree_vertices(g), args.sources, seed=2)
+    H, P = _lib.LOOP_HOST, _lib.LOOP_PERSISTENT
     modes = {
-        "auto": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_SC),
-        "auto_wa": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_WA),
-        "pull_sc": dict(mode=_lib.MODE_PULL_SC),
-        "pull_wa": dict(mode=_lib.MODE_PULL_WA),
-        "push": dict(mode=_lib.MODE_PUSH),
-        "cooc": dict(mode=_lib.MODE_COOC),
+        "auto": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, loop=P),
+        "auto_host": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, loop=H),
+        "auto_sc": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_SC, loop=H),
+        "auto_wa": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_WA, loop=H),
+        "pull_sc": dict(mode=_lib.MODE_PULL_SC, loop=H),
+        "pull_wa": dict(mode=_lib.MODE_PULL_WA, loop=H),
+        "push": dict(mode=_lib.MODE_PUSH, pull=_lib.PULL_LB, loop=P),
+        "cooc": dict(mode=_lib.MODE_COOC, loop=H),
     }
-    opts = tb.default_opts(**modes[args.mode])
+    kw = dict(modes[args.mode])
+    if args.grid:
+        kw["grid"] = args.grid
+    if args.alpha:
+        kw["alpha"] = args.alpha
+    if args.beta:
+        kw["beta"] = args.beta
+    opts = tb.default_opts(**kw)
     gr = tb.Graph(g.CP, g.IC, symmetric=True)
     ctx = tb.Context(g.n)
     stream = torch.cuda.ExternalStream(ctx.stream)
 
-    for i in range(args.warmup):
-        ctx.run(gr, int(sources[i % len(sources)]), opts)
+    roots = np.array([sources[i % len(sources)] for i in range(args.steps)], np.int32)
+    if args.warmup:
+        ctx.run_many(gr, roots[:args.warmup] if args.warmup <= len(roots) else
+                     np.resize(roots, args.warmup), opts)
     torch.cuda.synchronize()
 
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
-    edges = 0
-    kern_ms = 0.0
-    levels = []
     ev0.record(stream)
-    for i in range(args.steps):
-        st = ctx.run(gr, int(sources[i % len(sources)]), opts)
-        edges += st["edges"]
-        kern_ms += st["ms"]
-        levels.append(st["levels"])
+    stats = ctx.run_many(gr, roots, opts)
     ev1.record(stream)
     torch.cuda.synchronize()
     total_ms = ev0.elapsed_time(ev1)
+    edges = sum(st["edges"] for st in stats)
+    kern_ms = sum(st["ms"] for st in stats)
+    levels = [st["levels"] for st in stats]
     gteps = (edges / 2) / (total_ms * 1e-3) / 1e9
 
     # roofline: algorithmic bytes of one BFS = 4(n+1) + 4 m_s + 12 n
@@ -161,11 +174,15 @@ def main():
                         f"ef16 a.57 b.19 c.19 symmetrized relabelled seed 2",
             "n": n, "nz": g.nz, "sources": len(sources), "source_seed": 2,
             "mode": args.mode, "mean_levels": float(np.mean(levels)),
+            "push_levels": float(np.mean([st["push_levels"] for st in stats])),
+            "pull_levels": float(np.mean([st["pull_levels"] for st in stats])),
+            "bfs_ms_median": float(np.median([st["ms"] for st in stats])),
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "whole BFS (all level kernels of one source)",
+            "kernel": "k_persist (one launch = one whole BFS)" if args.mode in ("auto", "push")
+                      else "whole BFS (all level kernels of one source)",
         },
     }
     if not args.no_cpu:

@@ -48,12 +48,17 @@ extern "C" {
 #define TBFS_MODE_PUSH 3    /* top-down push from a queue, every level */
 #define TBFS_MODE_COOC 4    /* edge-parallel over the COOC nonzeros */
 
-/* pull kernel used by TBFS_MODE_AUTO for its bottom-up levels (tbfs_opts.pull) */
-#define TBFS_PULL_SC 0 /* one lane per column */
-#define TBFS_PULL_WA 1 /* one wavefront per column */
+/* kernels used by TBFS_MODE_AUTO / TBFS_MODE_PUSH (tbfs_opts.pull) */
+#define TBFS_PULL_SC 0 /* pull: one lane per column; push: one wavefront per slice */
+#define TBFS_PULL_WA 1 /* pull: one wavefront per column; push: as above */
+#define TBFS_PULL_LB 2 /* load-balanced: wavefronts split work by edge count */
 
 /* level loop (tbfs_opts.loop) */
-#define TBFS_LOOP_HOST 0    /* host launches each level, reads its counters */
+#define TBFS_LOOP_HOST 0       /* host launches each level, reads its counters */
+#define TBFS_LOOP_PERSISTENT 1 /* one cooperative launch per BFS, levels
+                                  separated by a device-wide barrier; used for
+                                  AUTO / PUSH with TBFS_PULL_LB, otherwise the
+                                  host loop runs */
 
 typedef struct tbfs_graph tbfs_graph;
 typedef struct tbfs_ctx tbfs_ctx;
@@ -78,11 +83,12 @@ typedef struct tbfs_stats {
   int32_t pad;
   int64_t reached;     /* vertices with S >= 0 */
   int64_t edges;       /* m_s: in-list entries of the reached vertices */
-  int64_t ovf64;       /* reached vertices whose count is >= 2^64 */
+  int64_t ovf64;       /* -1 when some reached count is >= 2^64, else 0 */
   double ms;           /* device time of the BFS, first to last kernel */
 } tbfs_stats;
 
-/* Fill *o with the defaults: AUTO, host loop, sigma on, alpha 4, beta 24. */
+/* Fill *o with the defaults: AUTO, load-balanced kernels, persistent loop,
+ * sigma on, alpha 4, beta 24. */
 void tbfs_opts_default(tbfs_opts *o);
 
 /* Human-readable name of a status code. */
@@ -119,6 +125,13 @@ void *tbfs_ctx_stream(tbfs_ctx *c);
  * until the next BFS; st (may be NULL) receives the counters. */
 int tbfs_bfs(tbfs_ctx *c, const tbfs_graph *g, int32_t root,
              const tbfs_opts *o, tbfs_stats *st);
+
+/* Run `count` BFS back to back (roots[i]), with no host synchronisation in
+ * between when the persistent loop applies.  Results of the last BFS stay in
+ * the context; st (may be NULL) receives count entries, each with its own
+ * device time. */
+int tbfs_bfs_many(tbfs_ctx *c, const tbfs_graph *g, const int32_t *roots,
+                  int32_t count, const tbfs_opts *o, tbfs_stats *st);
 
 /* Copy the last BFS's results to host memory (any pointer may be NULL;
  * n must equal that graph's vertex count). */

@@ -88,13 +88,22 @@ def test_reference_fixtures_dropin(gpu, fname, r):
     _check_dropin(g, r)
 
 
+L = tb._lib
+HOST = L.LOOP_HOST
+PERS = L.LOOP_PERSISTENT
 MODES = [
-    ("auto_sc", dict(mode=tb._lib.MODE_AUTO, pull=tb._lib.PULL_SC)),
-    ("auto_wa", dict(mode=tb._lib.MODE_AUTO, pull=tb._lib.PULL_WA)),
-    ("pull_sc", dict(mode=tb._lib.MODE_PULL_SC)),
-    ("pull_wa", dict(mode=tb._lib.MODE_PULL_WA)),
-    ("push", dict(mode=tb._lib.MODE_PUSH)),
-    ("cooc", dict(mode=tb._lib.MODE_COOC)),
+    ("auto_sc", dict(mode=L.MODE_AUTO, pull=L.PULL_SC, loop=HOST)),
+    ("auto_wa", dict(mode=L.MODE_AUTO, pull=L.PULL_WA, loop=HOST)),
+    ("auto_lb", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=HOST)),
+    ("pull_sc", dict(mode=L.MODE_PULL_SC, loop=HOST)),
+    ("pull_wa", dict(mode=L.MODE_PULL_WA, loop=HOST)),
+    ("push", dict(mode=L.MODE_PUSH, pull=L.PULL_SC, loop=HOST)),
+    ("push_lb", dict(mode=L.MODE_PUSH, pull=L.PULL_LB, loop=HOST)),
+    ("cooc", dict(mode=L.MODE_COOC, loop=HOST)),
+    ("pers_auto", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=PERS)),
+    ("pers_push", dict(mode=L.MODE_PUSH, pull=L.PULL_LB, loop=PERS)),
+    ("pers_pull", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=PERS, alpha=1e9)),
+    ("pers_auto_g64", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=PERS, grid=64)),
 ]
 
 
@@ -132,6 +141,53 @@ def test_grid_overflow_all_modes(gpu):
 def test_perturbed_grid_modes(gpu):
     g = graphs.grid(96, 64, 0.1, seed=3, relabel=True)
     _check_modes(g, [0, 17, 5000], symmetric=True)
+
+
+def test_hub_columns_split_across_slices(gpu):
+    """Star + clique mix: columns with in-degree > 1024 take the static
+    pull-slice path and out-lists > 64 the sliced push path."""
+    hub_leaves = 5000
+    n = hub_leaves + 1 + 40
+    src, dst = [], []
+    for v in range(1, hub_leaves + 1):          # star around vertex 0
+        src += [0, v]
+        dst += [v, 0]
+    base = hub_leaves + 1
+    for a in range(40):                          # clique joined to two leaves
+        for b in range(40):
+            if a != b:
+                src.append(base + a)
+                dst.append(base + b)
+    for a in range(40):
+        for leaf in (7, 11):
+            src += [base + a, leaf]
+            dst += [leaf, base + a]
+    src = np.array(src, np.int32)
+    dst = np.array(dst, np.int32)
+    order = np.lexsort((src, dst))
+    IC = src[order]
+    CP = np.zeros(n + 1, np.int64)
+    CP[1:] = np.cumsum(np.bincount(dst, minlength=n))
+    g = graphs.CSC(CP.astype(np.int32), IC)
+    _check_modes(g, [0, 3, base + 5], symmetric=True)
+    _check_modes(g, [base], symmetric=False)
+
+
+def test_run_many_matches_single(gpu):
+    g = graphs.rmat(12, 16, seed=7)
+    roots = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 6, seed=1)
+    gr = tb.Graph(g.CP, g.IC, symmetric=True)
+    ctx = tb.Context(g.n)
+    stats = ctx.run_many(gr, roots)
+    S, sig, ovf = ctx.read(g.n)
+    S2, sig2, ovf2, lv = oracle.textbook(g.CP, g.IC, int(roots[-1]))
+    np.testing.assert_array_equal(S, S2)
+    np.testing.assert_array_equal(sig, sig2)
+    for r, st in zip(roots, stats):
+        Sr, _, _, lvr = oracle.textbook(g.CP, g.IC, int(r))
+        assert st["levels"] == lvr and st["reached"] == int((Sr >= 0).sum())
+        assert st["edges"] == int(np.diff(g.CP)[Sr >= 0].sum())
+        assert st["ms"] > 0
 
 
 def test_directed_reference_fixture_modes(gpu):

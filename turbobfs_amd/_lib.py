@@ -32,8 +32,10 @@ MODE_COOC = 4
 
 PULL_SC = 0
 PULL_WA = 1
+PULL_LB = 2
 
 LOOP_HOST = 0
+LOOP_PERSISTENT = 1
 
 _i32p = C.POINTER(C.c_int32)
 _u64p = C.POINTER(C.c_uint64)
@@ -90,6 +92,7 @@ SIGNATURES = {
     "tbfs_ctx_destroy": (C.c_int, [C.c_void_p]),
     "tbfs_ctx_stream": (C.c_void_p, [C.c_void_p]),
     "tbfs_bfs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(Opts), C.POINTER(Stats)]),
+    "tbfs_bfs_many": (C.c_int, [C.c_void_p, C.c_void_p, _i32p, C.c_int32, C.POINTER(Opts), C.POINTER(Stats)]),
     "tbfs_ctx_read": (C.c_int, [C.c_void_p, C.c_int32, _i32p, _u64p, _u8p]),
     "tbfs_ctx_device_ptrs": (C.c_int, [C.c_void_p, C.POINTER(_i32p), C.POINTER(_u64p), C.POINTER(_u8p)]),
     # generators (tbfs_gen.h)

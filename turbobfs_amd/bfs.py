@@ -169,6 +169,17 @@ class Context:
               "tbfs_bfs")
         return {f: getattr(st, f) for f, _ in Stats._fields_ if f != "pad"}
 
+    def run_many(self, g: Graph, roots, opts: Opts | None = None) -> list[dict]:
+        """BFS from every root in turn, back to back on the device; returns
+        the per-BFS stats (results of the last BFS stay on the device)."""
+        roots = np.ascontiguousarray(roots, dtype=np.int32)
+        k = int(roots.shape[0])
+        st = (Stats * k)()
+        o = opts if opts is not None else default_opts()
+        check(_lib.load().tbfs_bfs_many(self._h, g.handle, _ptr(roots, _i32p), k, C.byref(o),
+                                        st), "tbfs_bfs_many")
+        return [{f: getattr(x, f) for f, _ in Stats._fields_ if f != "pad"} for x in st]
+
     def read(self, n: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         S = np.empty(n, dtype=np.int32)
         sigma = np.empty(n, dtype=np.uint64)
@@ -180,7 +191,7 @@ class Context:
     def bfs(self, g: Graph, root: int, opts: Opts | None = None) -> Result:
         stats = self.run(g, root, opts)
         S, sigma, ovf = self.read(g.n)
-        stats["ovf64"] = int(ovf.sum())
+        stats["ovf64"] = int(ovf[S >= 0].sum())
         return Result(S, sigma, ovf, stats)
 
     def close(self):

@@ -137,6 +137,8 @@ int td_csc(const char* name, int* IC_h, int* CP_h, int* S_h, float* sigma_h,
   tbfs_opts o;
   tbfs_opts_default(&o);
   o.mode = mode;
+  o.loop = TBFS_LOOP_HOST;
+  o.pull = mode == TBFS_MODE_PULL_WA ? TBFS_PULL_WA : TBFS_PULL_SC;
   DropinResult res;
   int rc = run_dropin(name, CP_h, IC_h, nz, n, r, repetition, o, false, true, &res);
   if (rc) return rc;
@@ -150,6 +152,8 @@ int bu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int r, int nz,
   tbfs_opts o;
   tbfs_opts_default(&o);
   o.mode = mode;
+  o.loop = TBFS_LOOP_HOST;
+  o.pull = mode == TBFS_MODE_PULL_WA ? TBFS_PULL_WA : TBFS_PULL_SC;
   o.sigma = 0;
   o.bbreak = bbreak ? 1 : 0;
   DropinResult res;
@@ -166,6 +170,7 @@ int tdbu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int* sigma_h,
   tbfs_opts_default(&o);
   o.mode = TBFS_MODE_AUTO;
   o.pull = pull;
+  o.loop = pull == TBFS_PULL_LB ? TBFS_LOOP_PERSISTENT : TBFS_LOOP_HOST;
   DropinResult res;
   int rc = run_dropin(name, CP_h, I_h, nz, n, r, repetition, o, false, true, &res);
   if (rc) return rc;
@@ -205,7 +210,7 @@ int bfs_gpu_tdbu_csc_sc(int* I_h, int* CP_h, int* S_h, int* sigma_h, int r,
                         int nz, int n, int repetition, int bbreak) {
   (void)bbreak;  // counts need every parent: no early exit with sigma on
   return tdbu_csc("bfsgputdbucsc_sc", I_h, CP_h, S_h, sigma_h, r, nz, n,
-                  repetition, TBFS_PULL_SC);
+                  repetition, TBFS_PULL_LB);
 }
 
 int bfs_gpu_tdbu_csc_wa(int* IC_h, int* CP_h, int* S_h, int* sigma_h, int r,
@@ -221,6 +226,7 @@ int bfs_gpu_td_cooc_sc(int* I_h, int* J_h, float* sigma_h, int r, int nz,
   tbfs_opts o;
   tbfs_opts_default(&o);
   o.mode = TBFS_MODE_COOC;
+  o.loop = TBFS_LOOP_HOST;
   DropinResult res;
   int rc = run_dropin("bfsgputdcooc_sc", I_h, J_h, nz, n, r, repetition, o,
                       true, true, &res);

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "tbfs.h"
 #include "tbfs_internal.h"
 
@@ -19,6 +21,8 @@ struct tbfs_graph {
   int32_t* J = nullptr;   // COOC destination of each entry
   bool J_owned = false;
   bool has_csc = true;
+  uint64_t* bigc = nullptr;  // static pull slices of high in-degree columns
+  int32_t nbigc = 0;
 };
 
 // Per-BFS device state bound to one stream.
@@ -30,11 +34,18 @@ struct tbfs_ctx {
   uint64_t* sigma = nullptr;
   uint8_t* ovf = nullptr;
   uint64_t* fb[3] = {nullptr, nullptr, nullptr};
-  int32_t* q[2] = {nullptr, nullptr};
-  tbfs::LevelCtr* ctr = nullptr;    // device
+  uint64_t* q[2] = {nullptr, nullptr};
+  int64_t q_cap = 0;
+  tbfs::LevelCtr* ctr = nullptr;    // device, 3 rotating slots (+1 spare)
   tbfs::LevelCtr* ctr_h = nullptr;  // pinned host mirror
+  unsigned* bar = nullptr;          // grid barrier words
+  tbfs::RunStats* stats = nullptr;  // device, stats_cap slots
+  tbfs::RunStats* stats_h = nullptr;
+  int stats_cap = 0;
+  std::vector<hipEvent_t> evs;
+  int device = 0;
+  int persist_grid_max = 0;         // 0: no cooperative launch available
   int32_t last_n = 0;
-  uint32_t last_ovf_any = 0;
 };
 
 namespace tbfs {
@@ -48,6 +59,8 @@ int ctx_create(int32_t n_cap, tbfs_ctx** out);
 void ctx_free(tbfs_ctx* c);
 int run_bfs(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
             const tbfs_opts& o, tbfs_stats* st);
+int run_bfs_many(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots,
+                 int count, const tbfs_opts& o, tbfs_stats* st);
 int ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma,
              uint8_t* ovf);
 }  // namespace tbfs

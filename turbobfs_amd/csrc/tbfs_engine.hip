@@ -85,6 +85,7 @@ void graph_free(tbfs_graph* g) {
   if (g->JR != g->IC) dev_free(g->JR);
   if (g->J_owned) dev_free(g->J);
   if (g->I != g->IC) dev_free(g->I);
+  dev_free(g->bigc);
   dev_free(g->CP);
   dev_free(g->IC);
   delete g;
@@ -165,6 +166,8 @@ out:
   return rc;
 }
 
+int graph_build_bigc(tbfs_graph* g);
+
 int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
                      int32_t nz, uint32_t flags, tbfs_graph** out) {
   if (!out) return TBFS_ERR_INVALID;
@@ -216,6 +219,8 @@ int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
   }
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__); goto fail; }
+  rc = graph_build_bigc(g);
+  if (rc) goto fail;
   (void)hipStreamDestroy(s);
   *out = g;
   return TBFS_OK;
@@ -272,6 +277,10 @@ int graph_create_cooc(const int32_t* I, const int32_t* J, int32_t n, int32_t nz,
   }
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__); goto fail; }
+  if (sorted) {
+    rc = graph_build_bigc(g);
+    if (rc) goto fail;
+  }
   (void)hipStreamDestroy(s);
   *out = g;
   return TBFS_OK;
@@ -292,6 +301,27 @@ int graph_ensure_cooc(tbfs_graph* g, hipStream_t s) {
   return TBFS_OK;
 }
 
+// Static pull slices of the columns with in-degree > kBigIn.
+int graph_build_bigc(tbfs_graph* g) {
+  std::vector<int32_t> cp((size_t)g->n + 1);
+  HIPCHK(hipMemcpy(cp.data(), g->CP, cp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<uint64_t> big;
+  for (int32_t v = 0; v < g->n; ++v) {
+    const int64_t deg = (int64_t)cp[v + 1] - cp[v];
+    if (deg <= kBigIn) continue;
+    const int64_t nch = (deg + kPullChunk - 1) / kPullChunk;
+    for (int64_t c = 0; c < nch; ++c)
+      big.push_back((uint64_t)(uint32_t)v | ((uint64_t)c << 32));
+  }
+  g->nbigc = (int32_t)big.size();
+  int rc = dev_alloc(&g->bigc, big.size());
+  if (rc) return rc;
+  if (!big.empty())
+    HIPCHK(hipMemcpy(g->bigc, big.data(), big.size() * sizeof(uint64_t),
+                     hipMemcpyHostToDevice));
+  return TBFS_OK;
+}
+
 // ----------------------------------------------------------------------------
 // contexts
 void ctx_free(tbfs_ctx* c) {
@@ -303,7 +333,11 @@ void ctx_free(tbfs_ctx* c) {
   for (int i = 0; i < 3; ++i) dev_free(c->fb[i]);
   for (int i = 0; i < 2; ++i) dev_free(c->q[i]);
   dev_free(c->ctr);
+  dev_free(c->bar);
+  dev_free(c->stats);
   if (c->ctr_h) (void)hipHostFree(c->ctr_h);
+  if (c->stats_h) (void)hipHostFree(c->stats_h);
+  for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -318,7 +352,8 @@ int ctx_create(int32_t n_cap, tbfs_ctx** out) {
   c->n_cap = n_cap;
   const size_t nw = ((size_t)n_cap + 63) / 64;
   int rc = TBFS_OK;
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e != hipSuccess) { rc = hip_fail(e, "ctx stream/events", __FILE__, __LINE__); goto fail; }
@@ -326,11 +361,19 @@ int ctx_create(int32_t n_cap, tbfs_ctx** out) {
   if (!rc) rc = dev_alloc(&c->sigma, (size_t)n_cap);
   if (!rc) rc = dev_alloc(&c->ovf, (size_t)n_cap);
   for (int i = 0; i < 3 && !rc; ++i) rc = dev_alloc(&c->fb[i], nw);
-  for (int i = 0; i < 2 && !rc; ++i) rc = dev_alloc(&c->q[i], (size_t)n_cap);
   if (!rc) rc = dev_alloc(&c->ctr, 4);
+  if (!rc) rc = dev_alloc(&c->bar, 16);
+  if (!rc) rc = dev_alloc(&c->stats, 1);
   if (rc) goto fail;
   e = hipHostMalloc((void**)&c->ctr_h, 4 * sizeof(LevelCtr), hipHostMallocDefault);
   if (e != hipSuccess) { rc = hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); goto fail; }
+  c->stats_cap = 1;
+  e = hipHostMalloc((void**)&c->stats_h, sizeof(RunStats), hipHostMallocDefault);
+  if (e != hipSuccess) { rc = hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); goto fail; }
+  if (kern::persist_grid(c->device, &c->persist_grid_max) != hipSuccess) {
+    (void)hipGetLastError();
+    c->persist_grid_max = 0;  // no cooperative launch: host loop only
+  }
   *out = c;
   return TBFS_OK;
 fail:
@@ -338,58 +381,152 @@ fail:
   return rc;
 }
 
+// Queue capacity for graph g: one entry per kPushChunk slice of every
+// out-list in a level, bounded by n + nz / kPushChunk.
+static int ctx_reserve_queue(tbfs_ctx* c, const tbfs_graph* g) {
+  const int64_t need = (int64_t)g->n + (int64_t)g->nz / kPushChunk + 64;
+  if (c->q_cap >= need) return TBFS_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 2; ++i) dev_free(c->q[i]);
+  c->q_cap = 0;
+  for (int i = 0; i < 2; ++i) {
+    int rc = dev_alloc(&c->q[i], (size_t)need);
+    if (rc) return rc;
+  }
+  c->q_cap = need;
+  return TBFS_OK;
+}
+
+static int ctx_reserve_stats(tbfs_ctx* c, int count) {
+  if (c->stats_cap >= count) return TBFS_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  dev_free(c->stats);
+  if (c->stats_h) (void)hipHostFree(c->stats_h);
+  c->stats_h = nullptr;
+  c->stats_cap = 0;
+  int rc = dev_alloc(&c->stats, (size_t)count);
+  if (rc) return rc;
+  HIPCHK(hipHostMalloc((void**)&c->stats_h, (size_t)count * sizeof(RunStats),
+                       hipHostMallocDefault));
+  c->stats_cap = count;
+  return TBFS_OK;
+}
+
+static int ctx_reserve_events(tbfs_ctx* c, size_t count) {
+  while (c->evs.size() < count) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->evs.push_back(e);
+  }
+  return TBFS_OK;
+}
+
 // ----------------------------------------------------------------------------
-// the BFS: host-driven level loop
-int run_bfs(tbfs_ctx* c, const tbfs_graph* gc, int32_t root, const tbfs_opts& o,
-            tbfs_stats* st) {
-  tbfs_graph* g = const_cast<tbfs_graph*>(gc);
+// argument checks shared by both level loops
+static int check_run(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
+                     const tbfs_opts& o) {
   if (!c || !g) return TBFS_ERR_INVALID;
   if (g->n > c->n_cap) return TBFS_ERR_INVALID;
   if (root < 0 || root >= g->n) return TBFS_ERR_INVALID;
   if (o.mode < TBFS_MODE_AUTO || o.mode > TBFS_MODE_COOC) return TBFS_ERR_INVALID;
-  if (o.loop != TBFS_LOOP_HOST) return TBFS_ERR_UNSUPPORTED;
-  const bool csc_ok = (g->CP != nullptr);
-  if (o.mode != TBFS_MODE_COOC && !csc_ok) return TBFS_ERR_UNSUPPORTED;
+  if (o.pull < TBFS_PULL_SC || o.pull > TBFS_PULL_LB) return TBFS_ERR_INVALID;
+  if (o.loop != TBFS_LOOP_HOST && o.loop != TBFS_LOOP_PERSISTENT) return TBFS_ERR_INVALID;
+  if (o.mode != TBFS_MODE_COOC && !g->CP) return TBFS_ERR_UNSUPPORTED;
+  if (o.mode == TBFS_MODE_COOC && !o.sigma) return TBFS_ERR_UNSUPPORTED;
+  if (o.block != 0 && (o.block % 64 != 0 || o.block > 1024 || o.block < 64))
+    return TBFS_ERR_INVALID;
+  return TBFS_OK;
+}
+
+// The persistent loop applies when asked for and available; the COOC edge
+// sweep and the one-lane / one-wavefront reference mappings run host-driven.
+static bool use_persistent(const tbfs_ctx* c, const tbfs_opts& o) {
+  if (o.loop != TBFS_LOOP_PERSISTENT || c->persist_grid_max <= 0) return false;
+  if (o.mode == TBFS_MODE_COOC) return false;
+  if (o.mode == TBFS_MODE_PULL_SC || o.mode == TBFS_MODE_PULL_WA) return false;
+  return o.pull == TBFS_PULL_LB;
+}
+
+// Enqueue one device-resident BFS (no host synchronisation).
+static int enqueue_persistent(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
+                              const tbfs_opts& o, RunStats* stats_slot) {
+  PersistParams p;
+  p.g = GraphView{g->n, g->nz, g->CP, g->IC, g->RP, g->JR};
+  p.S = c->S;
+  p.sigma = c->sigma;
+  p.ovf = c->ovf;
+  for (int i = 0; i < 3; ++i) p.fb[i] = c->fb[i];
+  for (int i = 0; i < 2; ++i) p.q[i] = c->q[i];
+  p.ctr = c->ctr;
+  p.bar = c->bar;
+  p.stats = stats_slot;
+  p.bigc = g->bigc;
+  p.nbigc = g->nbigc;
+  p.root = root;
+  p.mode = o.mode;
+  p.sigma_on = o.sigma ? 1 : 0;
+  p.alpha = o.alpha > 0 ? o.alpha : 4.0f;
+  p.beta = o.beta > 0 ? o.beta : 24.0f;
+  int grid = c->persist_grid_max;
+  if (o.grid > 0 && o.grid < grid) grid = o.grid;
+  HIPCHK(hipMemsetAsync(c->bar, 0, 2 * sizeof(unsigned), c->stream));
+  HIPCHK(kern::persist(c->stream, grid, p));
+  return TBFS_OK;
+}
+
+static void fill_stats(tbfs_stats* st, const RunStats& rs, float ms) {
+  st->levels = rs.levels;
+  st->push_levels = rs.push_levels;
+  st->pull_levels = rs.pull_levels;
+  st->pad = 0;
+  st->reached = (int64_t)rs.reached;
+  st->edges = (int64_t)rs.edges;
+  st->ovf64 = rs.ovf_any ? -1 : 0;  // count known after tbfs_ctx_read
+  st->ms = (double)ms;
+}
+
+// ----------------------------------------------------------------------------
+// the BFS: host-driven level loop (one launch per level, counters read back)
+static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
+                         const tbfs_opts& o, tbfs_stats* st) {
   if (o.mode == TBFS_MODE_COOC) {
-    if (!o.sigma) return TBFS_ERR_UNSUPPORTED;  // COOC reaches vertices through counts
     int rc = graph_ensure_cooc(g, c->stream);
     if (rc) return rc;
   }
-  if (o.pull != TBFS_PULL_SC && o.pull != TBFS_PULL_WA) return TBFS_ERR_INVALID;
-  int block = o.block > 0 ? o.block : 256;
-  if (block % 64 != 0 || block > 1024) return TBFS_ERR_INVALID;
-
+  const bool csc_ok = (g->CP != nullptr);
+  const int block = o.block > 0 ? o.block : 256;
   const int32_t n = g->n;
   const int32_t nwords = (int32_t)(((int64_t)n + 63) / 64);
   hipStream_t s = c->stream;
   GraphView gv{n, g->nz, g->CP, g->IC, g->RP, g->JR};
   const int fill_grid = grid_for(n, 256, 8192);
+  const int lb_grid = grid_for((int64_t)nwords * 64, kLbBlock, 8192);
 
-  HIPCHK(hipEventRecord(c->ev0, s));
-  HIPCHK(kern::init(s, fill_grid, n, root, c->S, c->sigma, c->ovf, c->fb[0],
-                    c->fb[1], c->fb[2], c->q[0]));
-
-  // counters of the root's level (depth 0)
-  int64_t qsize = 1;
-  uint64_t m_frontier = 0, m_unexplored = (uint64_t)g->nz;
-  int64_t reached = 1;
-  uint64_t edges = 0;
+  // root degrees (the level-0 counters)
+  int32_t cp2[2] = {0, 0}, rp2[2] = {0, 0};
   if (csc_ok) {
-    int32_t cp2[2] = {0, 0}, rp2[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(cp2, g->CP + root, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(rp2, g->RP + root, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    edges = (uint64_t)(cp2[1] - cp2[0]);
-    m_frontier = (uint64_t)(rp2[1] - rp2[0]);
-    m_unexplored -= edges;
   }
+  HIPCHK(hipEventRecord(c->ev0, s));
+  HIPCHK(kern::init(s, fill_grid, n, root, rp2[1] - rp2[0], c->S, c->sigma,
+                    c->ovf, c->fb[0], c->fb[1], c->fb[2], c->q[0]));
+
+  int64_t qsize = 1;
+  int64_t qe = (rp2[1] - rp2[0] + kPushChunk - 1) / kPushChunk;
+  uint64_t edges = (uint64_t)(cp2[1] - cp2[0]);
+  uint64_t m_frontier = (uint64_t)(rp2[1] - rp2[0]);
+  uint64_t m_unexplored = (uint64_t)g->nz - edges;
+  int64_t reached = 1;
   uint32_t ovf_any = 0;
-  int cur = 0;  // fb[cur] / q[cur&1] hold depth d
+  int cur = 0;  // fb[cur] holds depth d
   int32_t d = 0;
   int push_levels = 0, pull_levels = 0;
   bool pushing = (o.mode == TBFS_MODE_PUSH || o.mode == TBFS_MODE_AUTO);
   const float alpha = o.alpha > 0 ? o.alpha : 4.0f;
   const float beta = o.beta > 0 ? o.beta : 24.0f;
+  const int sig = o.sigma ? 1 : 0;
 
   for (;;) {
     if (o.mode == TBFS_MODE_AUTO) {
@@ -401,7 +538,6 @@ int run_bfs(tbfs_ctx* c, const tbfs_graph* gc, int32_t root, const tbfs_opts& o,
                  c->q[d & 1], c->q[(d + 1) & 1], c->ctr};
     HIPCHK(kern::level_prologue(s, grid_for(nwords, 256, 4096), c->fb[nxt],
                                 nwords, c->ctr));
-    const int sig = o.sigma ? 1 : 0;
     switch (o.mode) {
       case TBFS_MODE_PULL_SC:
         HIPCHK(kern::pull_sc(s, grid_for(n, block, 65536), block, gv, sv, d,
@@ -421,16 +557,23 @@ int run_bfs(tbfs_ctx* c, const tbfs_graph* gc, int32_t root, const tbfs_opts& o,
         break;
       default:  // PUSH, or AUTO on its current side
         if (pushing) {
-          HIPCHK(kern::push(s, grid_for(qsize * 64, block, 65536), block, gv,
-                            sv, d, qsize, (int)ovf_any));
+          if (o.pull == TBFS_PULL_LB)
+            HIPCHK(kern::push_lb(s, grid_for(qe, kLbBlock, 8192), gv, sv, d, qe,
+                                 (int)ovf_any));
+          else
+            HIPCHK(kern::push(s, grid_for(qe * 64, block, 65536), block, gv, sv,
+                              d, qe, (int)ovf_any));
           ++push_levels;
-        } else if (o.pull == TBFS_PULL_WA) {
-          HIPCHK(kern::pull_wa(s, grid_for((int64_t)nwords * 64, block, 65536),
-                               block, gv, sv, d, sig, o.bbreak, (int)ovf_any));
-          ++pull_levels;
         } else {
-          HIPCHK(kern::pull_sc(s, grid_for(n, block, 65536), block, gv, sv, d,
-                               sig, o.bbreak, (int)ovf_any));
+          if (o.pull == TBFS_PULL_LB)
+            HIPCHK(kern::pull_lb(s, lb_grid, gv, sv, d, g->bigc, g->nbigc, sig,
+                                 (int)ovf_any));
+          else if (o.pull == TBFS_PULL_WA)
+            HIPCHK(kern::pull_wa(s, grid_for((int64_t)nwords * 64, block, 65536),
+                                 block, gv, sv, d, sig, o.bbreak, (int)ovf_any));
+          else
+            HIPCHK(kern::pull_sc(s, grid_for(n, block, 65536), block, gv, sv, d,
+                                 sig, o.bbreak, (int)ovf_any));
           ++pull_levels;
         }
         break;
@@ -441,6 +584,7 @@ int run_bfs(tbfs_ctx* c, const tbfs_graph* gc, int32_t root, const tbfs_opts& o,
     const LevelCtr lc = c->ctr_h[0];
     if (lc.q == 0) break;
     qsize = (int64_t)lc.q;
+    qe = (int64_t)lc.qe;
     reached += (int64_t)lc.q;
     edges += lc.min;
     m_frontier = lc.mout;
@@ -454,18 +598,63 @@ int run_bfs(tbfs_ctx* c, const tbfs_graph* gc, int32_t root, const tbfs_opts& o,
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_n = n;
-  c->last_ovf_any = ovf_any;
   if (st) {
-    st->levels = d + 1;
-    st->push_levels = push_levels;
-    st->pull_levels = pull_levels;
-    st->pad = 0;
-    st->reached = reached;
-    st->edges = (int64_t)edges;
-    st->ovf64 = -1;  // filled by ctx_read when the flags are copied back
-    st->ms = (double)ms;
+    RunStats rs{};
+    rs.levels = d + 1;
+    rs.push_levels = push_levels;
+    rs.pull_levels = pull_levels;
+    rs.ovf_any = ovf_any;
+    rs.reached = (unsigned long long)reached;
+    rs.edges = edges;
+    fill_stats(st, rs, ms);
   }
   return TBFS_OK;
+}
+
+int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
+                 int count, const tbfs_opts& o, tbfs_stats* st) {
+  tbfs_graph* g = const_cast<tbfs_graph*>(gc);
+  if (count < 1 || !roots) return TBFS_ERR_INVALID;
+  for (int i = 0; i < count; ++i) {
+    int rc = check_run(c, g, roots[i], o);
+    if (rc) return rc;
+  }
+  int rc = ctx_reserve_queue(c, g);
+  if (rc) return rc;
+  if (!use_persistent(c, o)) {
+    for (int i = 0; i < count; ++i) {
+      rc = run_host_loop(c, g, roots[i], o, st ? &st[i] : nullptr);
+      if (rc) return rc;
+    }
+    return TBFS_OK;
+  }
+  rc = ctx_reserve_stats(c, count);
+  if (!rc) rc = ctx_reserve_events(c, (size_t)count + 1);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  HIPCHK(hipEventRecord(c->evs[0], s));
+  for (int i = 0; i < count; ++i) {
+    rc = enqueue_persistent(c, g, roots[i], o, c->stats + i);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c->evs[i + 1], s));
+  }
+  HIPCHK(hipMemcpyAsync(c->stats_h, c->stats, (size_t)count * sizeof(RunStats),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  c->last_n = g->n;
+  if (st) {
+    for (int i = 0; i < count; ++i) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, c->evs[i], c->evs[i + 1]));
+      fill_stats(&st[i], c->stats_h[i], ms);
+    }
+  }
+  return TBFS_OK;
+}
+
+int run_bfs(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs_opts& o,
+            tbfs_stats* st) {
+  return run_bfs_many(c, g, &root, 1, o, st);
 }
 
 int ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma, uint8_t* ovf) {
@@ -493,7 +682,8 @@ void tbfs_opts_default(tbfs_opts* o) {
   o->bbreak = 0;
   o->alpha = 4.0f;
   o->beta = 24.0f;
-  o->pull = TBFS_PULL_SC;
+  o->pull = TBFS_PULL_LB;
+  o->loop = TBFS_LOOP_PERSISTENT;
 }
 
 const char* tbfs_status_str(int s) {
@@ -550,6 +740,13 @@ int tbfs_bfs(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   tbfs_opts def;
   tbfs_opts_default(&def);
   return run_bfs(c, g, root, o ? *o : def, st);
+}
+
+int tbfs_bfs_many(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots,
+                  int32_t count, const tbfs_opts* o, tbfs_stats* st) {
+  tbfs_opts def;
+  tbfs_opts_default(&def);
+  return run_bfs_many(c, g, roots, count, o ? *o : def, st);
 }
 
 int tbfs_ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma,

@@ -4,13 +4,18 @@
 //   CP int32[n+1], IC int32[nz]   in-lists (CSC): column v lists the u with u->v
 //   RP int32[n+1], JR int32[nz]   out-lists (CSR = transpose); alias CP/IC when
 //                                 the graph is symmetric
+//   bigc uint64[nbigc]            static pull work list: one entry per
+//                                 kPullChunk-entry slice of every column whose
+//                                 in-degree exceeds kBigIn (v | slice << 32)
 //   S int32[n]                    depth, -1 = not reached
 //   sigma uint64[n], ovf uint8[n] shortest-path count mod 2^64 + sticky
 //                                 "count >= 2^64" flag
 //   fb[3] uint64[ceil(n/64)]      frontier bitmaps (bit v&63 of word v>>6);
 //                                 current / next / being cleared, rotating
-//   q[2] int32[n]                 frontier queues (current / next)
-//   LevelCtr                      per-level counters written by the kernels
+//   q[2] uint64[q_cap]            frontier work queues (current / next): one
+//                                 entry per kPushChunk-entry slice of a
+//                                 frontier vertex's out-list (v | slice << 32)
+//   LevelCtr[3]                   per-level counters, rotating
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -18,16 +23,31 @@
 namespace tbfs {
 
 constexpr int kWave = 64;
+constexpr int kPushChunk = 64;    // out-list entries per queue entry
+constexpr int kBigIn = 1024;      // in-degree above which a column is split
+constexpr int kPullChunk = 1024;  // in-list entries per static pull slice
 
-// Counters of the vertices discovered in one level.  All fields are updated
-// with device-scope atomics, one atomic per wavefront.
+// Counters of the vertices discovered in one level.  Every field is updated
+// with one device-scope atomic per wavefront.
 struct alignas(64) LevelCtr {
-  unsigned long long q;     // vertices discovered (= next queue length)
+  unsigned long long q;     // vertices discovered
+  unsigned long long qe;    // queue entries written for them
   unsigned long long mout;  // sum of their out-degrees (push work next level)
   unsigned long long min;   // sum of their in-degrees (leaves the unexplored set)
   unsigned int ovf;         // 1 if any of them has a count >= 2^64
   unsigned int pad0;
-  unsigned long long pad1[4];
+  unsigned long long pad1[3];
+};
+
+// Result counters of one BFS, written by the device-resident level loop.
+struct alignas(64) RunStats {
+  int32_t levels;
+  int32_t push_levels;
+  int32_t pull_levels;
+  uint32_t ovf_any;
+  unsigned long long reached;
+  unsigned long long edges;
+  unsigned long long pad[4];
 };
 
 struct GraphView {
@@ -45,17 +65,39 @@ struct StateView {
   uint8_t* ovf;
   const uint64_t* fcur;  // frontier of depth d
   uint64_t* fnext;       // frontier of depth d+1 (must be zero on entry)
-  const int32_t* qcur;
-  int32_t* qnext;
-  LevelCtr* ctr;         // counters of depth d+1
+  const uint64_t* qcur;  // queue of depth d
+  uint64_t* qnext;       // queue of depth d+1
+  LevelCtr* ctr;         // counters of depth d+1 (must be zero on entry)
 };
 
-// Kernel launch wrappers (defined in tbfs_kernels.hip; every launch is made
-// in the translation unit that defines the kernel).
+// Everything the device-resident level loop needs.
+struct PersistParams {
+  GraphView g;
+  int32_t* S;
+  uint64_t* sigma;
+  uint8_t* ovf;
+  uint64_t* fb[3];
+  uint64_t* q[2];
+  LevelCtr* ctr;    // [3]
+  unsigned* bar;    // [2]: arrival count, generation (zero at launch)
+  RunStats* stats;
+  const uint64_t* bigc;
+  int32_t nbigc;
+  int32_t root;
+  int32_t mode;     // TBFS_MODE_AUTO / _PUSH / _PULL_SC (all pulls are load-balanced)
+  int32_t sigma_on;
+  float alpha;
+  float beta;
+};
+
+// Kernel launch wrappers.  Every launch is made in the translation unit that
+// defines the kernel.
 namespace kern {
-hipError_t init(hipStream_t s, int grid, int32_t n, int32_t root, int32_t* S,
-                uint64_t* sigma, uint8_t* ovf, uint64_t* fb0, uint64_t* fb1,
-                uint64_t* fb2, int32_t* q0);
+// tbfs_kernels.hip: per-level kernels of the host-driven loop
+hipError_t init(hipStream_t s, int grid, int32_t n, int32_t root,
+                int32_t root_out_deg, int32_t* S, uint64_t* sigma,
+                uint8_t* ovf, uint64_t* fb0, uint64_t* fb1, uint64_t* fb2,
+                uint64_t* q0);
 hipError_t level_prologue(hipStream_t s, int grid, uint64_t* fnext,
                           int32_t nwords, LevelCtr* ctr);
 hipError_t pull_sc(hipStream_t s, int grid, int block, const GraphView& g,
@@ -78,6 +120,19 @@ hipError_t histogram(hipStream_t s, const int32_t* keys, int32_t nz,
                      int32_t* cnt);
 hipError_t check_csc(hipStream_t s, const int32_t* CP, const int32_t* IC,
                      int32_t n, int32_t nz, int32_t* bad);
+
+// tbfs_persist.hip: load-balanced levels and the device-resident level loop
+hipError_t push_lb(hipStream_t s, int grid, const GraphView& g,
+                   const StateView& st, int32_t d, int64_t qe, int ovf_any);
+hipError_t pull_lb(hipStream_t s, int grid, const GraphView& g,
+                   const StateView& st, int32_t d, const uint64_t* bigc,
+                   int32_t nbigc, int sigma, int ovf_any);
+// Largest cooperative grid for the persistent kernel (blocks of kPersistBlock).
+hipError_t persist_grid(int device, int* grid);
+hipError_t persist(hipStream_t s, int grid, const PersistParams& p);
 }  // namespace kern
+
+constexpr int kLbBlock = 256;       // threads per block of the load-balanced kernels
+constexpr int kPersistBlock = 256;  // threads per block of the persistent kernel
 
 }  // namespace tbfs

@@ -1,4 +1,4 @@
-// tbfs_kernels.hip -- gfx950 level kernels of the single-source BFS.
+// tbfs_kernels.hip -- gfx950 per-level kernels of the host-driven BFS loop.
 //
 // All kernels are integer work (no MFMA): depth S, path count sigma
 // (uint64 mod 2^64 + sticky overflow flag), frontier bitmaps and queues.
@@ -12,97 +12,27 @@
 //             off, bfsBuScKernel (TurboBFS_CSC_BU/bfsgpubucsc_sc.cu:122-148)
 //   pull_wa   spMvUgCscWaKernel (TurboBFS_CSC_TD/bfsgputdcsc_wa.cu:159-219)
 //             and bfsBuWaKernel (TurboBFS_CSC_BU/bfsgpubucsc_wa.cu:121-180):
-//             one wavefront per column, ballot early exit, DPP/shuffle sum
+//             one wavefront per column, ballot early exit, butterfly sum
 //   cooc_*    spMvUgCoocScKernel + bfsFunctionsKernel
 //             (TurboBFS_COOC_TD/bfsgputdcooc_sc.cu:152-188)
-//   push      top-down expansion from a frontier queue (the TD half of the
+//   push      top-down expansion from the frontier queue (the TD half of the
 //             direction-optimizing switch, TurboBFS_CSC_TDBU/bfsgputdbucsc_sc.cu:119-153)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tbfs_device.h"
 #include "tbfs_internal.h"
 
 namespace tbfs {
 namespace {
 
-__device__ __forceinline__ int lane() { return __lane_id(); }
-
-__device__ __forceinline__ uint64_t lanemask_lt() {
-  return (1ull << lane()) - 1ull;
-}
-
-__device__ __forceinline__ bool fbit(const uint64_t* __restrict__ fb, int u) {
-  return (fb[u >> 6] >> (u & 63)) & 1ull;
-}
-
-// s += x with the carry folded into the sticky flag o.
-__device__ __forceinline__ void add_carry(uint64_t& s, uint32_t& o, uint64_t x) {
-  const uint64_t t = s + x;
-  o |= (t < s) ? 1u : 0u;
-  s = t;
-}
-
-// Butterfly sum of (value, flag) pairs over the 64 lanes; every lane ends with
-// the total.  All lanes must be converged.
-__device__ __forceinline__ void wave_sum_carry(uint64_t& s, uint32_t& o) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint64_t t = __shfl_xor(s, off);
-    const uint32_t to = __shfl_xor(o, off);
-    const uint64_t r = s + t;
-    o = o | to | ((r < s) ? 1u : 0u);
-    s = r;
-  }
-}
-
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-  return x;
-}
-
-// Reserve one slot per lane with pred set in the queue counted by *ctr.
-// Returns this lane's slot (undefined for lanes without pred).
-__device__ __forceinline__ unsigned long long wave_append(
-    unsigned long long* ctr, bool pred) {
-  const uint64_t m = __ballot(pred);
-  unsigned long long base = 0;
-  if (m) {
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    if (lane() == leader)
-      base = atomicAdd(ctr, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader);
-  }
-  return base + (unsigned long long)__popcll(m & lanemask_lt());
-}
-
-// Publish the vertices this wavefront discovered: queue slots, level counters.
-// `found` marks a discovered vertex v on this lane; od/id are its out/in
-// degrees, ov its overflow flag.  All lanes must be converged.
-__device__ __forceinline__ void emit(const StateView& st, int v, bool found,
-                                     uint64_t od, uint64_t id, uint32_t ov) {
-  const uint64_t m = __ballot(found);
-  if (m == 0) return;
-  const unsigned long long pos = wave_append(&st.ctr->q, found);
-  if (found) st.qnext[pos] = v;
-  const uint64_t sod = wave_sum_u64(found ? od : 0);
-  const uint64_t sid = wave_sum_u64(found ? id : 0);
-  const bool anyov = __ballot(found && ov) != 0;
-  if (lane() == 0) {
-    atomicAdd(&st.ctr->mout, (unsigned long long)sod);
-    atomicAdd(&st.ctr->min, (unsigned long long)sid);
-    if (anyov) atomicOr(&st.ctr->ovf, 1u);
-  }
-}
-
-__device__ __forceinline__ uint64_t outdeg(const GraphView& g, int v) {
-  return (uint64_t)(g.RP[v + 1] - g.RP[v]);
-}
+using namespace dev;
 
 // ----------------------------------------------------------------------------
-__global__ void k_init(int32_t n, int32_t root, int32_t* S, uint64_t* sigma,
-                       uint8_t* ovf, uint64_t* fb0, uint64_t* fb1,
-                       uint64_t* fb2, int32_t* q0) {
+__global__ void k_init(int32_t n, int32_t root, int32_t root_out_deg,
+                       int32_t* S, uint64_t* sigma, uint8_t* ovf,
+                       uint64_t* fb0, uint64_t* fb1, uint64_t* fb2,
+                       uint64_t* q0) {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = tid; i < n; i += stride) {
@@ -116,7 +46,9 @@ __global__ void k_init(int32_t n, int32_t root, int32_t* S, uint64_t* sigma,
     fb1[w] = 0ull;
     fb2[w] = 0ull;
   }
-  if (tid == 0) q0[0] = root;
+  const int64_t nch = ((int64_t)root_out_deg + kPushChunk - 1) / kPushChunk;
+  for (int64_t c = tid; c < nch; c += stride)
+    q0[c] = (uint64_t)(uint32_t)root | ((uint64_t)c << 32);
 }
 
 __global__ void k_level_prologue(uint64_t* fnext, int32_t nwords,
@@ -126,6 +58,7 @@ __global__ void k_level_prologue(uint64_t* fnext, int32_t nwords,
   for (int64_t w = tid; w < nwords; w += stride) fnext[w] = 0ull;
   if (tid == 0) {
     ctr->q = 0;
+    ctr->qe = 0;
     ctr->mout = 0;
     ctr->min = 0;
     ctr->ovf = 0;
@@ -147,10 +80,8 @@ __global__ __launch_bounds__(256) void k_pull_sc(GraphView g, StateView st,
     bool found = false;
     uint64_t sum = 0;
     uint32_t ov = 0;
-    uint64_t id = 0;
     if (v < g.n && st.S[v] == -1) {
       const int beg = g.CP[v], end = g.CP[v + 1];
-      id = (uint64_t)(end - beg);
       for (int k = beg; k < end; ++k) {
         const int u = g.IC[k];
         if (fbit(st.fcur, u)) {
@@ -174,7 +105,7 @@ __global__ __launch_bounds__(256) void k_pull_sc(GraphView g, StateView st,
     const uint64_t m = __ballot(found);
     if (m) {
       if (lane() == 0) st.fnext[base >> 6] = m;
-      emit(st, v, found, found ? outdeg(g, v) : 0, id, ov);
+      emit(st, v, found, found ? outdeg(g, v) : 0u, found ? indeg(g, v) : 0u, ov);
     }
   }
 }
@@ -234,57 +165,55 @@ __global__ __launch_bounds__(256) void k_pull_wa(GraphView g, StateView st,
       if (lane() == 0) st.fnext[base >> 6] = word;
       const int v = (int)base + lane();
       const bool found = (word >> lane()) & 1ull;
-      emit(st, v, found, found ? outdeg(g, v) : 0,
-           found ? (uint64_t)(g.CP[v + 1] - g.CP[v]) : 0, myov);
+      emit(st, v, found, found ? outdeg(g, v) : 0u, found ? indeg(g, v) : 0u, myov);
     }
   }
 }
 
-// Top-down expansion from the frontier queue: one wavefront per frontier
-// vertex u, lanes stride u's out-list.  The first lane to move S[v] from -1
-// to d+1 enqueues v; every edge into a vertex of depth d+1 adds sigma[u]
-// with a 64-bit atomic whose returned old value exposes the carry.
+// Top-down expansion from the frontier queue: one wavefront per queue entry
+// (a slice of at most 64 out-list entries of frontier vertex u), one lane per
+// edge.  The first lane to move S[v] from -1 to d+1 enqueues v; every edge
+// into a vertex of depth d+1 adds sigma[u] with a 64-bit atomic whose
+// returned old value exposes the carry.
 __global__ __launch_bounds__(256) void k_push(GraphView g, StateView st,
-                                              int32_t d, int64_t qsize,
+                                              int32_t d, int64_t qe,
                                               int ovf_any) {
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int dn = d + 1;
-  for (int64_t i = wave; i < qsize; i += nwaves) {
-    const int u = st.qcur[i];
+  for (int64_t i = wave; i < qe; i += nwaves) {
+    const uint64_t ent = st.qcur[i];
+    const int u = (int)(uint32_t)ent;
+    const int beg = g.RP[u] + (int)(ent >> 32) * kPushChunk;
+    const int end = min(g.RP[u + 1], beg + kPushChunk);
     const uint64_t x = st.sigma[u];
     const uint32_t ou = ovf_any ? (uint32_t)st.ovf[u] : 0u;
-    const int beg = g.RP[u], end = g.RP[u + 1];
-    for (int k0 = beg; k0 < end; k0 += 64) {
-      const int k = k0 + lane();
-      bool nw = false;
-      int v = 0;
-      uint32_t ov = 0;
-      if (k < end) {
-        v = g.JR[k];
-        int s = st.S[v];
-        if (s == -1) {
-          const int old = atomicCAS(&st.S[v], -1, dn);
-          nw = (old == -1);
-          s = dn;  // old is -1 or dn: only this level writes S now
-        }
-        if (s == dn) {
-          const unsigned long long old =
-              atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)x);
-          if (old + x < old || ou) {
-            st.ovf[v] = 1;
-            ov = 1;
-          }
-        }
-        if (nw) atomicOr((unsigned long long*)&st.fnext[v >> 6], 1ull << (v & 63));
+    const int k = beg + lane();
+    bool nw = false;
+    int v = 0;
+    uint32_t ov = 0;
+    if (k < end) {
+      v = g.JR[k];
+      int s = st.S[v];
+      if (s == -1) {
+        const int old = atomicCAS(&st.S[v], -1, dn);
+        nw = (old == -1);
+        s = dn;  // old is -1 or dn: only this level writes S now
       }
-      const uint64_t m = __ballot(nw || ov);
-      if (m) {
-        emit(st, v, nw, nw ? outdeg(g, v) : 0,
-             nw ? (uint64_t)(g.CP[v + 1] - g.CP[v]) : 0, ov);
-        // an overflow on an edge into an already-claimed vertex
-        if (__ballot(ov && !nw) && lane() == 0) atomicOr(&st.ctr->ovf, 1u);
+      if (s == dn) {
+        const unsigned long long old =
+            atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)x);
+        if (old + x < old || ou) {
+          st.ovf[v] = 1;
+          ov = 1;
+        }
       }
+      if (nw) atomicOr((unsigned long long*)&st.fnext[v >> 6], 1ull << (v & 63));
+    }
+    if (__ballot(nw || ov)) {
+      emit(st, v, nw, nw ? outdeg(g, v) : 0u, nw ? indeg(g, v) : 0u, ov);
+      // an overflow on an edge into a vertex claimed by another lane
+      if (__ballot(ov && !nw) && lane() == 0) atomicOr(&st.ctr->ovf, 1u);
     }
   }
 }
@@ -327,10 +256,11 @@ __global__ __launch_bounds__(256) void k_cooc_finish(GraphView g, StateView st,
     const uint64_t m = __ballot(found);
     if (m) {
       if (lane() == 0) st.fnext[base >> 6] = m;
-      uint64_t od = 0, id = 0;
+      uint32_t od = 0, id = 0;
       if (found && g.RP) od = outdeg(g, v);
-      if (found && g.CP) id = (uint64_t)(g.CP[v + 1] - g.CP[v]);
-      emit(st, v, found, od, id, ov);
+      if (found && g.CP) id = indeg(g, v);
+      // without out-lists (unsorted COOC) there is no push queue to fill
+      emit(st, v, found, g.RP ? od : 0u, id, ov);
     }
   }
 }
@@ -372,11 +302,12 @@ __global__ void k_check_csc(const int32_t* CP, const int32_t* IC, int32_t n,
 // ----------------------------------------------------------------------------
 namespace kern {
 
-hipError_t init(hipStream_t s, int grid, int32_t n, int32_t root, int32_t* S,
-                uint64_t* sigma, uint8_t* ovf, uint64_t* fb0, uint64_t* fb1,
-                uint64_t* fb2, int32_t* q0) {
-  hipLaunchKernelGGL(k_init, dim3(grid), dim3(256), 0, s, n, root, S, sigma,
-                     ovf, fb0, fb1, fb2, q0);
+hipError_t init(hipStream_t s, int grid, int32_t n, int32_t root,
+                int32_t root_out_deg, int32_t* S, uint64_t* sigma,
+                uint8_t* ovf, uint64_t* fb0, uint64_t* fb1, uint64_t* fb2,
+                uint64_t* q0) {
+  hipLaunchKernelGGL(k_init, dim3(grid), dim3(256), 0, s, n, root,
+                     root_out_deg, S, sigma, ovf, fb0, fb1, fb2, q0);
   return hipGetLastError();
 }
 
@@ -412,8 +343,8 @@ hipError_t pull_wa(hipStream_t s, int grid, int block, const GraphView& g,
 }
 
 hipError_t push(hipStream_t s, int grid, int block, const GraphView& g,
-                const StateView& st, int32_t d, int64_t qsize, int ovf_any) {
-  hipLaunchKernelGGL(k_push, dim3(grid), dim3(block), 0, s, g, st, d, qsize,
+                const StateView& st, int32_t d, int64_t qe, int ovf_any) {
+  hipLaunchKernelGGL(k_push, dim3(grid), dim3(block), 0, s, g, st, d, qe,
                      ovf_any);
   return hipGetLastError();
 }

@@ -1,0 +1,398 @@
+// tbfs_persist.hip -- load-balanced BFS levels and the device-resident level
+// loop (one cooperative launch per BFS, grid barrier between levels).
+//
+// Direction-optimizing BFS with exact path counts (the reference's TD/BU
+// switch, TurboBFS_CSC_TDBU/bfsgputdbucsc_sc.cu:108-153, redone for wave64):
+//   push level  expands the frontier queue.  A wavefront takes 64 queue
+//               entries (slices of <= kPushChunk out-list entries), scans their
+//               lengths and walks the concatenated edges 64 at a time, so hub
+//               vertices spread over many wavefronts and small ones share one.
+//               Discovery is a CAS on S; counts are 64-bit atomics whose old
+//               value exposes the carry.
+//   pull level  sweeps the unvisited columns.  A wavefront takes the 64
+//               columns of one frontier word, scans their in-degrees and walks
+//               the concatenated in-lists; parents' counts are summed per
+//               column in LDS.  Columns with in-degree > kBigIn are instead cut
+//               into static kPullChunk slices (graph property, computed once)
+//               that many wavefronts reduce with global atomics.  No early exit:
+//               a column's count needs all of its parents.
+// The level loop keeps every decision on the device: each workgroup reads the
+// same level counters after the grid barrier and takes the same branch.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tbfs.h"
+#include "tbfs_device.h"
+#include "tbfs_internal.h"
+
+namespace tbfs {
+namespace {
+
+using namespace dev;
+
+struct WaveLds {
+  int32_t excl[64];  // exclusive prefix of per-lane work
+  int32_t beg[64];   // first edge index of each lane's slice
+  uint64_t x[64];    // push: count of the slice's vertex; pull: per-column sums
+};
+
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ void push_lb_level(const GraphView& g,
+                                              const StateView& st, int dn,
+                                              int64_t qe, uint32_t ovf_any,
+                                              int64_t wave, int64_t nwaves,
+                                              WaveLds* L) {
+  for (int64_t b = wave * 64; b < qe; b += nwaves * 64) {
+    const int64_t idx = b + lane();
+    int beg = 0, deg = 0;
+    uint64_t x = 0;
+    uint32_t ou = 0;
+    if (idx < qe) {
+      const uint64_t ent = st.qcur[idx];
+      const int u = (int)(uint32_t)ent;
+      const int re = g.RP[u + 1];
+      beg = g.RP[u] + (int)(ent >> 32) * kPushChunk;
+      deg = min(re, beg + kPushChunk) - beg;
+      x = st.sigma[u];
+      if (ovf_any) ou = st.ovf[u];
+    }
+    const uint32_t incl = wave_incl_scan((uint32_t)deg);
+    const int total = (int)__shfl(incl, 63);
+    const uint64_t oumask = __ballot(ou != 0);
+    wave_lds_sync();
+    L->excl[lane()] = (int)incl - deg;
+    L->beg[lane()] = beg;
+    L->x[lane()] = x;
+    wave_lds_sync();
+    for (int e0 = 0; e0 < total; e0 += 64) {
+      const int e = e0 + lane();
+      bool nw = false;
+      int v = 0;
+      uint32_t ov = 0;
+      if (e < total) {
+        const int o = owner_of(L->excl, e);
+        const int k = L->beg[o] + (e - L->excl[o]);
+        v = g.JR[k];
+        int s = st.S[v];
+        if (s == -1) {
+          const int old = atomicCAS(&st.S[v], -1, dn);
+          nw = (old == -1);
+          s = dn;
+        }
+        if (s == dn) {
+          const uint64_t xo = L->x[o];
+          const unsigned long long old =
+              atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)xo);
+          if (old + xo < old || ((oumask >> o) & 1ull)) {
+            st.ovf[v] = 1;
+            ov = 1;
+          }
+        }
+        if (nw) atomicOr((unsigned long long*)&st.fnext[v >> 6], 1ull << (v & 63));
+      }
+      if (__ballot(nw || ov)) {
+        emit(st, v, nw, nw ? outdeg(g, v) : 0u, nw ? indeg(g, v) : 0u, ov);
+        if (__ballot(ov && !nw) && lane() == 0) atomicOr(&st.ctr->ovf, 1u);
+      }
+    }
+  }
+}
+
+template <bool SIGMA>
+__device__ __forceinline__ void pull_lb_level(const GraphView& g,
+                                              const StateView& st, int dn,
+                                              uint32_t ovf_any,
+                                              const uint64_t* bigc,
+                                              int32_t nbigc, int64_t wave,
+                                              int64_t nwaves, WaveLds* L) {
+  // (a) slices of the high in-degree columns
+  for (int64_t i = wave; i < nbigc; i += nwaves) {
+    const uint64_t ent = bigc[i];
+    const int v = (int)(uint32_t)ent;
+    const int sv = st.S[v];
+    if (sv != -1 && sv != dn) continue;  // reached before this level
+    const int cb = g.CP[v] + (int)(ent >> 32) * kPullChunk;
+    const int ce = min(g.CP[v + 1], cb + kPullChunk);
+    bool f = false;
+    uint64_t sum = 0;
+    uint32_t ov = 0;
+    for (int k = cb + lane(); k < ce; k += 64) {
+      const int u = g.IC[k];
+      if (fbit(st.fcur, u)) {
+        f = true;
+        if (SIGMA) {
+          add_carry(sum, ov, st.sigma[u]);
+          if (ovf_any) ov |= st.ovf[u];
+        }
+      }
+    }
+    if (__ballot(f)) {
+      if (SIGMA) wave_sum_carry(sum, ov);
+      bool first = false;
+      if (lane() == 0) {
+        if (SIGMA) {
+          const unsigned long long old =
+              atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)sum);
+          if (old + sum < old) ov = 1;
+          if (ov) {
+            st.ovf[v] = 1;
+            atomicOr(&st.ctr->ovf, 1u);
+          }
+        }
+        const unsigned long long bit = 1ull << (v & 63);
+        const unsigned long long old =
+            atomicOr((unsigned long long*)&st.fnext[v >> 6], bit);
+        first = (old & bit) == 0;
+        if (first) st.S[v] = dn;
+      }
+      emit(st, v, first, first ? outdeg(g, v) : 0u, first ? indeg(g, v) : 0u, 0u);
+    }
+  }
+  // (b) the other columns, 64 consecutive ones per wavefront
+  const int64_t nwords = ((int64_t)g.n + 63) >> 6;
+  for (int64_t w = wave; w < nwords; w += nwaves) {
+    const int v = (int)(w * 64) + lane();
+    int beg = 0, deg = 0;
+    if (v < g.n && st.S[v] == -1) {
+      beg = g.CP[v];
+      const int dg = g.CP[v + 1] - beg;
+      deg = dg > kBigIn ? 0 : dg;
+    }
+    const uint32_t incl = wave_incl_scan((uint32_t)deg);
+    const int total = (int)__shfl(incl, 63);
+    if (total == 0) continue;
+    wave_lds_sync();
+    L->excl[lane()] = (int)incl - deg;
+    L->beg[lane()] = beg;
+    if (SIGMA) L->x[lane()] = 0;
+    wave_lds_sync();
+    uint64_t fmask = 0, omask = 0;
+    for (int e = lane(); e < total; e += 64) {
+      const int o = owner_of(L->excl, e);
+      const int k = L->beg[o] + (e - L->excl[o]);
+      const int u = g.IC[k];
+      if (fbit(st.fcur, u)) {
+        fmask |= 1ull << o;
+        if (SIGMA) {
+          const uint64_t xu = st.sigma[u];
+          const unsigned long long old =
+              atomicAdd((unsigned long long*)&L->x[o], (unsigned long long)xu);
+          if (old + xu < old || (ovf_any && st.ovf[u])) omask |= 1ull << o;
+        }
+      }
+    }
+    fmask = wave_or_u64(fmask);
+    if (SIGMA) omask = wave_or_u64(omask);
+    wave_lds_sync();
+    const bool found = (fmask >> lane()) & 1ull;
+    uint32_t ov = 0;
+    if (found) {
+      st.S[v] = dn;
+      if (SIGMA) {
+        st.sigma[v] = L->x[lane()];
+        ov = (uint32_t)((omask >> lane()) & 1ull);
+        if (ov) st.ovf[v] = 1;
+      }
+    }
+    if (fmask) {
+      if (lane() == 0) atomicOr((unsigned long long*)&st.fnext[w], fmask);
+      emit(st, v, found, found ? outdeg(g, v) : 0u, found ? indeg(g, v) : 0u, ov);
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// standalone level kernels (host-driven loop)
+__global__ __launch_bounds__(kLbBlock) void k_push_lb(GraphView g, StateView st,
+                                                      int32_t d, int64_t qe,
+                                                      int ovf_any) {
+  __shared__ WaveLds lds[kLbBlock / 64];
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  push_lb_level(g, st, d + 1, qe, (uint32_t)ovf_any, wave, nwaves,
+                &lds[threadIdx.x >> 6]);
+}
+
+template <bool SIGMA>
+__global__ __launch_bounds__(kLbBlock) void k_pull_lb(GraphView g, StateView st,
+                                                      int32_t d,
+                                                      const uint64_t* bigc,
+                                                      int32_t nbigc,
+                                                      int ovf_any) {
+  __shared__ WaveLds lds[kLbBlock / 64];
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  pull_lb_level<SIGMA>(g, st, d + 1, (uint32_t)ovf_any, bigc, nbigc, wave,
+                       nwaves, &lds[threadIdx.x >> 6]);
+}
+
+// ----------------------------------------------------------------------------
+// device-resident level loop
+__device__ __forceinline__ unsigned long long ld_agent(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_agent(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Barrier number k (1, 2, ...) over all workgroups of a cooperative launch.
+// bar[0] counts arrivals (k * gridDim.x after barrier k), bar[1] is the number
+// of the last completed barrier.  Every wave makes its global writes visible
+// device-wide before arriving and picks up the others' after leaving.
+__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned k) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned target = k * gridDim.x;
+    const unsigned old = atomicAdd(&bar[0], 1u);
+    if (old + 1 == target) {
+      __hip_atomic_store(&bar[1], k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < k)
+        __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+  __threadfence();
+}
+
+__global__ __launch_bounds__(kPersistBlock) void k_persist(PersistParams p) {
+  __shared__ WaveLds lds[kPersistBlock / 64];
+  WaveLds* L = &lds[threadIdx.x >> 6];
+  const GraphView& g = p.g;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t wave = gtid >> 6, nwaves = gstride >> 6;
+  const int32_t n = g.n;
+  const int64_t nwords = ((int64_t)n + 63) >> 6;
+  const int r = p.root;
+  const uint32_t rout = outdeg(g, r), rin = indeg(g, r);
+
+  // phase 0: state of depth 0
+  for (int64_t i = gtid; i < n; i += gstride) {
+    p.S[i] = (i == r) ? 0 : -1;
+    p.sigma[i] = (i == r) ? 1ull : 0ull;
+    p.ovf[i] = 0;
+  }
+  for (int64_t w = gtid; w < nwords; w += gstride) {
+    p.fb[0][w] = (w == (r >> 6)) ? (1ull << (r & 63)) : 0ull;
+    p.fb[1][w] = 0ull;
+    p.fb[2][w] = 0ull;
+  }
+  const int64_t rch = ((int64_t)rout + kPushChunk - 1) / kPushChunk;
+  for (int64_t c = gtid; c < rch; c += gstride)
+    p.q[0][c] = (uint64_t)(uint32_t)r | ((uint64_t)c << 32);
+  if (gtid < 3) {
+    LevelCtr* c = &p.ctr[gtid];
+    c->q = gtid == 0 ? 1ull : 0ull;
+    c->qe = gtid == 0 ? (unsigned long long)rch : 0ull;
+    c->mout = gtid == 0 ? rout : 0ull;
+    c->min = gtid == 0 ? rin : 0ull;
+    c->ovf = 0;
+  }
+  unsigned bk = 1;
+  grid_barrier(p.bar, bk);
+
+  uint32_t ovf_any = 0;
+  bool pushing = (p.mode == TBFS_MODE_AUTO || p.mode == TBFS_MODE_PUSH);
+  unsigned long long mu = (unsigned long long)g.nz;
+  unsigned long long reached = 0, edges = 0;
+  int push_levels = 0, pull_levels = 0;
+  int d = 0;
+  for (;; ++d) {
+    const LevelCtr* c = &p.ctr[d % 3];
+    const unsigned long long q = ld_agent(&c->q);
+    if (q == 0) break;
+    const unsigned long long qe = ld_agent(&c->qe);
+    const unsigned long long mf = ld_agent(&c->mout);
+    const unsigned long long mi = ld_agent(&c->min);
+    ovf_any |= ld_agent(&c->ovf);
+    reached += q;
+    edges += mi;
+    mu = mu > mi ? mu - mi : 0ull;
+    if (p.mode == TBFS_MODE_AUTO) {
+      if (pushing && (double)mf * p.alpha > (double)mu) pushing = false;
+      else if (!pushing && (double)q * p.beta < (double)n) pushing = true;
+    }
+    // buffers of depth d+2 were last used by depth d-1: clear them now
+    uint64_t* fclr = p.fb[(d + 2) % 3];
+    for (int64_t w = gtid; w < nwords; w += gstride) fclr[w] = 0ull;
+    if (gtid == 0) {
+      LevelCtr* z = &p.ctr[(d + 2) % 3];
+      z->q = 0;
+      z->qe = 0;
+      z->mout = 0;
+      z->min = 0;
+      z->ovf = 0;
+    }
+    StateView st{p.S, p.sigma, p.ovf, p.fb[d % 3], p.fb[(d + 1) % 3],
+                 p.q[d & 1], p.q[(d + 1) & 1], &p.ctr[(d + 1) % 3]};
+    if (pushing) {
+      push_lb_level(g, st, d + 1, (int64_t)qe, ovf_any, wave, nwaves, L);
+      ++push_levels;
+    } else {
+      if (p.sigma_on)
+        pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L);
+      else
+        pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L);
+      ++pull_levels;
+    }
+    grid_barrier(p.bar, ++bk);
+  }
+  if (gtid == 0) {
+    p.stats->levels = d;
+    p.stats->push_levels = push_levels;
+    p.stats->pull_levels = pull_levels;
+    p.stats->ovf_any = ovf_any;
+    p.stats->reached = reached;
+    p.stats->edges = edges;
+  }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+namespace kern {
+
+hipError_t push_lb(hipStream_t s, int grid, const GraphView& g,
+                   const StateView& st, int32_t d, int64_t qe, int ovf_any) {
+  hipLaunchKernelGGL(k_push_lb, dim3(grid), dim3(kLbBlock), 0, s, g, st, d, qe,
+                     ovf_any);
+  return hipGetLastError();
+}
+
+hipError_t pull_lb(hipStream_t s, int grid, const GraphView& g,
+                   const StateView& st, int32_t d, const uint64_t* bigc,
+                   int32_t nbigc, int sigma, int ovf_any) {
+  if (sigma)
+    hipLaunchKernelGGL(k_pull_lb<true>, dim3(grid), dim3(kLbBlock), 0, s, g, st,
+                       d, bigc, nbigc, ovf_any);
+  else
+    hipLaunchKernelGGL(k_pull_lb<false>, dim3(grid), dim3(kLbBlock), 0, s, g,
+                       st, d, bigc, nbigc, ovf_any);
+  return hipGetLastError();
+}
+
+hipError_t persist_grid(int device, int* grid) {
+  int per_cu = 0, cus = 0, coop = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(k_persist), kPersistBlock, 0);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
+  if (e != hipSuccess) return e;
+  if (!coop || per_cu < 1) return hipErrorNotSupported;
+  *grid = per_cu * cus;
+  return hipSuccess;
+}
+
+hipError_t persist(hipStream_t s, int grid, const PersistParams& p) {
+  PersistParams arg = p;
+  void* args[] = {&arg};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_persist),
+                                    dim3(grid), dim3(kPersistBlock), args, 0, s);
+}
+
+}  // namespace kern
+}  // namespace tbfs
