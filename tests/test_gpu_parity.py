@@ -233,3 +233,31 @@ def test_repetition_same_result(gpu):
     assert tb.bfs_gpu_td_csc_wa(g.IC, g.CP, S3, s3, 5, g.nz, g.n, 3) == 0
     np.testing.assert_array_equal(S1, S3)
     np.testing.assert_array_equal(s1, s3)
+
+
+@pytest.mark.parametrize("fixture", ["er4096_seed1", "rmat12_seed2", "luxembourg_osm_r0"])
+def test_dropin_equals_reference_golden_vectors(gpu, fixture):
+    """Golden vectors written by the reference's own sequential C code
+    (tests/golden/make_fixtures.py): every drop-in entry point reproduces them
+    exactly (all counts here are below 2^24, so float is exact)."""
+    fx = np.load(golden(fixture + ".npz"))
+    if fixture == "er4096_seed1":
+        g = graphs.CSC(fx["CP"], fx["IC"])
+    elif fixture == "rmat12_seed2":
+        g = graphs.rmat(12, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)
+    else:
+        g = mtx.read_csc(golden("luxembourg_osm.mtx.gz"))
+    for i, r in enumerate(fx["roots"]):
+        res = _dropin_all(g, int(r))
+        for name, (rc, S, sig) in res.items():
+            assert rc == 0, name
+            if name.startswith("bfs_gpu_td_csc"):
+                np.testing.assert_array_equal(S, fx["S_td"][i], err_msg=name)
+                np.testing.assert_array_equal(sig, fx["sigma_td"][i], err_msg=name)
+            elif name.startswith("bfs_gpu_bu_csc"):
+                np.testing.assert_array_equal(S, fx["S_bu"][i], err_msg=name)
+            elif name.startswith("bfs_gpu_tdbu_csc"):
+                np.testing.assert_array_equal(S, fx["S_bu"][i], err_msg=name)
+                np.testing.assert_array_equal(sig, fx["sigma_tdbu"][i], err_msg=name)
+            else:
+                np.testing.assert_array_equal(sig, fx["sigma_cooc"][i], err_msg=name)

@@ -23,6 +23,8 @@ struct tbfs_graph {
   bool has_csc = true;
   uint64_t* bigc = nullptr;  // static pull slices of high in-degree columns
   int32_t nbigc = 0;
+  uint64_t* bigo = nullptr;  // static push slices of long out-lists (may alias bigc)
+  int32_t nbigo = 0;
 };
 
 // Per-BFS device state bound to one stream.

@@ -85,6 +85,7 @@ void graph_free(tbfs_graph* g) {
   if (g->JR != g->IC) dev_free(g->JR);
   if (g->J_owned) dev_free(g->J);
   if (g->I != g->IC) dev_free(g->I);
+  if (g->bigo != g->bigc) dev_free(g->bigo);
   dev_free(g->bigc);
   dev_free(g->CP);
   dev_free(g->IC);
@@ -301,25 +302,38 @@ int graph_ensure_cooc(tbfs_graph* g, hipStream_t s) {
   return TBFS_OK;
 }
 
-// Static pull slices of the columns with in-degree > kBigIn.
-int graph_build_bigc(tbfs_graph* g) {
-  std::vector<int32_t> cp((size_t)g->n + 1);
-  HIPCHK(hipMemcpy(cp.data(), g->CP, cp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+// Static slices (kPullChunk entries) of every list longer than kBigIn: in-lists
+// for the pull levels, out-lists for the push levels.
+static int build_slices(const int32_t* ptr_dev, int32_t n, uint64_t** out,
+                        int32_t* count) {
+  std::vector<int32_t> ptr((size_t)n + 1);
+  HIPCHK(hipMemcpy(ptr.data(), ptr_dev, ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   std::vector<uint64_t> big;
-  for (int32_t v = 0; v < g->n; ++v) {
-    const int64_t deg = (int64_t)cp[v + 1] - cp[v];
+  for (int32_t v = 0; v < n; ++v) {
+    const int64_t deg = (int64_t)ptr[v + 1] - ptr[v];
     if (deg <= kBigIn) continue;
     const int64_t nch = (deg + kPullChunk - 1) / kPullChunk;
     for (int64_t c = 0; c < nch; ++c)
       big.push_back((uint64_t)(uint32_t)v | ((uint64_t)c << 32));
   }
-  g->nbigc = (int32_t)big.size();
-  int rc = dev_alloc(&g->bigc, big.size());
+  *count = (int32_t)big.size();
+  int rc = dev_alloc(out, big.size());
   if (rc) return rc;
   if (!big.empty())
-    HIPCHK(hipMemcpy(g->bigc, big.data(), big.size() * sizeof(uint64_t),
+    HIPCHK(hipMemcpy(*out, big.data(), big.size() * sizeof(uint64_t),
                      hipMemcpyHostToDevice));
   return TBFS_OK;
+}
+
+int graph_build_bigc(tbfs_graph* g) {
+  int rc = build_slices(g->CP, g->n, &g->bigc, &g->nbigc);
+  if (rc) return rc;
+  if (g->RP == g->CP) {
+    g->bigo = g->bigc;
+    g->nbigo = g->nbigc;
+    return TBFS_OK;
+  }
+  return build_slices(g->RP, g->n, &g->bigo, &g->nbigo);
 }
 
 // ----------------------------------------------------------------------------
@@ -456,12 +470,13 @@ static int enqueue_persistent(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   p.sigma = c->sigma;
   p.ovf = c->ovf;
   for (int i = 0; i < 3; ++i) p.fb[i] = c->fb[i];
-  for (int i = 0; i < 2; ++i) p.q[i] = c->q[i];
   p.ctr = c->ctr;
   p.bar = c->bar;
   p.stats = stats_slot;
   p.bigc = g->bigc;
   p.nbigc = g->nbigc;
+  p.bigo = g->bigo;
+  p.nbigo = g->nbigo;
   p.root = root;
   p.mode = o.mode;
   p.sigma_on = o.sigma ? 1 : 0;
@@ -558,7 +573,7 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
       default:  // PUSH, or AUTO on its current side
         if (pushing) {
           if (o.pull == TBFS_PULL_LB)
-            HIPCHK(kern::push_lb(s, grid_for(qe, kLbBlock, 8192), gv, sv, d, qe,
+            HIPCHK(kern::push_lb(s, lb_grid, gv, sv, d, g->bigo, g->nbigo,
                                  (int)ovf_any));
           else
             HIPCHK(kern::push(s, grid_for(qe * 64, block, 65536), block, gv, sv,

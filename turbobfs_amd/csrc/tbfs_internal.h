@@ -7,14 +7,19 @@
 //   bigc uint64[nbigc]            static pull work list: one entry per
 //                                 kPullChunk-entry slice of every column whose
 //                                 in-degree exceeds kBigIn (v | slice << 32)
+//   bigo uint64[nbigo]            the same for out-lists longer than kBigIn
+//                                 (push work; aliases bigc when symmetric)
 //   S int32[n]                    depth, -1 = not reached
 //   sigma uint64[n], ovf uint8[n] shortest-path count mod 2^64 + sticky
 //                                 "count >= 2^64" flag
 //   fb[3] uint64[ceil(n/64)]      frontier bitmaps (bit v&63 of word v>>6);
 //                                 current / next / being cleared, rotating
-//   q[2] uint64[q_cap]            frontier work queues (current / next): one
-//                                 entry per kPushChunk-entry slice of a
-//                                 frontier vertex's out-list (v | slice << 32)
+//   q[2] uint64[q_cap]            frontier work queues of the per-level
+//                                 reference-mapping kernels (current / next):
+//                                 one entry per kPushChunk-entry slice of a
+//                                 frontier vertex's out-list (v | slice << 32);
+//                                 the load-balanced kernels read the frontier
+//                                 bitmap instead and need no queue
 //   LevelCtr[3]                   per-level counters, rotating
 #pragma once
 #include <hip/hip_runtime.h>
@@ -77,12 +82,13 @@ struct PersistParams {
   uint64_t* sigma;
   uint8_t* ovf;
   uint64_t* fb[3];
-  uint64_t* q[2];
   LevelCtr* ctr;    // [3]
   unsigned* bar;    // [2]: arrival count, generation (zero at launch)
   RunStats* stats;
   const uint64_t* bigc;
   int32_t nbigc;
+  const uint64_t* bigo;
+  int32_t nbigo;
   int32_t root;
   int32_t mode;     // TBFS_MODE_AUTO / _PUSH / _PULL_SC (all pulls are load-balanced)
   int32_t sigma_on;
@@ -123,7 +129,8 @@ hipError_t check_csc(hipStream_t s, const int32_t* CP, const int32_t* IC,
 
 // tbfs_persist.hip: load-balanced levels and the device-resident level loop
 hipError_t push_lb(hipStream_t s, int grid, const GraphView& g,
-                   const StateView& st, int32_t d, int64_t qe, int ovf_any);
+                   const StateView& st, int32_t d, const uint64_t* bigo,
+                   int32_t nbigo, int ovf_any);
 hipError_t pull_lb(hipStream_t s, int grid, const GraphView& g,
                    const StateView& st, int32_t d, const uint64_t* bigc,
                    int32_t nbigc, int sigma, int ovf_any);

@@ -3,12 +3,12 @@
 //
 // Direction-optimizing BFS with exact path counts (the reference's TD/BU
 // switch, TurboBFS_CSC_TDBU/bfsgputdbucsc_sc.cu:108-153, redone for wave64):
-//   push level  expands the frontier queue.  A wavefront takes 64 queue
-//               entries (slices of <= kPushChunk out-list entries), scans their
-//               lengths and walks the concatenated edges 64 at a time, so hub
-//               vertices spread over many wavefronts and small ones share one.
-//               Discovery is a CAS on S; counts are 64-bit atomics whose old
-//               value exposes the carry.
+//   push level  expands the frontier, read from its bitmap.  A wavefront takes
+//               the set bits of one frontier word, scans their out-degrees and
+//               walks the concatenated out-lists 64 edges at a time; out-lists
+//               longer than kBigIn are cut into static slices (graph property)
+//               spread over all wavefronts.  Discovery is a CAS on S; counts are
+//               64-bit atomics whose old value exposes the carry.
 //   pull level  sweeps the unvisited columns.  A wavefront takes the 64
 //               columns of one frontier word, scans their in-degrees and walks
 //               the concatenated in-lists; parents' counts are summed per
@@ -16,6 +16,8 @@
 //               into static kPullChunk slices (graph property, computed once)
 //               that many wavefronts reduce with global atomics.  No early exit:
 //               a column's count needs all of its parents.
+// Level counters (vertices found, their out- and in-degree sums) are tallied
+// per lane and added to the device counters once per workgroup per level.
 // The level loop keeps every decision on the device: each workgroup reads the
 // same level counters after the grid barrier and takes the same branch.
 #include <hip/hip_runtime.h>
@@ -33,79 +35,149 @@ using namespace dev;
 struct WaveLds {
   int32_t excl[64];  // exclusive prefix of per-lane work
   int32_t beg[64];   // first edge index of each lane's slice
-  uint64_t x[64];    // push: count of the slice's vertex; pull: per-column sums
+  uint64_t x[64];    // push: count of each lane's vertex; pull: per-column sums
 };
 
+// Per-lane tallies of the vertices this lane discovered in the current level.
+struct LevelAcc {
+  uint64_t q = 0, mout = 0, min = 0;
+  uint32_t ovf = 0;
+
+  __device__ __forceinline__ void found(const GraphView& g, int v) {
+    q += 1;
+    mout += outdeg(g, v);
+    min += indeg(g, v);
+  }
+};
+
+// Sum the tallies of the workgroup and add them to the level counters with
+// one atomic per field.  Every thread of the workgroup must call it.
+__device__ __forceinline__ void flush_level(LevelCtr* ctr, LevelAcc& a) {
+  __shared__ unsigned long long red[kPersistBlock / 64][3];
+  __shared__ unsigned int redo[kPersistBlock / 64];
+  const uint64_t q = wave_sum_u64(a.q);
+  const uint64_t mo = wave_sum_u64(a.mout);
+  const uint64_t mi = wave_sum_u64(a.min);
+  const bool ov = __ballot(a.ovf != 0) != 0;
+  const int wv = threadIdx.x >> 6;
+  if (lane() == 0) {
+    red[wv][0] = q;
+    red[wv][1] = mo;
+    red[wv][2] = mi;
+    redo[wv] = ov ? 1u : 0u;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t0 = 0, t1 = 0, t2 = 0;
+    unsigned int to = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+      t0 += red[i][0];
+      t1 += red[i][1];
+      t2 += red[i][2];
+      to |= redo[i];
+    }
+    if (t0) {
+      atomicAdd(&ctr->q, t0);
+      atomicAdd(&ctr->mout, t1);
+      atomicAdd(&ctr->min, t2);
+    }
+    if (to) atomicOr(&ctr->ovf, 1u);
+  }
+  __syncthreads();
+  a = LevelAcc();
+}
+
+// One edge u -> v of a push level, u at depth d with count x (overflow ou).
+__device__ __forceinline__ void push_edge(const GraphView& g, const StateView& st,
+                                          int dn, int v, uint64_t x, bool ou,
+                                          LevelAcc& a) {
+  int s = st.S[v];
+  bool nw = false;
+  if (s == -1) {
+    const int old = atomicCAS(&st.S[v], -1, dn);
+    nw = (old == -1);
+    s = dn;  // old is -1 or dn: only this level writes S now
+  }
+  if (s == dn) {
+    const unsigned long long old =
+        atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)x);
+    if (old + x < old || ou) {
+      st.ovf[v] = 1;
+      a.ovf = 1;
+    }
+  }
+  if (nw) {
+    atomicOr((unsigned long long*)&st.fnext[v >> 6], 1ull << (v & 63));
+    a.found(g, v);
+  }
+}
+
 // ----------------------------------------------------------------------------
+// Push level, driven by the frontier bitmap: (a) static slices of the long
+// out-lists whose vertex is in the frontier, (b) for every other frontier
+// word, the out-lists of its set bits walked load-balanced by one wavefront.
 __device__ __forceinline__ void push_lb_level(const GraphView& g,
                                               const StateView& st, int dn,
-                                              int64_t qe, uint32_t ovf_any,
-                                              int64_t wave, int64_t nwaves,
-                                              WaveLds* L) {
-  for (int64_t b = wave * 64; b < qe; b += nwaves * 64) {
-    const int64_t idx = b + lane();
+                                              uint32_t ovf_any,
+                                              const uint64_t* bigo,
+                                              int32_t nbigo, int64_t wave,
+                                              int64_t nwaves, WaveLds* L,
+                                              LevelAcc& a) {
+  for (int64_t i = wave; i < nbigo; i += nwaves) {
+    const uint64_t ent = bigo[i];
+    const int u = (int)(uint32_t)ent;
+    if (!fbit(st.fcur, u)) continue;
+    const int cb = g.RP[u] + (int)(ent >> 32) * kPullChunk;
+    const int ce = min(g.RP[u + 1], cb + kPullChunk);
+    const uint64_t x = st.sigma[u];
+    const bool ou = ovf_any && st.ovf[u];
+    for (int k = cb + lane(); k < ce; k += 64) push_edge(g, st, dn, g.JR[k], x, ou, a);
+  }
+  const int64_t nwords = ((int64_t)g.n + 63) >> 6;
+  for (int64_t w = wave; w < nwords; w += nwaves) {
+    const uint64_t bits = st.fcur[w];
+    if (bits == 0) continue;
+    const int u = (int)(w * 64) + lane();
     int beg = 0, deg = 0;
     uint64_t x = 0;
     uint32_t ou = 0;
-    if (idx < qe) {
-      const uint64_t ent = st.qcur[idx];
-      const int u = (int)(uint32_t)ent;
-      const int re = g.RP[u + 1];
-      beg = g.RP[u] + (int)(ent >> 32) * kPushChunk;
-      deg = min(re, beg + kPushChunk) - beg;
-      x = st.sigma[u];
-      if (ovf_any) ou = st.ovf[u];
+    if ((bits >> lane()) & 1ull) {
+      beg = g.RP[u];
+      const int dg = g.RP[u + 1] - beg;
+      if (dg <= kBigIn) {
+        deg = dg;
+        x = st.sigma[u];
+        if (ovf_any) ou = st.ovf[u];
+      }
     }
     const uint32_t incl = wave_incl_scan((uint32_t)deg);
     const int total = (int)__shfl(incl, 63);
+    if (total == 0) continue;
     const uint64_t oumask = __ballot(ou != 0);
     wave_lds_sync();
     L->excl[lane()] = (int)incl - deg;
     L->beg[lane()] = beg;
     L->x[lane()] = x;
     wave_lds_sync();
-    for (int e0 = 0; e0 < total; e0 += 64) {
-      const int e = e0 + lane();
-      bool nw = false;
-      int v = 0;
-      uint32_t ov = 0;
-      if (e < total) {
-        const int o = owner_of(L->excl, e);
-        const int k = L->beg[o] + (e - L->excl[o]);
-        v = g.JR[k];
-        int s = st.S[v];
-        if (s == -1) {
-          const int old = atomicCAS(&st.S[v], -1, dn);
-          nw = (old == -1);
-          s = dn;
-        }
-        if (s == dn) {
-          const uint64_t xo = L->x[o];
-          const unsigned long long old =
-              atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)xo);
-          if (old + xo < old || ((oumask >> o) & 1ull)) {
-            st.ovf[v] = 1;
-            ov = 1;
-          }
-        }
-        if (nw) atomicOr((unsigned long long*)&st.fnext[v >> 6], 1ull << (v & 63));
-      }
-      if (__ballot(nw || ov)) {
-        emit(st, v, nw, nw ? outdeg(g, v) : 0u, nw ? indeg(g, v) : 0u, ov);
-        if (__ballot(ov && !nw) && lane() == 0) atomicOr(&st.ctr->ovf, 1u);
-      }
+    for (int e = lane(); e < total; e += 64) {
+      const int o = owner_of(L->excl, e);
+      const int k = L->beg[o] + (e - L->excl[o]);
+      push_edge(g, st, dn, g.JR[k], L->x[o], (oumask >> o) & 1ull, a);
     }
   }
 }
 
+// Pull level: (a) static slices of the high in-degree columns, reduced with
+// global atomics; (b) the other unvisited columns, 64 consecutive ones per
+// wavefront, in-lists walked load-balanced, per-column sums in LDS.
 template <bool SIGMA>
 __device__ __forceinline__ void pull_lb_level(const GraphView& g,
                                               const StateView& st, int dn,
                                               uint32_t ovf_any,
                                               const uint64_t* bigc,
                                               int32_t nbigc, int64_t wave,
-                                              int64_t nwaves, WaveLds* L) {
-  // (a) slices of the high in-degree columns
+                                              int64_t nwaves, WaveLds* L,
+                                              LevelAcc& a) {
   for (int64_t i = wave; i < nbigc; i += nwaves) {
     const uint64_t ent = bigc[i];
     const int v = (int)(uint32_t)ent;
@@ -128,7 +200,6 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
     }
     if (__ballot(f)) {
       if (SIGMA) wave_sum_carry(sum, ov);
-      bool first = false;
       if (lane() == 0) {
         if (SIGMA) {
           const unsigned long long old =
@@ -136,19 +207,19 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
           if (old + sum < old) ov = 1;
           if (ov) {
             st.ovf[v] = 1;
-            atomicOr(&st.ctr->ovf, 1u);
+            a.ovf = 1;
           }
         }
         const unsigned long long bit = 1ull << (v & 63);
         const unsigned long long old =
             atomicOr((unsigned long long*)&st.fnext[v >> 6], bit);
-        first = (old & bit) == 0;
-        if (first) st.S[v] = dn;
+        if ((old & bit) == 0) {  // first slice of v to find a parent
+          st.S[v] = dn;
+          a.found(g, v);
+        }
       }
-      emit(st, v, first, first ? outdeg(g, v) : 0u, first ? indeg(g, v) : 0u, 0u);
     }
   }
-  // (b) the other columns, 64 consecutive ones per wavefront
   const int64_t nwords = ((int64_t)g.n + 63) >> 6;
   for (int64_t w = wave; w < nwords; w += nwaves) {
     const int v = (int)(w * 64) + lane();
@@ -182,35 +253,38 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
       }
     }
     fmask = wave_or_u64(fmask);
+    if (fmask == 0) continue;
     if (SIGMA) omask = wave_or_u64(omask);
     wave_lds_sync();
-    const bool found = (fmask >> lane()) & 1ull;
-    uint32_t ov = 0;
-    if (found) {
+    if ((fmask >> lane()) & 1ull) {
       st.S[v] = dn;
       if (SIGMA) {
         st.sigma[v] = L->x[lane()];
-        ov = (uint32_t)((omask >> lane()) & 1ull);
-        if (ov) st.ovf[v] = 1;
+        if ((omask >> lane()) & 1ull) {
+          st.ovf[v] = 1;
+          a.ovf = 1;
+        }
       }
+      a.found(g, v);
     }
-    if (fmask) {
-      if (lane() == 0) atomicOr((unsigned long long*)&st.fnext[w], fmask);
-      emit(st, v, found, found ? outdeg(g, v) : 0u, found ? indeg(g, v) : 0u, ov);
-    }
+    if (lane() == 0) atomicOr((unsigned long long*)&st.fnext[w], fmask);
   }
 }
 
 // ----------------------------------------------------------------------------
 // standalone level kernels (host-driven loop)
 __global__ __launch_bounds__(kLbBlock) void k_push_lb(GraphView g, StateView st,
-                                                      int32_t d, int64_t qe,
+                                                      int32_t d,
+                                                      const uint64_t* bigo,
+                                                      int32_t nbigo,
                                                       int ovf_any) {
   __shared__ WaveLds lds[kLbBlock / 64];
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  push_lb_level(g, st, d + 1, qe, (uint32_t)ovf_any, wave, nwaves,
-                &lds[threadIdx.x >> 6]);
+  LevelAcc a;
+  push_lb_level(g, st, d + 1, (uint32_t)ovf_any, bigo, nbigo, wave, nwaves,
+                &lds[threadIdx.x >> 6], a);
+  flush_level(st.ctr, a);
 }
 
 template <bool SIGMA>
@@ -222,8 +296,10 @@ __global__ __launch_bounds__(kLbBlock) void k_pull_lb(GraphView g, StateView st,
   __shared__ WaveLds lds[kLbBlock / 64];
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  LevelAcc a;
   pull_lb_level<SIGMA>(g, st, d + 1, (uint32_t)ovf_any, bigc, nbigc, wave,
-                       nwaves, &lds[threadIdx.x >> 6]);
+                       nwaves, &lds[threadIdx.x >> 6], a);
+  flush_level(st.ctr, a);
 }
 
 // ----------------------------------------------------------------------------
@@ -279,13 +355,10 @@ __global__ __launch_bounds__(kPersistBlock) void k_persist(PersistParams p) {
     p.fb[1][w] = 0ull;
     p.fb[2][w] = 0ull;
   }
-  const int64_t rch = ((int64_t)rout + kPushChunk - 1) / kPushChunk;
-  for (int64_t c = gtid; c < rch; c += gstride)
-    p.q[0][c] = (uint64_t)(uint32_t)r | ((uint64_t)c << 32);
   if (gtid < 3) {
     LevelCtr* c = &p.ctr[gtid];
     c->q = gtid == 0 ? 1ull : 0ull;
-    c->qe = gtid == 0 ? (unsigned long long)rch : 0ull;
+    c->qe = 0ull;
     c->mout = gtid == 0 ? rout : 0ull;
     c->min = gtid == 0 ? rin : 0ull;
     c->ovf = 0;
@@ -298,12 +371,12 @@ __global__ __launch_bounds__(kPersistBlock) void k_persist(PersistParams p) {
   unsigned long long mu = (unsigned long long)g.nz;
   unsigned long long reached = 0, edges = 0;
   int push_levels = 0, pull_levels = 0;
+  LevelAcc acc;
   int d = 0;
   for (;; ++d) {
     const LevelCtr* c = &p.ctr[d % 3];
     const unsigned long long q = ld_agent(&c->q);
     if (q == 0) break;
-    const unsigned long long qe = ld_agent(&c->qe);
     const unsigned long long mf = ld_agent(&c->mout);
     const unsigned long long mi = ld_agent(&c->min);
     ovf_any |= ld_agent(&c->ovf);
@@ -326,17 +399,18 @@ __global__ __launch_bounds__(kPersistBlock) void k_persist(PersistParams p) {
       z->ovf = 0;
     }
     StateView st{p.S, p.sigma, p.ovf, p.fb[d % 3], p.fb[(d + 1) % 3],
-                 p.q[d & 1], p.q[(d + 1) & 1], &p.ctr[(d + 1) % 3]};
+                 nullptr, nullptr, &p.ctr[(d + 1) % 3]};
     if (pushing) {
-      push_lb_level(g, st, d + 1, (int64_t)qe, ovf_any, wave, nwaves, L);
+      push_lb_level(g, st, d + 1, ovf_any, p.bigo, p.nbigo, wave, nwaves, L, acc);
       ++push_levels;
     } else {
       if (p.sigma_on)
-        pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L);
+        pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc);
       else
-        pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L);
+        pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc);
       ++pull_levels;
     }
+    flush_level(st.ctr, acc);
     grid_barrier(p.bar, ++bk);
   }
   if (gtid == 0) {
@@ -355,9 +429,10 @@ __global__ __launch_bounds__(kPersistBlock) void k_persist(PersistParams p) {
 namespace kern {
 
 hipError_t push_lb(hipStream_t s, int grid, const GraphView& g,
-                   const StateView& st, int32_t d, int64_t qe, int ovf_any) {
-  hipLaunchKernelGGL(k_push_lb, dim3(grid), dim3(kLbBlock), 0, s, g, st, d, qe,
-                     ovf_any);
+                   const StateView& st, int32_t d, const uint64_t* bigo,
+                   int32_t nbigo, int ovf_any) {
+  hipLaunchKernelGGL(k_push_lb, dim3(grid), dim3(kLbBlock), 0, s, g, st, d,
+                     bigo, nbigo, ovf_any);
   return hipGetLastError();
 }
 
