@@ -104,11 +104,13 @@ def main():
               "the single-source BFS is measured on one MI355X", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import turbobfs_amd as tb
     from turbobfs_amd import _lib, graphs
 
-    torch.cuda.set_device(0)
+    tb.load()
+    if tb.device_count() < 1:
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
     g = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)
     sources = graphs.sample_sources(graphs.nonzero_degree_vertices(g), args.sources, seed=2)
     H, P = _lib.LOOP_HOST, _lib.LOOP_PERSISTENT
@@ -132,21 +134,17 @@ def main():
     opts = tb.default_opts(**kw)
     gr = tb.Graph(g.CP, g.IC, symmetric=True)
     ctx = tb.Context(g.n)
-    stream = torch.cuda.ExternalStream(ctx.stream)
 
     roots = np.array([sources[i % len(sources)] for i in range(args.steps)], np.int32)
     if args.warmup:
         ctx.run_many(gr, roots[:args.warmup] if args.warmup <= len(roots) else
                      np.resize(roots, args.warmup), opts)
-    torch.cuda.synchronize()
 
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
+    # HIP events on the context's stream (the stream every BFS kernel runs on)
+    ctx.mark(0)
     stats = ctx.run_many(gr, roots, opts)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    total_ms = ev0.elapsed_time(ev1)
+    ctx.mark(1)
+    total_ms = ctx.elapsed(0, 1)
     edges = sum(st["edges"] for st in stats)
     kern_ms = sum(st["ms"] for st in stats)
     levels = [st["levels"] for st in stats]

@@ -138,6 +138,21 @@ int tbfs_bfs_many(tbfs_ctx *c, const tbfs_graph *g, const int32_t *roots,
 int tbfs_ctx_read(tbfs_ctx *c, int32_t n, int32_t *S, uint64_t *sigma,
                   uint8_t *ovf);
 
+/* Stream-ordered timing: record marker `slot` (0..15) on the context's stream;
+ * tbfs_ctx_elapsed waits for marker b and gives the device time from a to b.
+ * (Lets a caller time work on the library's own stream without another
+ * runtime in the process.) */
+int tbfs_ctx_mark(tbfs_ctx *c, int32_t slot);
+int tbfs_ctx_elapsed(tbfs_ctx *c, int32_t a, int32_t b, double *ms);
+
+/* Number of visible HIP devices (0 when none; never fails for lack of one). */
+int tbfs_device_count(int32_t *count);
+
+/* Properties of device dev: name (NUL-terminated, up to name_len bytes),
+ * compute units, HBM bytes, cooperative-launch support (any may be NULL). */
+int tbfs_device_info(int32_t dev, char *name, int32_t name_len, int32_t *cus,
+                     int64_t *hbm_bytes, int32_t *coop);
+
 /* Device pointers of the context's S / sigma / ovf buffers. */
 int tbfs_ctx_device_ptrs(tbfs_ctx *c, int32_t **S, uint64_t **sigma,
                          uint8_t **ovf);

@@ -95,6 +95,11 @@ SIGNATURES = {
     "tbfs_bfs_many": (C.c_int, [C.c_void_p, C.c_void_p, _i32p, C.c_int32, C.POINTER(Opts), C.POINTER(Stats)]),
     "tbfs_ctx_read": (C.c_int, [C.c_void_p, C.c_int32, _i32p, _u64p, _u8p]),
     "tbfs_ctx_device_ptrs": (C.c_int, [C.c_void_p, C.POINTER(_i32p), C.POINTER(_u64p), C.POINTER(_u8p)]),
+    "tbfs_ctx_mark": (C.c_int, [C.c_void_p, C.c_int32]),
+    "tbfs_ctx_elapsed": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    "tbfs_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+    "tbfs_device_info": (C.c_int, [C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     # generators (tbfs_gen.h)
     "tbfs_gen_er": (C.c_int, [C.c_int32, C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
     "tbfs_gen_rmat": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
@@ -124,6 +129,23 @@ def load() -> C.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    load().tbfs_device_count(C.byref(n))
+    return int(n.value)
+
+
+def device_info(dev: int = 0) -> dict:
+    name = C.create_string_buffer(128)
+    cus = C.c_int32()
+    hbm = C.c_int64()
+    coop = C.c_int32()
+    check(load().tbfs_device_info(dev, name, 128, C.byref(cus), C.byref(hbm), C.byref(coop)),
+          "tbfs_device_info")
+    return {"name": name.value.decode(), "cus": cus.value, "hbm_bytes": hbm.value,
+            "cooperative": bool(coop.value)}
 
 
 def status_str(rc: int) -> str:

@@ -180,6 +180,17 @@ class Context:
                                         st), "tbfs_bfs_many")
         return [{f: getattr(x, f) for f, _ in Stats._fields_ if f != "pad"} for x in st]
 
+    def mark(self, slot: int) -> None:
+        """Record timing marker `slot` on this context's stream."""
+        check(_lib.load().tbfs_ctx_mark(self._h, int(slot)), "tbfs_ctx_mark")
+
+    def elapsed(self, a: int, b: int) -> float:
+        """Device milliseconds between markers a and b (waits for b)."""
+        ms = C.c_double()
+        check(_lib.load().tbfs_ctx_elapsed(self._h, int(a), int(b), C.byref(ms)),
+              "tbfs_ctx_elapsed")
+        return float(ms.value)
+
     def read(self, n: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         S = np.empty(n, dtype=np.int32)
         sigma = np.empty(n, dtype=np.uint64)

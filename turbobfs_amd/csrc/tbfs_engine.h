@@ -45,6 +45,7 @@ struct tbfs_ctx {
   tbfs::RunStats* stats_h = nullptr;
   int stats_cap = 0;
   std::vector<hipEvent_t> evs;
+  hipEvent_t marks[16] = {};
   int device = 0;
   int persist_grid_max = 0;         // 0: no cooperative launch available
   int32_t last_n = 0;

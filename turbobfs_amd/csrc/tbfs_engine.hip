@@ -352,6 +352,8 @@ void ctx_free(tbfs_ctx* c) {
   if (c->ctr_h) (void)hipHostFree(c->ctr_h);
   if (c->stats_h) (void)hipHostFree(c->stats_h);
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->marks)
+    if (e) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -767,6 +769,48 @@ int tbfs_bfs_many(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots,
 int tbfs_ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma,
                   uint8_t* ovf) {
   return ctx_read(c, n, S, sigma, ovf);
+}
+
+int tbfs_ctx_mark(tbfs_ctx* c, int32_t slot) {
+  if (!c || slot < 0 || slot >= 16) return TBFS_ERR_INVALID;
+  if (!c->marks[slot]) HIPCHK(hipEventCreate(&c->marks[slot]));
+  HIPCHK(hipEventRecord(c->marks[slot], c->stream));
+  return TBFS_OK;
+}
+
+int tbfs_ctx_elapsed(tbfs_ctx* c, int32_t a, int32_t b, double* ms) {
+  if (!c || !ms || a < 0 || a >= 16 || b < 0 || b >= 16) return TBFS_ERR_INVALID;
+  if (!c->marks[a] || !c->marks[b]) return TBFS_ERR_INVALID;
+  HIPCHK(hipEventSynchronize(c->marks[b]));
+  float f = 0.f;
+  HIPCHK(hipEventElapsedTime(&f, c->marks[a], c->marks[b]));
+  *ms = (double)f;
+  return TBFS_OK;
+}
+
+int tbfs_device_count(int32_t* count) {
+  if (!count) return TBFS_ERR_INVALID;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    n = 0;
+  }
+  *count = n;
+  return TBFS_OK;
+}
+
+int tbfs_device_info(int32_t dev, char* name, int32_t name_len, int32_t* cus,
+                     int64_t* hbm_bytes, int32_t* coop) {
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, dev));
+  if (name && name_len > 0) {
+    std::strncpy(name, p.gcnArchName[0] ? p.gcnArchName : p.name, (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (cus) *cus = p.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = (int64_t)p.totalGlobalMem;
+  if (coop) *coop = p.cooperativeLaunch;
+  return TBFS_OK;
 }
 
 int tbfs_ctx_device_ptrs(tbfs_ctx* c, int32_t** S, uint64_t** sigma,
