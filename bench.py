@@ -90,8 +90,9 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--sources", type=int, default=64)
     ap.add_argument("--mode", default="auto",
-                    choices=["auto", "auto_host", "auto_sc", "auto_wa", "pull_sc", "pull_wa",
-                             "push", "cooc"])
+                    choices=["auto", "auto_chain", "auto_host", "auto_sc", "auto_wa", "pull_sc",
+                             "pull_wa", "push", "cooc"])
+    ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0, help="workgroups of the persistent kernel (0: all co-resident)")
     ap.add_argument("--alpha", type=float, default=0.0)
     ap.add_argument("--beta", type=float, default=0.0)
@@ -117,6 +118,7 @@ def main():
     modes = {
         "auto": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, loop=P),
         "auto_host": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, loop=H),
+        "auto_chain": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, loop=_lib.LOOP_CHAINED),
         "auto_sc": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_SC, loop=H),
         "auto_wa": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_WA, loop=H),
         "pull_sc": dict(mode=_lib.MODE_PULL_SC, loop=H),
@@ -127,6 +129,8 @@ def main():
     kw = dict(modes[args.mode])
     if args.grid:
         kw["grid"] = args.grid
+    if args.block:
+        kw["block"] = args.block
     if args.alpha:
         kw["alpha"] = args.alpha
     if args.beta:

@@ -56,9 +56,13 @@ extern "C" {
 /* level loop (tbfs_opts.loop) */
 #define TBFS_LOOP_HOST 0       /* host launches each level, reads its counters */
 #define TBFS_LOOP_PERSISTENT 1 /* one cooperative launch per BFS, levels
-                                  separated by a device-wide barrier; used for
-                                  AUTO / PUSH with TBFS_PULL_LB, otherwise the
-                                  host loop runs */
+                                  separated by a device-wide barrier */
+#define TBFS_LOOP_CHAINED 2    /* one kernel per level, enqueued back to back;
+                                  each decides its direction from the device
+                                  counters, the host checks for the end once
+                                  per chunk of levels */
+/* PERSISTENT and CHAINED apply to AUTO / PUSH with TBFS_PULL_LB; other
+ * combinations run the host loop. */
 
 typedef struct tbfs_graph tbfs_graph;
 typedef struct tbfs_ctx tbfs_ctx;
@@ -70,7 +74,8 @@ typedef struct tbfs_opts {
   int32_t bbreak; /* depths only: stop a column scan at its first parent */
   float alpha;    /* AUTO: switch push->pull when m_frontier*alpha > m_unexplored */
   float beta;     /* AUTO: switch pull->push when n_frontier*beta < n */
-  int32_t block;  /* threads per workgroup, multiple of 64 (0: default) */
+  int32_t block;  /* threads per workgroup, multiple of 64 (0: default);
+                     the persistent loop takes 256 or 1024 */
   int32_t grid;   /* workgroups (0: default) */
   int32_t pull;   /* TBFS_PULL_*: AUTO's bottom-up kernel */
   int32_t reserved[7];

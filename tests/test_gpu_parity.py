@@ -104,6 +104,11 @@ MODES = [
     ("pers_push", dict(mode=L.MODE_PUSH, pull=L.PULL_LB, loop=PERS)),
     ("pers_pull", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=PERS, alpha=1e9)),
     ("pers_auto_g64", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=PERS, grid=64)),
+    ("pers_auto_b1024", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=PERS, block=1024)),
+    ("chain_auto", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED)),
+    ("chain_push", dict(mode=L.MODE_PUSH, pull=L.PULL_LB, loop=L.LOOP_CHAINED)),
+    ("chain_pull_depth_only", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED,
+                                   alpha=1e9, sigma=0)),
 ]
 
 
@@ -115,6 +120,9 @@ def _check_modes(g, roots, symmetric, modes=MODES):
         for name, kw in modes:
             res = ctx.bfs(gr, int(r), tb.default_opts(**kw))
             np.testing.assert_array_equal(res.S, S, err_msg=f"{name} r={r} S")
+            if kw.get("sigma", 1) == 0:
+                assert res.stats["levels"] == lv, name
+                continue
             np.testing.assert_array_equal(res.sigma, sig, err_msg=f"{name} r={r} sigma")
             np.testing.assert_array_equal(res.ovf, ovf, err_msg=f"{name} r={r} ovf")
             assert res.stats["levels"] == lv, name
@@ -173,12 +181,13 @@ def test_hub_columns_split_across_slices(gpu):
     _check_modes(g, [base], symmetric=False)
 
 
-def test_run_many_matches_single(gpu):
+@pytest.mark.parametrize("loop", [L.LOOP_PERSISTENT, L.LOOP_CHAINED, L.LOOP_HOST])
+def test_run_many_matches_single(gpu, loop):
     g = graphs.rmat(12, 16, seed=7)
     roots = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 6, seed=1)
     gr = tb.Graph(g.CP, g.IC, symmetric=True)
     ctx = tb.Context(g.n)
-    stats = ctx.run_many(gr, roots)
+    stats = ctx.run_many(gr, roots, tb.default_opts(loop=loop))
     S, sig, ovf = ctx.read(g.n)
     S2, sig2, ovf2, lv = oracle.textbook(g.CP, g.IC, int(roots[-1]))
     np.testing.assert_array_equal(S, S2)

@@ -48,6 +48,10 @@ struct tbfs_ctx {
   hipEvent_t marks[16] = {};
   int device = 0;
   int persist_grid_max = 0;         // 0: no cooperative launch available
+  int persist_grid_max1024 = 0;     // the same for 1024-thread workgroups
+  tbfs::LoopState* loop = nullptr;  // [2] chained-loop state slots
+  tbfs::LoopState* loop_h = nullptr;
+  int chain_k = 8;                  // levels enqueued per host check
   int32_t last_n = 0;
 };
 

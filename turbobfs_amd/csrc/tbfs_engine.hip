@@ -349,6 +349,8 @@ void ctx_free(tbfs_ctx* c) {
   dev_free(c->ctr);
   dev_free(c->bar);
   dev_free(c->stats);
+  dev_free(c->loop);
+  if (c->loop_h) (void)hipHostFree(c->loop_h);
   if (c->ctr_h) (void)hipHostFree(c->ctr_h);
   if (c->stats_h) (void)hipHostFree(c->stats_h);
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
@@ -380,15 +382,22 @@ int ctx_create(int32_t n_cap, tbfs_ctx** out) {
   if (!rc) rc = dev_alloc(&c->ctr, 4);
   if (!rc) rc = dev_alloc(&c->bar, 16);
   if (!rc) rc = dev_alloc(&c->stats, 1);
+  if (!rc) rc = dev_alloc(&c->loop, 2);
   if (rc) goto fail;
+  e = hipHostMalloc((void**)&c->loop_h, 2 * sizeof(LoopState), hipHostMallocDefault);
+  if (e != hipSuccess) { rc = hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); goto fail; }
   e = hipHostMalloc((void**)&c->ctr_h, 4 * sizeof(LevelCtr), hipHostMallocDefault);
   if (e != hipSuccess) { rc = hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); goto fail; }
   c->stats_cap = 1;
   e = hipHostMalloc((void**)&c->stats_h, sizeof(RunStats), hipHostMallocDefault);
   if (e != hipSuccess) { rc = hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); goto fail; }
-  if (kern::persist_grid(c->device, &c->persist_grid_max) != hipSuccess) {
+  if (kern::persist_grid(c->device, 256, &c->persist_grid_max) != hipSuccess) {
     (void)hipGetLastError();
-    c->persist_grid_max = 0;  // no cooperative launch: host loop only
+    c->persist_grid_max = 0;  // no cooperative launch: no persistent loop
+  }
+  if (kern::persist_grid(c->device, 1024, &c->persist_grid_max1024) != hipSuccess) {
+    (void)hipGetLastError();
+    c->persist_grid_max1024 = 0;
   }
   *out = c;
   return TBFS_OK;
@@ -446,26 +455,33 @@ static int check_run(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   if (root < 0 || root >= g->n) return TBFS_ERR_INVALID;
   if (o.mode < TBFS_MODE_AUTO || o.mode > TBFS_MODE_COOC) return TBFS_ERR_INVALID;
   if (o.pull < TBFS_PULL_SC || o.pull > TBFS_PULL_LB) return TBFS_ERR_INVALID;
-  if (o.loop != TBFS_LOOP_HOST && o.loop != TBFS_LOOP_PERSISTENT) return TBFS_ERR_INVALID;
+  if (o.loop < TBFS_LOOP_HOST || o.loop > TBFS_LOOP_CHAINED) return TBFS_ERR_INVALID;
   if (o.mode != TBFS_MODE_COOC && !g->CP) return TBFS_ERR_UNSUPPORTED;
   if (o.mode == TBFS_MODE_COOC && !o.sigma) return TBFS_ERR_UNSUPPORTED;
   if (o.block != 0 && (o.block % 64 != 0 || o.block > 1024 || o.block < 64))
     return TBFS_ERR_INVALID;
+  if (o.loop == TBFS_LOOP_PERSISTENT && o.block != 0 && o.block != 256 && o.block != 1024)
+    return TBFS_ERR_INVALID;
   return TBFS_OK;
 }
 
-// The persistent loop applies when asked for and available; the COOC edge
-// sweep and the one-lane / one-wavefront reference mappings run host-driven.
-static bool use_persistent(const tbfs_ctx* c, const tbfs_opts& o) {
-  if (o.loop != TBFS_LOOP_PERSISTENT || c->persist_grid_max <= 0) return false;
-  if (o.mode == TBFS_MODE_COOC) return false;
-  if (o.mode == TBFS_MODE_PULL_SC || o.mode == TBFS_MODE_PULL_WA) return false;
-  return o.pull == TBFS_PULL_LB;
+// Which level loop runs: the device-resident loops (persistent, chained)
+// cover the load-balanced AUTO / PUSH path; the COOC edge sweep and the
+// one-lane / one-wavefront reference mappings run host-driven.
+static int pick_loop(const tbfs_ctx* c, const tbfs_opts& o) {
+  if (o.loop == TBFS_LOOP_HOST) return TBFS_LOOP_HOST;
+  if (o.mode == TBFS_MODE_COOC || o.mode == TBFS_MODE_PULL_SC ||
+      o.mode == TBFS_MODE_PULL_WA || o.pull != TBFS_PULL_LB)
+    return TBFS_LOOP_HOST;
+  if (o.loop == TBFS_LOOP_PERSISTENT) {
+    const int gmax = o.block == 1024 ? c->persist_grid_max1024 : c->persist_grid_max;
+    return gmax > 0 ? TBFS_LOOP_PERSISTENT : TBFS_LOOP_CHAINED;
+  }
+  return TBFS_LOOP_CHAINED;
 }
 
-// Enqueue one device-resident BFS (no host synchronisation).
-static int enqueue_persistent(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
-                              const tbfs_opts& o, RunStats* stats_slot) {
+static PersistParams make_params(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
+                                 const tbfs_opts& o, RunStats* stats_slot) {
   PersistParams p;
   p.g = GraphView{g->n, g->nz, g->CP, g->IC, g->RP, g->JR};
   p.S = c->S;
@@ -484,10 +500,43 @@ static int enqueue_persistent(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   p.sigma_on = o.sigma ? 1 : 0;
   p.alpha = o.alpha > 0 ? o.alpha : 4.0f;
   p.beta = o.beta > 0 ? o.beta : 24.0f;
-  int grid = c->persist_grid_max;
+  p.loop = c->loop;
+  return p;
+}
+
+// Enqueue one device-resident BFS (no host synchronisation).
+static int enqueue_persistent(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
+                              const tbfs_opts& o, RunStats* stats_slot) {
+  const PersistParams p = make_params(c, g, root, o, stats_slot);
+  const int block = o.block == 1024 ? 1024 : 256;
+  int grid = block == 1024 ? c->persist_grid_max1024 : c->persist_grid_max;
   if (o.grid > 0 && o.grid < grid) grid = o.grid;
   HIPCHK(hipMemsetAsync(c->bar, 0, 2 * sizeof(unsigned), c->stream));
-  HIPCHK(kern::persist(c->stream, grid, p));
+  HIPCHK(kern::persist(c->stream, grid, block, p));
+  return TBFS_OK;
+}
+
+// Chained loop for one BFS: init, then chunks of chain_k level kernels; the
+// host reads the loop state once per chunk.  Ends with the stream drained.
+static int run_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
+                       const tbfs_opts& o, RunStats* stats_slot) {
+  const PersistParams p = make_params(c, g, root, o, stats_slot);
+  const int64_t nwords = ((int64_t)g->n + 63) / 64;
+  int grid = grid_for(nwords * 64, kLbBlock, 8192);
+  if (o.grid > 0 && o.grid < grid) grid = o.grid;
+  hipStream_t s = c->stream;
+  HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
+  int32_t d = 0;
+  for (;;) {
+    for (int i = 0; i < c->chain_k; ++i) HIPCHK(kern::chain_level(s, grid, p, d + i));
+    d += c->chain_k;
+    HIPCHK(hipMemcpyAsync(c->loop_h, c->loop + (d & 1), sizeof(LoopState),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (c->loop_h->done) break;
+  }
+  // next BFS on this context: enough levels for this one plus a margin
+  c->chain_k = std::max(4, std::min(64, c->loop_h->d + 2));
   return TBFS_OK;
 }
 
@@ -638,7 +687,8 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
   }
   int rc = ctx_reserve_queue(c, g);
   if (rc) return rc;
-  if (!use_persistent(c, o)) {
+  const int loop = pick_loop(c, o);
+  if (loop == TBFS_LOOP_HOST) {
     for (int i = 0; i < count; ++i) {
       rc = run_host_loop(c, g, roots[i], o, st ? &st[i] : nullptr);
       if (rc) return rc;
@@ -651,7 +701,9 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
   hipStream_t s = c->stream;
   HIPCHK(hipEventRecord(c->evs[0], s));
   for (int i = 0; i < count; ++i) {
-    rc = enqueue_persistent(c, g, roots[i], o, c->stats + i);
+    rc = loop == TBFS_LOOP_PERSISTENT
+             ? enqueue_persistent(c, g, roots[i], o, c->stats + i)
+             : run_chained(c, g, roots[i], o, c->stats + i);
     if (rc) return rc;
     HIPCHK(hipEventRecord(c->evs[i + 1], s));
   }

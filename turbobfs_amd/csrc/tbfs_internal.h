@@ -75,7 +75,23 @@ struct StateView {
   LevelCtr* ctr;         // counters of depth d+1 (must be zero on entry)
 };
 
-// Everything the device-resident level loop needs.
+// Loop state carried from one level to the next by the device-resident
+// loops (registers in the persistent kernel; two alternating slots in device
+// memory for the chained per-level kernels, slot d & 1 read by level d).
+struct alignas(64) LoopState {
+  int32_t d;            // depth of the current frontier
+  int32_t pushing;      // direction used by the previous level
+  int32_t done;         // frontier empty: the BFS has finished
+  uint32_t ovf_any;     // some count reached so far is >= 2^64
+  int32_t push_levels;
+  int32_t pull_levels;
+  unsigned long long mu;       // in-list entries of the unreached vertices
+  unsigned long long reached;  // vertices reached before this level
+  unsigned long long edges;    // their in-list entries
+  unsigned long long pad[2];
+};
+
+// Everything the device-resident level loops need.
 struct PersistParams {
   GraphView g;
   int32_t* S;
@@ -94,6 +110,7 @@ struct PersistParams {
   int32_t sigma_on;
   float alpha;
   float beta;
+  LoopState* loop;  // [2] (chained loop only)
 };
 
 // Kernel launch wrappers.  Every launch is made in the translation unit that
@@ -134,12 +151,17 @@ hipError_t push_lb(hipStream_t s, int grid, const GraphView& g,
 hipError_t pull_lb(hipStream_t s, int grid, const GraphView& g,
                    const StateView& st, int32_t d, const uint64_t* bigc,
                    int32_t nbigc, int sigma, int ovf_any);
-// Largest cooperative grid for the persistent kernel (blocks of kPersistBlock).
-hipError_t persist_grid(int device, int* grid);
-hipError_t persist(hipStream_t s, int grid, const PersistParams& p);
+// Largest cooperative grid for the persistent kernel with `block` threads
+// per workgroup (256 or 1024).
+hipError_t persist_grid(int device, int block, int* grid);
+hipError_t persist(hipStream_t s, int grid, int block, const PersistParams& p);
+// Chained loop: one init kernel, then one kernel per level (level index d),
+// each deciding its own direction from the device-side counters.
+hipError_t chain_init(hipStream_t s, int grid, const PersistParams& p);
+hipError_t chain_level(hipStream_t s, int grid, const PersistParams& p, int32_t d);
 }  // namespace kern
 
-constexpr int kLbBlock = 256;       // threads per block of the load-balanced kernels
-constexpr int kPersistBlock = 256;  // threads per block of the persistent kernel
+constexpr int kLbBlock = 256;   // threads per block of the load-balanced kernels
+constexpr int kMaxBlock = 1024; // largest workgroup of any level kernel
 
 }  // namespace tbfs

@@ -53,8 +53,8 @@ struct LevelAcc {
 // Sum the tallies of the workgroup and add them to the level counters with
 // one atomic per field.  Every thread of the workgroup must call it.
 __device__ __forceinline__ void flush_level(LevelCtr* ctr, LevelAcc& a) {
-  __shared__ unsigned long long red[kPersistBlock / 64][3];
-  __shared__ unsigned int redo[kPersistBlock / 64];
+  __shared__ unsigned long long red[kMaxBlock / 64][3];
+  __shared__ unsigned int redo[kMaxBlock / 64];
   const uint64_t q = wave_sum_u64(a.q);
   const uint64_t mo = wave_sum_u64(a.mout);
   const uint64_t mi = wave_sum_u64(a.min);
@@ -313,12 +313,16 @@ __device__ __forceinline__ unsigned ld_agent(const unsigned* p) {
 
 // Barrier number k (1, 2, ...) over all workgroups of a cooperative launch.
 // bar[0] counts arrivals (k * gridDim.x after barrier k), bar[1] is the number
-// of the last completed barrier.  Every wave makes its global writes visible
-// device-wide before arriving and picks up the others' after leaving.
+// of the last completed barrier.  Each wave waits for its own stores, then
+// one wave per workgroup makes the workgroup's writes visible device-wide
+// (L2 write-back, release) and, after the barrier, drops stale lines (L1/L2
+// invalidate, acquire) for the whole workgroup: one device-scope flush per
+// workgroup instead of one per wave.
 __device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned k) {
-  __threadfence();
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's loads, stores and LDS ops done
   __syncthreads();
   if (threadIdx.x == 0) {
+    __threadfence();
     const unsigned target = k * gridDim.x;
     const unsigned old = atomicAdd(&bar[0], 1u);
     if (old + 1 == target) {
@@ -327,24 +331,17 @@ __device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned k) {
       while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < k)
         __builtin_amdgcn_s_sleep(1);
     }
+    __threadfence();
   }
   __syncthreads();
-  __threadfence();
 }
 
-__global__ __launch_bounds__(kPersistBlock) void k_persist(PersistParams p) {
-  __shared__ WaveLds lds[kPersistBlock / 64];
-  WaveLds* L = &lds[threadIdx.x >> 6];
+__device__ __forceinline__ void init_state(const PersistParams& p, int64_t gtid,
+                                           int64_t gstride) {
   const GraphView& g = p.g;
-  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t wave = gtid >> 6, nwaves = gstride >> 6;
   const int32_t n = g.n;
   const int64_t nwords = ((int64_t)n + 63) >> 6;
   const int r = p.root;
-  const uint32_t rout = outdeg(g, r), rin = indeg(g, r);
-
-  // phase 0: state of depth 0
   for (int64_t i = gtid; i < n; i += gstride) {
     p.S[i] = (i == r) ? 0 : -1;
     p.sigma[i] = (i == r) ? 1ull : 0ull;
@@ -359,68 +356,147 @@ __global__ __launch_bounds__(kPersistBlock) void k_persist(PersistParams p) {
     LevelCtr* c = &p.ctr[gtid];
     c->q = gtid == 0 ? 1ull : 0ull;
     c->qe = 0ull;
-    c->mout = gtid == 0 ? rout : 0ull;
-    c->min = gtid == 0 ? rin : 0ull;
+    c->mout = gtid == 0 ? outdeg(g, r) : 0ull;
+    c->min = gtid == 0 ? indeg(g, r) : 0ull;
     c->ovf = 0;
   }
+}
+
+__device__ __forceinline__ LoopState first_state(const PersistParams& p) {
+  LoopState s{};
+  s.d = 0;
+  s.pushing = (p.mode == TBFS_MODE_AUTO || p.mode == TBFS_MODE_PUSH) ? 1 : 0;
+  s.done = 0;
+  s.ovf_any = 0;
+  s.mu = (unsigned long long)p.g.nz;
+  return s;
+}
+
+// Fold the counters of the current frontier (depth s.d) into the loop state,
+// choose this level's direction, and report whether the BFS goes on.  Every
+// workgroup evaluates it on the same inputs and gets the same answer.
+__device__ __forceinline__ bool step_state(const PersistParams& p,
+                                           const LevelCtr* c, LoopState& s) {
+  const unsigned long long q = c->q;
+  if (q == 0) {
+    s.done = 1;
+    return false;
+  }
+  const unsigned long long mf = c->mout;
+  const unsigned long long mi = c->min;
+  s.ovf_any |= c->ovf;
+  s.reached += q;
+  s.edges += mi;
+  s.mu = s.mu > mi ? s.mu - mi : 0ull;
+  if (p.mode == TBFS_MODE_AUTO) {
+    if (s.pushing && (double)mf * p.alpha > (double)s.mu) s.pushing = 0;
+    else if (!s.pushing && (double)q * p.beta < (double)p.g.n) s.pushing = 1;
+  }
+  if (s.pushing) ++s.push_levels;
+  else ++s.pull_levels;
+  return true;
+}
+
+__device__ __forceinline__ void write_stats(const PersistParams& p,
+                                            const LoopState& s) {
+  p.stats->levels = s.d;
+  p.stats->push_levels = s.push_levels;
+  p.stats->pull_levels = s.pull_levels;
+  p.stats->ovf_any = s.ovf_any;
+  p.stats->reached = s.reached;
+  p.stats->edges = s.edges;
+}
+
+// Housekeeping for depth d+2 plus the level's work (frontier depth d).
+__device__ __forceinline__ void run_level(const PersistParams& p, int d,
+                                          bool pushing, uint32_t ovf_any,
+                                          int64_t gtid, int64_t gstride,
+                                          WaveLds* L, LevelAcc& acc) {
+  const GraphView& g = p.g;
+  const int64_t nwords = ((int64_t)g.n + 63) >> 6;
+  const int64_t wave = gtid >> 6, nwaves = gstride >> 6;
+  // buffers of depth d+2 were last used by depth d-1: clear them now
+  uint64_t* fclr = p.fb[(d + 2) % 3];
+  for (int64_t w = gtid; w < nwords; w += gstride) fclr[w] = 0ull;
+  if (gtid == 0) {
+    LevelCtr* z = &p.ctr[(d + 2) % 3];
+    z->q = 0;
+    z->qe = 0;
+    z->mout = 0;
+    z->min = 0;
+    z->ovf = 0;
+  }
+  StateView st{p.S, p.sigma, p.ovf, p.fb[d % 3], p.fb[(d + 1) % 3],
+               nullptr, nullptr, &p.ctr[(d + 1) % 3]};
+  if (pushing) {
+    push_lb_level(g, st, d + 1, ovf_any, p.bigo, p.nbigo, wave, nwaves, L, acc);
+  } else if (p.sigma_on) {
+    pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc);
+  } else {
+    pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc);
+  }
+  flush_level(st.ctr, acc);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_persist(PersistParams p) {
+  __shared__ WaveLds lds[BLOCK / 64];
+  WaveLds* L = &lds[threadIdx.x >> 6];
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+
+  init_state(p, gtid, gstride);
   unsigned bk = 1;
   grid_barrier(p.bar, bk);
 
-  uint32_t ovf_any = 0;
-  bool pushing = (p.mode == TBFS_MODE_AUTO || p.mode == TBFS_MODE_PUSH);
-  unsigned long long mu = (unsigned long long)g.nz;
-  unsigned long long reached = 0, edges = 0;
-  int push_levels = 0, pull_levels = 0;
+  LoopState s = first_state(p);
   LevelAcc acc;
-  int d = 0;
-  for (;; ++d) {
-    const LevelCtr* c = &p.ctr[d % 3];
-    const unsigned long long q = ld_agent(&c->q);
-    if (q == 0) break;
-    const unsigned long long mf = ld_agent(&c->mout);
-    const unsigned long long mi = ld_agent(&c->min);
-    ovf_any |= ld_agent(&c->ovf);
-    reached += q;
-    edges += mi;
-    mu = mu > mi ? mu - mi : 0ull;
-    if (p.mode == TBFS_MODE_AUTO) {
-      if (pushing && (double)mf * p.alpha > (double)mu) pushing = false;
-      else if (!pushing && (double)q * p.beta < (double)n) pushing = true;
-    }
-    // buffers of depth d+2 were last used by depth d-1: clear them now
-    uint64_t* fclr = p.fb[(d + 2) % 3];
-    for (int64_t w = gtid; w < nwords; w += gstride) fclr[w] = 0ull;
-    if (gtid == 0) {
-      LevelCtr* z = &p.ctr[(d + 2) % 3];
-      z->q = 0;
-      z->qe = 0;
-      z->mout = 0;
-      z->min = 0;
-      z->ovf = 0;
-    }
-    StateView st{p.S, p.sigma, p.ovf, p.fb[d % 3], p.fb[(d + 1) % 3],
-                 nullptr, nullptr, &p.ctr[(d + 1) % 3]};
-    if (pushing) {
-      push_lb_level(g, st, d + 1, ovf_any, p.bigo, p.nbigo, wave, nwaves, L, acc);
-      ++push_levels;
-    } else {
-      if (p.sigma_on)
-        pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc);
-      else
-        pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc);
-      ++pull_levels;
-    }
-    flush_level(st.ctr, acc);
+  for (;;) {
+    LevelCtr c;
+    const LevelCtr* cp = &p.ctr[s.d % 3];
+    c.q = ld_agent(&cp->q);
+    c.mout = ld_agent(&cp->mout);
+    c.min = ld_agent(&cp->min);
+    c.ovf = ld_agent(&cp->ovf);
+    if (!step_state(p, &c, s)) break;
+    run_level(p, s.d, s.pushing != 0, s.ovf_any, gtid, gstride, L, acc);
+    ++s.d;
     grid_barrier(p.bar, ++bk);
   }
-  if (gtid == 0) {
-    p.stats->levels = d;
-    p.stats->push_levels = push_levels;
-    p.stats->pull_levels = pull_levels;
-    p.stats->ovf_any = ovf_any;
-    p.stats->reached = reached;
-    p.stats->edges = edges;
+  if (gtid == 0) write_stats(p, s);
+}
+
+// Chained loop: state of depth 0 and the first loop-state slot.
+__global__ __launch_bounds__(256) void k_chain_init(PersistParams p) {
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  init_state(p, gtid, gstride);
+  if (gtid == 0) p.loop[0] = first_state(p);
+}
+
+// Chained loop: level d.  Reads slot d & 1 and the counters of depth d (both
+// final: written by earlier kernels), writes slot (d + 1) & 1.
+__global__ __launch_bounds__(kLbBlock) void k_chain_level(PersistParams p, int32_t d) {
+  __shared__ WaveLds lds[kLbBlock / 64];
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  LoopState s = p.loop[d & 1];
+  if (s.done) {
+    if (gtid == 0) p.loop[(d + 1) & 1] = s;
+    return;
   }
+  if (!step_state(p, &p.ctr[d % 3], s)) {
+    if (gtid == 0) {
+      p.loop[(d + 1) & 1] = s;
+      write_stats(p, s);
+    }
+    return;
+  }
+  LoopState nx = s;
+  nx.d = d + 1;
+  if (gtid == 0) p.loop[(d + 1) & 1] = nx;
+  LevelAcc acc;
+  run_level(p, d, s.pushing != 0, s.ovf_any, gtid, gstride, &lds[threadIdx.x >> 6], acc);
 }
 
 }  // namespace
@@ -448,10 +524,11 @@ hipError_t pull_lb(hipStream_t s, int grid, const GraphView& g,
   return hipGetLastError();
 }
 
-hipError_t persist_grid(int device, int* grid) {
+hipError_t persist_grid(int device, int block, int* grid) {
   int per_cu = 0, cus = 0, coop = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void*>(k_persist), kPersistBlock, 0);
+  const void* fn = block == 1024 ? reinterpret_cast<const void*>(k_persist<1024>)
+                                 : reinterpret_cast<const void*>(k_persist<256>);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0);
   if (e != hipSuccess) return e;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) return e;
@@ -462,11 +539,22 @@ hipError_t persist_grid(int device, int* grid) {
   return hipSuccess;
 }
 
-hipError_t persist(hipStream_t s, int grid, const PersistParams& p) {
+hipError_t persist(hipStream_t s, int grid, int block, const PersistParams& p) {
   PersistParams arg = p;
   void* args[] = {&arg};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_persist),
-                                    dim3(grid), dim3(kPersistBlock), args, 0, s);
+  const void* fn = block == 1024 ? reinterpret_cast<const void*>(k_persist<1024>)
+                                 : reinterpret_cast<const void*>(k_persist<256>);
+  return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, 0, s);
+}
+
+hipError_t chain_init(hipStream_t s, int grid, const PersistParams& p) {
+  hipLaunchKernelGGL(k_chain_init, dim3(grid), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t chain_level(hipStream_t s, int grid, const PersistParams& p, int32_t d) {
+  hipLaunchKernelGGL(k_chain_level, dim3(grid), dim3(kLbBlock), 0, s, p, d);
+  return hipGetLastError();
 }
 
 }  // namespace kern
