@@ -40,7 +40,9 @@ struct tbfs_ctx {
   int64_t q_cap = 0;
   tbfs::LevelCtr* ctr = nullptr;    // device, 3 rotating slots (+1 spare)
   tbfs::LevelCtr* ctr_h = nullptr;  // pinned host mirror
-  unsigned* bar = nullptr;          // grid barrier words
+  unsigned* bar = nullptr;          // grid barrier words (kBarWords)
+  tbfs::CtrSet* cset = nullptr;     // device, 3 striped counter sets
+  tbfs::CtrSet* cset_h = nullptr;   // pinned host mirror of one set
   tbfs::RunStats* stats = nullptr;  // device, stats_cap slots
   tbfs::RunStats* stats_h = nullptr;
   int stats_cap = 0;

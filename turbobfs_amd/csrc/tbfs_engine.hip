@@ -350,6 +350,8 @@ void ctx_free(tbfs_ctx* c) {
   dev_free(c->bar);
   dev_free(c->stats);
   dev_free(c->loop);
+  dev_free(c->cset);
+  if (c->cset_h) (void)hipHostFree(c->cset_h);
   if (c->loop_h) (void)hipHostFree(c->loop_h);
   if (c->ctr_h) (void)hipHostFree(c->ctr_h);
   if (c->stats_h) (void)hipHostFree(c->stats_h);
@@ -380,10 +382,13 @@ int ctx_create(int32_t n_cap, tbfs_ctx** out) {
   if (!rc) rc = dev_alloc(&c->ovf, (size_t)n_cap);
   for (int i = 0; i < 3 && !rc; ++i) rc = dev_alloc(&c->fb[i], nw);
   if (!rc) rc = dev_alloc(&c->ctr, 4);
-  if (!rc) rc = dev_alloc(&c->bar, 16);
+  if (!rc) rc = dev_alloc(&c->bar, kBarWords);
+  if (!rc) rc = dev_alloc(&c->cset, 3);
   if (!rc) rc = dev_alloc(&c->stats, 1);
   if (!rc) rc = dev_alloc(&c->loop, 2);
   if (rc) goto fail;
+  e = hipHostMalloc((void**)&c->cset_h, sizeof(CtrSet), hipHostMallocDefault);
+  if (e != hipSuccess) { rc = hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); goto fail; }
   e = hipHostMalloc((void**)&c->loop_h, 2 * sizeof(LoopState), hipHostMallocDefault);
   if (e != hipSuccess) { rc = hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); goto fail; }
   e = hipHostMalloc((void**)&c->ctr_h, 4 * sizeof(LevelCtr), hipHostMallocDefault);
@@ -488,7 +493,7 @@ static PersistParams make_params(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   p.sigma = c->sigma;
   p.ovf = c->ovf;
   for (int i = 0; i < 3; ++i) p.fb[i] = c->fb[i];
-  p.ctr = c->ctr;
+  p.cset = c->cset;
   p.bar = c->bar;
   p.stats = stats_slot;
   p.bigc = g->bigc;
@@ -511,7 +516,7 @@ static int enqueue_persistent(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   const int block = o.block == 1024 ? 1024 : 256;
   int grid = block == 1024 ? c->persist_grid_max1024 : c->persist_grid_max;
   if (o.grid > 0 && o.grid < grid) grid = o.grid;
-  HIPCHK(hipMemsetAsync(c->bar, 0, 2 * sizeof(unsigned), c->stream));
+  HIPCHK(hipMemsetAsync(c->bar, 0, kBarWords * sizeof(unsigned), c->stream));
   HIPCHK(kern::persist(c->stream, grid, block, p));
   return TBFS_OK;
 }
@@ -522,8 +527,10 @@ static int run_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
                        const tbfs_opts& o, RunStats* stats_slot) {
   const PersistParams p = make_params(c, g, root, o, stats_slot);
   const int64_t nwords = ((int64_t)g->n + 63) / 64;
-  int grid = grid_for(nwords * 64, kLbBlock, 8192);
-  if (o.grid > 0 && o.grid < grid) grid = o.grid;
+  // about four frontier words per wavefront: fewer workgroups to dispatch
+  // per level and fewer counter atomics, still >> 256 CUs' worth of waves
+  int grid = grid_for(nwords * 16, kLbBlock, 2048);
+  if (o.grid > 0) grid = std::min(o.grid, 65536);
   hipStream_t s = c->stream;
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
   int32_t d = 0;
@@ -566,7 +573,7 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
   hipStream_t s = c->stream;
   GraphView gv{n, g->nz, g->CP, g->IC, g->RP, g->JR};
   const int fill_grid = grid_for(n, 256, 8192);
-  const int lb_grid = grid_for((int64_t)nwords * 64, kLbBlock, 8192);
+  const int lb_grid = grid_for((int64_t)nwords * 16, kLbBlock, 2048);
 
   // root degrees (the level-0 counters)
   int32_t cp2[2] = {0, 0}, rp2[2] = {0, 0};
@@ -601,9 +608,10 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
     }
     const int nxt = (cur + 1) % 3;
     StateView sv{c->S, c->sigma, c->ovf, c->fb[cur], c->fb[nxt],
-                 c->q[d & 1], c->q[(d + 1) & 1], c->ctr};
+                 c->q[d & 1], c->q[(d + 1) & 1], c->ctr, c->cset};
     HIPCHK(kern::level_prologue(s, grid_for(nwords, 256, 4096), c->fb[nxt],
-                                nwords, c->ctr));
+                                nwords, c->ctr, c->cset));
+    bool striped = false;  // load-balanced kernels count into the stripes
     switch (o.mode) {
       case TBFS_MODE_PULL_SC:
         HIPCHK(kern::pull_sc(s, grid_for(n, block, 65536), block, gv, sv, d,
@@ -623,17 +631,21 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
         break;
       default:  // PUSH, or AUTO on its current side
         if (pushing) {
-          if (o.pull == TBFS_PULL_LB)
+          if (o.pull == TBFS_PULL_LB) {
             HIPCHK(kern::push_lb(s, lb_grid, gv, sv, d, g->bigo, g->nbigo,
                                  (int)ovf_any));
+            striped = true;
+          }
           else
             HIPCHK(kern::push(s, grid_for(qe * 64, block, 65536), block, gv, sv,
                               d, qe, (int)ovf_any));
           ++push_levels;
         } else {
-          if (o.pull == TBFS_PULL_LB)
+          if (o.pull == TBFS_PULL_LB) {
             HIPCHK(kern::pull_lb(s, lb_grid, gv, sv, d, g->bigc, g->nbigc, sig,
                                  (int)ovf_any));
+            striped = true;
+          }
           else if (o.pull == TBFS_PULL_WA)
             HIPCHK(kern::pull_wa(s, grid_for((int64_t)nwords * 64, block, 65536),
                                  block, gv, sv, d, sig, o.bbreak, (int)ovf_any));
@@ -644,10 +656,21 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
         }
         break;
     }
-    HIPCHK(hipMemcpyAsync(c->ctr_h, c->ctr, sizeof(LevelCtr),
-                          hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    const LevelCtr lc = c->ctr_h[0];
+    LevelCtr lc{};
+    if (striped) {
+      HIPCHK(hipMemcpyAsync(c->cset_h, c->cset, sizeof(CtrSet), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      for (int i = 0; i < kStripes; ++i) {
+        lc.q += c->cset_h->s[i].qm >> 32;
+        lc.mout += c->cset_h->s[i].qm & 0xffffffffull;
+        lc.min += c->cset_h->s[i].min;
+        lc.ovf |= c->cset_h->s[i].ovf;
+      }
+    } else {
+      HIPCHK(hipMemcpyAsync(c->ctr_h, c->ctr, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      lc = c->ctr_h[0];
+    }
     if (lc.q == 0) break;
     qsize = (int64_t)lc.q;
     qe = (int64_t)lc.qe;

@@ -44,6 +44,24 @@ struct alignas(64) LevelCtr {
   unsigned long long pad1[3];
 };
 
+// Level counters of the load-balanced kernels, striped over kStripes cache
+// lines so that the per-workgroup atomics do not all hit one address: a
+// workgroup adds into stripe (blockIdx & (kStripes-1)); readers sum the
+// stripes.  qm packs the vertex count (high 32 bits) with the out-degree sum
+// (low 32 bits); both totals stay below 2^31, so one 64-bit atomic carries
+// both without interference.
+constexpr int kStripes = 16;
+struct alignas(128) CtrStripe {
+  unsigned long long qm;
+  unsigned long long min;
+  unsigned int ovf;
+  unsigned int pad0;
+  unsigned long long pad1[13];
+};
+struct CtrSet {
+  CtrStripe s[kStripes];
+};
+
 // Result counters of one BFS, written by the device-resident level loop.
 struct alignas(64) RunStats {
   int32_t levels;
@@ -73,6 +91,7 @@ struct StateView {
   const uint64_t* qcur;  // queue of depth d
   uint64_t* qnext;       // queue of depth d+1
   LevelCtr* ctr;         // counters of depth d+1 (must be zero on entry)
+  CtrSet* cset;          // the same, striped (load-balanced kernels)
 };
 
 // Loop state carried from one level to the next by the device-resident
@@ -98,8 +117,8 @@ struct PersistParams {
   uint64_t* sigma;
   uint8_t* ovf;
   uint64_t* fb[3];
-  LevelCtr* ctr;    // [3]
-  unsigned* bar;    // [2]: arrival count, generation (zero at launch)
+  CtrSet* cset;     // [3] striped level counters
+  unsigned* bar;    // kBarWords barrier words (zero at launch)
   RunStats* stats;
   const uint64_t* bigc;
   int32_t nbigc;
@@ -122,7 +141,7 @@ hipError_t init(hipStream_t s, int grid, int32_t n, int32_t root,
                 uint8_t* ovf, uint64_t* fb0, uint64_t* fb1, uint64_t* fb2,
                 uint64_t* q0);
 hipError_t level_prologue(hipStream_t s, int grid, uint64_t* fnext,
-                          int32_t nwords, LevelCtr* ctr);
+                          int32_t nwords, LevelCtr* ctr, CtrSet* cset);
 hipError_t pull_sc(hipStream_t s, int grid, int block, const GraphView& g,
                    const StateView& st, int32_t d, int sigma, int bbreak,
                    int ovf_any);
@@ -161,6 +180,7 @@ hipError_t chain_init(hipStream_t s, int grid, const PersistParams& p);
 hipError_t chain_level(hipStream_t s, int grid, const PersistParams& p, int32_t d);
 }  // namespace kern
 
+constexpr int kBarWords = 512;  // grid barrier: top count, generation, 8 group counts
 constexpr int kLbBlock = 256;   // threads per block of the load-balanced kernels
 constexpr int kMaxBlock = 1024; // largest workgroup of any level kernel
 

@@ -52,7 +52,7 @@ __global__ void k_init(int32_t n, int32_t root, int32_t root_out_deg,
 }
 
 __global__ void k_level_prologue(uint64_t* fnext, int32_t nwords,
-                                 LevelCtr* ctr) {
+                                 LevelCtr* ctr, CtrSet* cset) {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t w = tid; w < nwords; w += stride) fnext[w] = 0ull;
@@ -62,6 +62,11 @@ __global__ void k_level_prologue(uint64_t* fnext, int32_t nwords,
     ctr->mout = 0;
     ctr->min = 0;
     ctr->ovf = 0;
+  }
+  if (tid < kStripes) {
+    cset->s[tid].qm = 0;
+    cset->s[tid].min = 0;
+    cset->s[tid].ovf = 0;
   }
 }
 
@@ -312,9 +317,9 @@ hipError_t init(hipStream_t s, int grid, int32_t n, int32_t root,
 }
 
 hipError_t level_prologue(hipStream_t s, int grid, uint64_t* fnext,
-                          int32_t nwords, LevelCtr* ctr) {
+                          int32_t nwords, LevelCtr* ctr, CtrSet* cset) {
   hipLaunchKernelGGL(k_level_prologue, dim3(grid), dim3(256), 0, s, fnext,
-                     nwords, ctr);
+                     nwords, ctr, cset);
   return hipGetLastError();
 }
 

@@ -50,9 +50,9 @@ struct LevelAcc {
   }
 };
 
-// Sum the tallies of the workgroup and add them to the level counters with
-// one atomic per field.  Every thread of the workgroup must call it.
-__device__ __forceinline__ void flush_level(LevelCtr* ctr, LevelAcc& a) {
+// Sum the tallies of the workgroup and add them to its stripe of the level
+// counters (two 64-bit atomics).  Every thread of the workgroup must call it.
+__device__ __forceinline__ void flush_level(CtrSet* set, LevelAcc& a) {
   __shared__ unsigned long long red[kMaxBlock / 64][3];
   __shared__ unsigned int redo[kMaxBlock / 64];
   const uint64_t q = wave_sum_u64(a.q);
@@ -76,15 +76,45 @@ __device__ __forceinline__ void flush_level(LevelCtr* ctr, LevelAcc& a) {
       t2 += red[i][2];
       to |= redo[i];
     }
+    CtrStripe* st = &set->s[blockIdx.x & (kStripes - 1)];
     if (t0) {
-      atomicAdd(&ctr->q, t0);
-      atomicAdd(&ctr->mout, t1);
-      atomicAdd(&ctr->min, t2);
+      atomicAdd(&st->qm, (t0 << 32) + t1);
+      atomicAdd(&st->min, t2);
     }
-    if (to) atomicOr(&ctr->ovf, 1u);
+    if (to) atomicOr(&st->ovf, 1u);
   }
   __syncthreads();
   a = LevelAcc();
+}
+
+// Totals of a striped counter set, on every lane of the calling wavefront
+// (lanes 0..kStripes-1 load one stripe each).  `agent`: the loads bypass
+// caches that may hold values from before the last grid barrier.
+__device__ __forceinline__ LevelCtr read_set(const CtrSet* set, bool agent) {
+  uint64_t qm = 0, mi = 0;
+  uint32_t ov = 0;
+  const int l = lane();
+  if (l < kStripes) {
+    const CtrStripe* st = &set->s[l];
+    if (agent) {
+      qm = __hip_atomic_load(&st->qm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      mi = __hip_atomic_load(&st->min, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ov = __hip_atomic_load(&st->ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      qm = st->qm;
+      mi = st->min;
+      ov = st->ovf;
+    }
+  }
+  const uint64_t q = wave_sum_u64(qm >> 32);
+  const uint64_t mo = wave_sum_u64(qm & 0xffffffffull);
+  LevelCtr c;
+  c.q = q;
+  c.qe = 0;
+  c.mout = mo;
+  c.min = wave_sum_u64(mi);
+  c.ovf = __ballot(ov != 0) ? 1u : 0u;
+  return c;
 }
 
 // One edge u -> v of a push level, u at depth d with count x (overflow ou).
@@ -284,7 +314,7 @@ __global__ __launch_bounds__(kLbBlock) void k_push_lb(GraphView g, StateView st,
   LevelAcc a;
   push_lb_level(g, st, d + 1, (uint32_t)ovf_any, bigo, nbigo, wave, nwaves,
                 &lds[threadIdx.x >> 6], a);
-  flush_level(st.ctr, a);
+  flush_level(st.cset, a);
 }
 
 template <bool SIGMA>
@@ -299,36 +329,37 @@ __global__ __launch_bounds__(kLbBlock) void k_pull_lb(GraphView g, StateView st,
   LevelAcc a;
   pull_lb_level<SIGMA>(g, st, d + 1, (uint32_t)ovf_any, bigc, nbigc, wave,
                        nwaves, &lds[threadIdx.x >> 6], a);
-  flush_level(st.ctr, a);
+  flush_level(st.cset, a);
 }
 
 // ----------------------------------------------------------------------------
 // device-resident level loop
-__device__ __forceinline__ unsigned long long ld_agent(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned ld_agent(const unsigned* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Barrier number k (1, 2, ...) over all workgroups of a cooperative launch.
-// bar[0] counts arrivals (k * gridDim.x after barrier k), bar[1] is the number
-// of the last completed barrier.  Each wave waits for its own stores, then
-// one wave per workgroup makes the workgroup's writes visible device-wide
-// (L2 write-back, release) and, after the barrier, drops stale lines (L1/L2
-// invalidate, acquire) for the whole workgroup: one device-scope flush per
-// workgroup instead of one per wave.
+// Barrier number k (1, 2, ...) over all workgroups of a cooperative launch,
+// in two levels so that no address takes more than ~gridDim/8 atomics: the
+// workgroups with blockIdx % 8 == g count into group word g; the last of
+// each group counts into the top word; the last group publishes k as the
+// generation, which every workgroup polls.  Each wave waits for its own
+// memory operations, then one wave per workgroup makes the workgroup's
+// writes visible device-wide (release) and, after the barrier, drops stale
+// cached lines for the workgroup (acquire).
 __device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned k) {
-  __builtin_amdgcn_s_waitcnt(0);  // this wave's loads, stores and LDS ops done
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
-    const unsigned target = k * gridDim.x;
-    const unsigned old = atomicAdd(&bar[0], 1u);
-    if (old + 1 == target) {
-      __hip_atomic_store(&bar[1], k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned ng = gridDim.x < 8 ? gridDim.x : 8u;
+    const unsigned g = blockIdx.x & 7u;
+    const unsigned gsize = (gridDim.x - g + 7u) / 8u;
+    unsigned* top = &bar[0];
+    unsigned* gen = &bar[32];
+    unsigned* grp = &bar[64 + 32 * g];
+    const unsigned old = atomicAdd(grp, 1u);
+    bool last = false;
+    if (old + 1 == k * gsize) last = (atomicAdd(top, 1u) + 1 == k * ng);
+    if (last) {
+      __hip_atomic_store(gen, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < k)
+      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < k)
         __builtin_amdgcn_s_sleep(1);
     }
     __threadfence();
@@ -352,12 +383,11 @@ __device__ __forceinline__ void init_state(const PersistParams& p, int64_t gtid,
     p.fb[1][w] = 0ull;
     p.fb[2][w] = 0ull;
   }
-  if (gtid < 3) {
-    LevelCtr* c = &p.ctr[gtid];
-    c->q = gtid == 0 ? 1ull : 0ull;
-    c->qe = 0ull;
-    c->mout = gtid == 0 ? outdeg(g, r) : 0ull;
-    c->min = gtid == 0 ? indeg(g, r) : 0ull;
+  if (gtid < 3 * kStripes) {
+    CtrStripe* c = &p.cset[gtid / kStripes].s[gtid % kStripes];
+    const bool root_slot = gtid == 0;
+    c->qm = root_slot ? ((1ull << 32) + outdeg(g, r)) : 0ull;
+    c->min = root_slot ? indeg(g, r) : 0ull;
     c->ovf = 0;
   }
 }
@@ -418,16 +448,14 @@ __device__ __forceinline__ void run_level(const PersistParams& p, int d,
   // buffers of depth d+2 were last used by depth d-1: clear them now
   uint64_t* fclr = p.fb[(d + 2) % 3];
   for (int64_t w = gtid; w < nwords; w += gstride) fclr[w] = 0ull;
-  if (gtid == 0) {
-    LevelCtr* z = &p.ctr[(d + 2) % 3];
-    z->q = 0;
-    z->qe = 0;
-    z->mout = 0;
+  if (gtid < kStripes) {
+    CtrStripe* z = &p.cset[(d + 2) % 3].s[gtid];
+    z->qm = 0;
     z->min = 0;
     z->ovf = 0;
   }
   StateView st{p.S, p.sigma, p.ovf, p.fb[d % 3], p.fb[(d + 1) % 3],
-               nullptr, nullptr, &p.ctr[(d + 1) % 3]};
+               nullptr, nullptr, nullptr, &p.cset[(d + 1) % 3]};
   if (pushing) {
     push_lb_level(g, st, d + 1, ovf_any, p.bigo, p.nbigo, wave, nwaves, L, acc);
   } else if (p.sigma_on) {
@@ -435,7 +463,7 @@ __device__ __forceinline__ void run_level(const PersistParams& p, int d,
   } else {
     pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc);
   }
-  flush_level(st.ctr, acc);
+  flush_level(st.cset, acc);
 }
 
 template <int BLOCK>
@@ -452,12 +480,7 @@ __global__ __launch_bounds__(BLOCK) void k_persist(PersistParams p) {
   LoopState s = first_state(p);
   LevelAcc acc;
   for (;;) {
-    LevelCtr c;
-    const LevelCtr* cp = &p.ctr[s.d % 3];
-    c.q = ld_agent(&cp->q);
-    c.mout = ld_agent(&cp->mout);
-    c.min = ld_agent(&cp->min);
-    c.ovf = ld_agent(&cp->ovf);
+    const LevelCtr c = read_set(&p.cset[s.d % 3], true);
     if (!step_state(p, &c, s)) break;
     run_level(p, s.d, s.pushing != 0, s.ovf_any, gtid, gstride, L, acc);
     ++s.d;
@@ -485,7 +508,8 @@ __global__ __launch_bounds__(kLbBlock) void k_chain_level(PersistParams p, int32
     if (gtid == 0) p.loop[(d + 1) & 1] = s;
     return;
   }
-  if (!step_state(p, &p.ctr[d % 3], s)) {
+  const LevelCtr c = read_set(&p.cset[d % 3], false);
+  if (!step_state(p, &c, s)) {
     if (gtid == 0) {
       p.loop[(d + 1) & 1] = s;
       write_stats(p, s);
