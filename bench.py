@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--beta", type=float, default=0.0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-mib", type=int, default=512,
+                    help="size of the plain copy kernel run first (byte-counter reference)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -138,6 +140,11 @@ def main():
     opts = tb.default_opts(**kw)
     gr = tb.Graph(g.CP, g.IC, symmetric=True)
     ctx = tb.Context(g.n)
+
+    # a plain copy of known size: achievable HBM bandwidth, and the reference
+    # dispatch for the FETCH_SIZE / WRITE_SIZE counters of a profiled run
+    calib_bytes = args.calib_mib << 20
+    copy_ms = ctx.calibrate_copy(calib_bytes, 3) if args.calib_mib > 0 else 0.0
 
     roots = np.array([sources[i % len(sources)] for i in range(args.steps)], np.int32)
     if args.warmup:
@@ -180,6 +187,7 @@ def main():
             "pull_levels": float(np.mean([st["pull_levels"] for st in stats])),
             "bfs_ms_median": float(np.median([st["ms"] for st in stats])),
         },
+        "copy_kernel_gbs": (2 * calib_bytes / (copy_ms * 1e-3) / 1e9) if copy_ms else None,
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,

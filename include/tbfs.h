@@ -150,6 +150,13 @@ int tbfs_ctx_read(tbfs_ctx *c, int32_t n, int32_t *S, uint64_t *sigma,
 int tbfs_ctx_mark(tbfs_ctx *c, int32_t slot);
 int tbfs_ctx_elapsed(tbfs_ctx *c, int32_t a, int32_t b, double *ms);
 
+/* Copy `bytes` (rounded down to 16) between two fresh device buffers with a
+ * plain streaming kernel, `reps` times, on the context's stream; *ms gets the
+ * average kernel time.  A known-size reference for hardware byte counters
+ * (each launch reads `bytes` and writes `bytes`) and for achievable HBM
+ * bandwidth. */
+int tbfs_calibrate_copy(tbfs_ctx *c, int64_t bytes, int32_t reps, double *ms);
+
 /* Number of visible HIP devices (0 when none; never fails for lack of one). */
 int tbfs_device_count(int32_t *count);
 

@@ -98,6 +98,7 @@ SIGNATURES = {
     "tbfs_ctx_device_ptrs": (C.c_int, [C.c_void_p, C.POINTER(_i32p), C.POINTER(_u64p), C.POINTER(_u8p)]),
     "tbfs_ctx_mark": (C.c_int, [C.c_void_p, C.c_int32]),
     "tbfs_ctx_elapsed": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    "tbfs_calibrate_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "tbfs_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
     "tbfs_device_info": (C.c_int, [C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),

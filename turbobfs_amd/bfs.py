@@ -191,6 +191,13 @@ class Context:
               "tbfs_ctx_elapsed")
         return float(ms.value)
 
+    def calibrate_copy(self, nbytes: int, reps: int = 5) -> float:
+        """Average ms of a plain device copy of nbytes (reads and writes nbytes)."""
+        ms = C.c_double()
+        check(_lib.load().tbfs_calibrate_copy(self._h, int(nbytes), int(reps), C.byref(ms)),
+              "tbfs_calibrate_copy")
+        return float(ms.value)
+
     def read(self, n: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         S = np.empty(n, dtype=np.int32)
         sigma = np.empty(n, dtype=np.uint64)

@@ -863,6 +863,32 @@ int tbfs_ctx_elapsed(tbfs_ctx* c, int32_t a, int32_t b, double* ms) {
   return TBFS_OK;
 }
 
+int tbfs_calibrate_copy(tbfs_ctx* c, int64_t bytes, int32_t reps, double* ms) {
+  if (!c || !ms || bytes < 16 || reps < 1) return TBFS_ERR_INVALID;
+  const int64_t n16 = bytes / 16;
+  void *src = nullptr, *dst = nullptr;
+  HIPCHK(hipMalloc(&src, (size_t)n16 * 16));
+  hipError_t e = hipMalloc(&dst, (size_t)n16 * 16);
+  if (e != hipSuccess) {
+    (void)hipFree(src);
+    return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+  }
+  int rc = TBFS_OK;
+  float f = 0.f;
+  e = hipMemsetAsync(src, 1, (size_t)n16 * 16, c->stream);
+  if (e == hipSuccess) e = kern::copy16(c->stream, src, dst, n16);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+  for (int i = 0; i < reps && e == hipSuccess; ++i) e = kern::copy16(c->stream, src, dst, n16);
+  if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&f, c->ev0, c->ev1);
+  if (e != hipSuccess) rc = hip_fail(e, "calibrate_copy", __FILE__, __LINE__);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  *ms = (double)f / reps;
+  return rc;
+}
+
 int tbfs_device_count(int32_t* count) {
   if (!count) return TBFS_ERR_INVALID;
   int n = 0;

@@ -155,6 +155,8 @@ hipError_t cooc_scatter(hipStream_t s, int grid, int block, const int32_t* I,
                         int ovf_any);
 hipError_t cooc_finish(hipStream_t s, int grid, int block, const GraphView& g,
                        const StateView& st, int32_t d);
+// plain streaming copy of n16 16-byte words (calibration)
+hipError_t copy16(hipStream_t s, const void* src, void* dst, int64_t n16);
 // setup helpers
 hipError_t expand_cols(hipStream_t s, const int32_t* CP, int32_t n,
                        int32_t* col);

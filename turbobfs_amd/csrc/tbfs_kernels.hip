@@ -271,6 +271,17 @@ __global__ __launch_bounds__(256) void k_cooc_finish(GraphView g, StateView st,
 }
 
 // ----------------------------------------------------------------------------
+// calibration: one 16-byte load and store per thread and iteration
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src,
+                                                uint4* __restrict__ dst,
+                                                int64_t n16) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += stride)
+    dst[i] = src[i];
+}
+
+// ----------------------------------------------------------------------------
 // setup kernels
 __global__ void k_expand_cols(const int32_t* CP, int32_t n, int32_t* col) {
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -365,6 +376,15 @@ hipError_t cooc_scatter(hipStream_t s, int grid, int block, const int32_t* I,
 hipError_t cooc_finish(hipStream_t s, int grid, int block, const GraphView& g,
                        const StateView& st, int32_t d) {
   hipLaunchKernelGGL(k_cooc_finish, dim3(grid), dim3(block), 0, s, g, st, d);
+  return hipGetLastError();
+}
+
+hipError_t copy16(hipStream_t s, const void* src, void* dst, int64_t n16) {
+  int64_t grid = (n16 + 255) / 256;
+  if (grid > 8192) grid = 8192;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)grid), dim3(256), 0, s,
+                     (const uint4*)src, (uint4*)dst, n16);
   return hipGetLastError();
 }
 
