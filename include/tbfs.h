@@ -93,7 +93,7 @@ typedef struct tbfs_stats {
 } tbfs_stats;
 
 /* Fill *o with the defaults: AUTO, load-balanced kernels, persistent loop,
- * sigma on, alpha 4, beta 24. */
+ * sigma on, alpha 14, beta 24. */
 void tbfs_opts_default(tbfs_opts *o);
 
 /* Human-readable name of a status code. */

@@ -25,6 +25,7 @@ struct tbfs_graph {
   int32_t nbigc = 0;
   uint64_t* bigo = nullptr;  // static push slices of long out-lists (may alias bigc)
   int32_t nbigo = 0;
+  uint64_t* skip = nullptr;  // static: in-degree 0 or > kBigIn (one bit per vertex)
 };
 
 // Per-BFS device state bound to one stream.
@@ -36,6 +37,7 @@ struct tbfs_ctx {
   uint64_t* sigma = nullptr;
   uint8_t* ovf = nullptr;
   uint64_t* fb[3] = {nullptr, nullptr, nullptr};
+  uint64_t* vis = nullptr;          // reached-or-skipped bitmap
   uint64_t* q[2] = {nullptr, nullptr};
   int64_t q_cap = 0;
   tbfs::LevelCtr* ctr = nullptr;    // device, 3 rotating slots (+1 spare)

@@ -86,6 +86,7 @@ void graph_free(tbfs_graph* g) {
   if (g->J_owned) dev_free(g->J);
   if (g->I != g->IC) dev_free(g->I);
   if (g->bigo != g->bigc) dev_free(g->bigo);
+  dev_free(g->skip);
   dev_free(g->bigc);
   dev_free(g->CP);
   dev_free(g->IC);
@@ -328,6 +329,23 @@ static int build_slices(const int32_t* ptr_dev, int32_t n, uint64_t** out,
 int graph_build_bigc(tbfs_graph* g) {
   int rc = build_slices(g->CP, g->n, &g->bigc, &g->nbigc);
   if (rc) return rc;
+  {
+    std::vector<int32_t> cp((size_t)g->n + 1);
+    HIPCHK(hipMemcpy(cp.data(), g->CP, cp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const size_t nw = ((size_t)g->n + 63) / 64;
+    std::vector<uint64_t> skip(nw, 0ull);
+    for (size_t v = 0; v < nw * 64; ++v) {
+      bool sk = true;  // bits past n count as skipped
+      if (v < (size_t)g->n) {
+        const int64_t deg = (int64_t)cp[v + 1] - cp[v];
+        sk = deg == 0 || deg > kBigIn;
+      }
+      if (sk) skip[v >> 6] |= 1ull << (v & 63);
+    }
+    rc = dev_alloc(&g->skip, nw);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(g->skip, skip.data(), nw * sizeof(uint64_t), hipMemcpyHostToDevice));
+  }
   if (g->RP == g->CP) {
     g->bigo = g->bigc;
     g->nbigo = g->nbigc;
@@ -345,6 +363,7 @@ void ctx_free(tbfs_ctx* c) {
   dev_free(c->sigma);
   dev_free(c->ovf);
   for (int i = 0; i < 3; ++i) dev_free(c->fb[i]);
+  dev_free(c->vis);
   for (int i = 0; i < 2; ++i) dev_free(c->q[i]);
   dev_free(c->ctr);
   dev_free(c->bar);
@@ -381,6 +400,7 @@ int ctx_create(int32_t n_cap, tbfs_ctx** out) {
   if (!rc) rc = dev_alloc(&c->sigma, (size_t)n_cap);
   if (!rc) rc = dev_alloc(&c->ovf, (size_t)n_cap);
   for (int i = 0; i < 3 && !rc; ++i) rc = dev_alloc(&c->fb[i], nw);
+  if (!rc) rc = dev_alloc(&c->vis, nw);
   if (!rc) rc = dev_alloc(&c->ctr, 4);
   if (!rc) rc = dev_alloc(&c->bar, kBarWords);
   if (!rc) rc = dev_alloc(&c->cset, 3);
@@ -493,6 +513,8 @@ static PersistParams make_params(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   p.sigma = c->sigma;
   p.ovf = c->ovf;
   for (int i = 0; i < 3; ++i) p.fb[i] = c->fb[i];
+  p.vis = c->vis;
+  p.skip = g->skip;
   p.cset = c->cset;
   p.bar = c->bar;
   p.stats = stats_slot;
@@ -503,7 +525,7 @@ static PersistParams make_params(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   p.root = root;
   p.mode = o.mode;
   p.sigma_on = o.sigma ? 1 : 0;
-  p.alpha = o.alpha > 0 ? o.alpha : 4.0f;
+  p.alpha = o.alpha > 0 ? o.alpha : 14.0f;
   p.beta = o.beta > 0 ? o.beta : 24.0f;
   p.loop = c->loop;
   return p;
@@ -521,16 +543,33 @@ static int enqueue_persistent(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   return TBFS_OK;
 }
 
-// Chained loop for one BFS: init, then chunks of chain_k level kernels; the
-// host reads the loop state once per chunk.  Ends with the stream drained.
-static int run_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
-                       const tbfs_opts& o, RunStats* stats_slot) {
-  const PersistParams p = make_params(c, g, root, o, stats_slot);
+static int chain_grid(const tbfs_graph* g, const tbfs_opts& o) {
   const int64_t nwords = ((int64_t)g->n + 63) / 64;
   // about four frontier words per wavefront: fewer workgroups to dispatch
   // per level and fewer counter atomics, still >> 256 CUs' worth of waves
   int grid = grid_for(nwords * 16, kLbBlock, 2048);
   if (o.grid > 0) grid = std::min(o.grid, 65536);
+  return grid;
+}
+
+// Chained loop, enqueue only: init plus `levels` level kernels.  The BFS has
+// finished within them iff stats_slot->levels >= 0 afterwards.
+static int enqueue_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
+                           const tbfs_opts& o, RunStats* stats_slot, int levels) {
+  const PersistParams p = make_params(c, g, root, o, stats_slot);
+  const int grid = chain_grid(g, o);
+  hipStream_t s = c->stream;
+  HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
+  for (int d = 0; d < levels; ++d) HIPCHK(kern::chain_level(s, grid, p, d));
+  return TBFS_OK;
+}
+
+// Chained loop for one BFS: init, then chunks of chain_k level kernels; the
+// host reads the loop state once per chunk.  Ends with the stream drained.
+static int run_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
+                       const tbfs_opts& o, RunStats* stats_slot) {
+  const PersistParams p = make_params(c, g, root, o, stats_slot);
+  const int grid = chain_grid(g, o);
   hipStream_t s = c->stream;
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
   int32_t d = 0;
@@ -542,8 +581,6 @@ static int run_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
     HIPCHK(hipStreamSynchronize(s));
     if (c->loop_h->done) break;
   }
-  // next BFS on this context: enough levels for this one plus a margin
-  c->chain_k = std::max(4, std::min(64, c->loop_h->d + 2));
   return TBFS_OK;
 }
 
@@ -584,7 +621,7 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
   }
   HIPCHK(hipEventRecord(c->ev0, s));
   HIPCHK(kern::init(s, fill_grid, n, root, rp2[1] - rp2[0], c->S, c->sigma,
-                    c->ovf, c->fb[0], c->fb[1], c->fb[2], c->q[0]));
+                    c->ovf, c->fb[0], c->fb[1], c->fb[2], c->q[0], c->vis, g->skip));
 
   int64_t qsize = 1;
   int64_t qe = (rp2[1] - rp2[0] + kPushChunk - 1) / kPushChunk;
@@ -597,7 +634,7 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
   int32_t d = 0;
   int push_levels = 0, pull_levels = 0;
   bool pushing = (o.mode == TBFS_MODE_PUSH || o.mode == TBFS_MODE_AUTO);
-  const float alpha = o.alpha > 0 ? o.alpha : 4.0f;
+  const float alpha = o.alpha > 0 ? o.alpha : 14.0f;
   const float beta = o.beta > 0 ? o.beta : 24.0f;
   const int sig = o.sigma ? 1 : 0;
 
@@ -608,9 +645,9 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
     }
     const int nxt = (cur + 1) % 3;
     StateView sv{c->S, c->sigma, c->ovf, c->fb[cur], c->fb[nxt],
-                 c->q[d & 1], c->q[(d + 1) & 1], c->ctr, c->cset};
+                 c->q[d & 1], c->q[(d + 1) & 1], c->ctr, c->cset, c->vis};
     HIPCHK(kern::level_prologue(s, grid_for(nwords, 256, 4096), c->fb[nxt],
-                                nwords, c->ctr, c->cset));
+                                nwords, c->ctr, c->cset, c->fb[cur], c->vis));
     bool striped = false;  // load-balanced kernels count into the stripes
     switch (o.mode) {
       case TBFS_MODE_PULL_SC:
@@ -722,24 +759,49 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
   if (!rc) rc = ctx_reserve_events(c, (size_t)count + 1);
   if (rc) return rc;
   hipStream_t s = c->stream;
+  // Chained sources are enqueued back to back with a level budget of
+  // chain_k kernels each; a source that needs more is found afterwards from
+  // its stats slot and run again with host checks.
+  const int budget = c->chain_k;
   HIPCHK(hipEventRecord(c->evs[0], s));
   for (int i = 0; i < count; ++i) {
     rc = loop == TBFS_LOOP_PERSISTENT
              ? enqueue_persistent(c, g, roots[i], o, c->stats + i)
-             : run_chained(c, g, roots[i], o, c->stats + i);
+             : enqueue_chained(c, g, roots[i], o, c->stats + i, budget);
     if (rc) return rc;
     HIPCHK(hipEventRecord(c->evs[i + 1], s));
   }
   HIPCHK(hipMemcpyAsync(c->stats_h, c->stats, (size_t)count * sizeof(RunStats),
                         hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  std::vector<float> ms((size_t)count, 0.f);
+  for (int i = 0; i < count; ++i)
+    HIPCHK(hipEventElapsedTime(&ms[i], c->evs[i], c->evs[i + 1]));
+  if (loop == TBFS_LOOP_CHAINED) {
+    bool redo_last = false;
+    int maxlev = 0;
+    for (int i = 0; i < count; ++i) {
+      if (c->stats_h[i].levels < 0) {
+        HIPCHK(hipEventRecord(c->ev0, s));
+        rc = run_chained(c, g, roots[i], o, c->stats + i);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(c->ev1, s));
+        HIPCHK(hipEventSynchronize(c->ev1));
+        HIPCHK(hipEventElapsedTime(&ms[i], c->ev0, c->ev1));
+        HIPCHK(hipMemcpy(&c->stats_h[i], c->stats + i, sizeof(RunStats), hipMemcpyDeviceToHost));
+        redo_last = i != count - 1;
+      }
+      maxlev = std::max(maxlev, c->stats_h[i].levels);
+    }
+    if (redo_last) {  // leave the last source's results in the context
+      rc = run_chained(c, g, roots[count - 1], o, c->stats + count - 1);
+      if (rc) return rc;
+    }
+    c->chain_k = std::max(4, std::min(4096, maxlev + 2));
+  }
   c->last_n = g->n;
   if (st) {
-    for (int i = 0; i < count; ++i) {
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, c->evs[i], c->evs[i + 1]));
-      fill_stats(&st[i], c->stats_h[i], ms);
-    }
+    for (int i = 0; i < count; ++i) fill_stats(&st[i], c->stats_h[i], ms[i]);
   }
   return TBFS_OK;
 }
@@ -772,7 +834,7 @@ void tbfs_opts_default(tbfs_opts* o) {
   o->loop = TBFS_LOOP_HOST;
   o->sigma = 1;
   o->bbreak = 0;
-  o->alpha = 4.0f;
+  o->alpha = 14.0f;
   o->beta = 24.0f;
   o->pull = TBFS_PULL_LB;
   o->loop = TBFS_LOOP_PERSISTENT;

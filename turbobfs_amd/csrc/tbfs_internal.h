@@ -12,8 +12,12 @@
 //   S int32[n]                    depth, -1 = not reached
 //   sigma uint64[n], ovf uint8[n] shortest-path count mod 2^64 + sticky
 //                                 "count >= 2^64" flag
+//   skip uint64[ceil(n/64)]       static bitmap: in-degree 0, or in-degree
+//                                 > kBigIn (pulled through bigc slices)
 //   fb[3] uint64[ceil(n/64)]      frontier bitmaps (bit v&63 of word v>>6);
 //                                 current / next / being cleared, rotating
+//   vis uint64[ceil(n/64)]        skip | every frontier so far: a clear bit is
+//                                 an unreached column of the light pull path
 //   q[2] uint64[q_cap]            frontier work queues of the per-level
 //                                 reference-mapping kernels (current / next):
 //                                 one entry per kPushChunk-entry slice of a
@@ -92,6 +96,7 @@ struct StateView {
   uint64_t* qnext;       // queue of depth d+1
   LevelCtr* ctr;         // counters of depth d+1 (must be zero on entry)
   CtrSet* cset;          // the same, striped (load-balanced kernels)
+  uint64_t* vis;         // reached-or-skipped bitmap (load-balanced pull)
 };
 
 // Loop state carried from one level to the next by the device-resident
@@ -117,6 +122,8 @@ struct PersistParams {
   uint64_t* sigma;
   uint8_t* ovf;
   uint64_t* fb[3];
+  uint64_t* vis;          // reached-or-skipped bitmap
+  const uint64_t* skip;   // static: in-degree 0 or split column (graph)
   CtrSet* cset;     // [3] striped level counters
   unsigned* bar;    // kBarWords barrier words (zero at launch)
   RunStats* stats;
@@ -139,9 +146,10 @@ namespace kern {
 hipError_t init(hipStream_t s, int grid, int32_t n, int32_t root,
                 int32_t root_out_deg, int32_t* S, uint64_t* sigma,
                 uint8_t* ovf, uint64_t* fb0, uint64_t* fb1, uint64_t* fb2,
-                uint64_t* q0);
+                uint64_t* q0, uint64_t* vis, const uint64_t* skip);
 hipError_t level_prologue(hipStream_t s, int grid, uint64_t* fnext,
-                          int32_t nwords, LevelCtr* ctr, CtrSet* cset);
+                          int32_t nwords, LevelCtr* ctr, CtrSet* cset,
+                          const uint64_t* fcur, uint64_t* vis);
 hipError_t pull_sc(hipStream_t s, int grid, int block, const GraphView& g,
                    const StateView& st, int32_t d, int sigma, int bbreak,
                    int ovf_any);

@@ -143,9 +143,13 @@ __device__ __forceinline__ void push_edge(const GraphView& g, const StateView& s
 }
 
 // ----------------------------------------------------------------------------
+constexpr int kUnroll = 4;  // independent edges in flight per lane
+
 // Push level, driven by the frontier bitmap: (a) static slices of the long
-// out-lists whose vertex is in the frontier, (b) for every other frontier
-// word, the out-lists of its set bits walked load-balanced by one wavefront.
+// out-lists whose vertex is in the frontier, (b) every other frontier vertex:
+// a wavefront loads 64 frontier words at once, and for each nonzero one
+// walks the out-lists of its set bits load-balanced, kUnroll edges per lane
+// in flight.
 __device__ __forceinline__ void push_lb_level(const GraphView& g,
                                               const StateView& st, int dn,
                                               uint32_t ovf_any,
@@ -153,53 +157,83 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
                                               int32_t nbigo, int64_t wave,
                                               int64_t nwaves, WaveLds* L,
                                               LevelAcc& a) {
-  for (int64_t i = wave; i < nbigo; i += nwaves) {
-    const uint64_t ent = bigo[i];
-    const int u = (int)(uint32_t)ent;
-    if (!fbit(st.fcur, u)) continue;
-    const int cb = g.RP[u] + (int)(ent >> 32) * kPullChunk;
-    const int ce = min(g.RP[u + 1], cb + kPullChunk);
-    const uint64_t x = st.sigma[u];
-    const bool ou = ovf_any && st.ovf[u];
-    for (int k = cb + lane(); k < ce; k += 64) push_edge(g, st, dn, g.JR[k], x, ou, a);
+  for (int64_t b = wave * 64; b < nbigo; b += nwaves * 64) {
+    const int64_t i = b + lane();
+    bool act = false;
+    uint64_t ent = 0;
+    if (i < nbigo) {
+      ent = bigo[i];
+      act = fbit(st.fcur, (int)(uint32_t)ent);
+    }
+    uint64_t todo = __ballot(act);
+    while (todo) {
+      const int j = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const uint64_t e = __shfl(ent, j);
+      const int u = (int)(uint32_t)e;
+      const int cb = g.RP[u] + (int)(e >> 32) * kPullChunk;
+      const int ce = min(g.RP[u + 1], cb + kPullChunk);
+      const uint64_t x = st.sigma[u];
+      const bool ou = ovf_any && st.ovf[u];
+      for (int k = cb + lane(); k < ce; k += 64) push_edge(g, st, dn, g.JR[k], x, ou, a);
+    }
   }
   const int64_t nwords = ((int64_t)g.n + 63) >> 6;
-  for (int64_t w = wave; w < nwords; w += nwaves) {
-    const uint64_t bits = st.fcur[w];
-    if (bits == 0) continue;
-    const int u = (int)(w * 64) + lane();
-    int beg = 0, deg = 0;
-    uint64_t x = 0;
-    uint32_t ou = 0;
-    if ((bits >> lane()) & 1ull) {
-      beg = g.RP[u];
-      const int dg = g.RP[u + 1] - beg;
-      if (dg <= kBigIn) {
-        deg = dg;
-        x = st.sigma[u];
-        if (ovf_any) ou = st.ovf[u];
+  for (int64_t sb = wave * 64; sb < nwords; sb += nwaves * 64) {
+    const int64_t wl = sb + lane();
+    const uint64_t bits_l = wl < nwords ? st.fcur[wl] : 0ull;
+    uint64_t todo = __ballot(bits_l != 0);
+    while (todo) {
+      const int j = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const uint64_t bits = __shfl(bits_l, j);
+      const int u = (int)((sb + j) * 64) + lane();
+      int beg = 0, deg = 0;
+      uint64_t x = 0;
+      uint32_t ou = 0;
+      if ((bits >> lane()) & 1ull) {
+        beg = g.RP[u];
+        const int dg = g.RP[u + 1] - beg;
+        if (dg <= kBigIn) {
+          deg = dg;
+          x = st.sigma[u];
+          if (ovf_any) ou = st.ovf[u];
+        }
       }
-    }
-    const uint32_t incl = wave_incl_scan((uint32_t)deg);
-    const int total = (int)__shfl(incl, 63);
-    if (total == 0) continue;
-    const uint64_t oumask = __ballot(ou != 0);
-    wave_lds_sync();
-    L->excl[lane()] = (int)incl - deg;
-    L->beg[lane()] = beg;
-    L->x[lane()] = x;
-    wave_lds_sync();
-    for (int e = lane(); e < total; e += 64) {
-      const int o = owner_of(L->excl, e);
-      const int k = L->beg[o] + (e - L->excl[o]);
-      push_edge(g, st, dn, g.JR[k], L->x[o], (oumask >> o) & 1ull, a);
+      const uint32_t incl = wave_incl_scan((uint32_t)deg);
+      const int total = (int)__shfl(incl, 63);
+      if (total == 0) continue;
+      const uint64_t oumask = __ballot(ou != 0);
+      wave_lds_sync();
+      L->excl[lane()] = (int)incl - deg;
+      L->beg[lane()] = beg;
+      L->x[lane()] = x;
+      wave_lds_sync();
+      for (int e0 = lane(); e0 < total; e0 += 64 * kUnroll) {
+        int v[kUnroll], o[kUnroll];
+#pragma unroll
+        for (int t = 0; t < kUnroll; ++t) {
+          const int e = e0 + 64 * t;
+          o[t] = -1;
+          if (e < total) {
+            o[t] = owner_of(L->excl, e);
+            v[t] = g.JR[L->beg[o[t]] + (e - L->excl[o[t]])];
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < kUnroll; ++t)
+          if (o[t] >= 0)
+            push_edge(g, st, dn, v[t], L->x[o[t]], (oumask >> o[t]) & 1ull, a);
+      }
     }
   }
 }
 
 // Pull level: (a) static slices of the high in-degree columns, reduced with
-// global atomics; (b) the other unvisited columns, 64 consecutive ones per
-// wavefront, in-lists walked load-balanced, per-column sums in LDS.
+// global atomics; (b) the other unreached columns: a wavefront loads 64 words
+// of the reached-or-skipped bitmap at once, and for each word with a clear
+// bit walks the in-lists of its clear bits load-balanced (kUnroll edges per
+// lane in flight), summing parents' counts per column in LDS.
 template <bool SIGMA>
 __device__ __forceinline__ void pull_lb_level(const GraphView& g,
                                               const StateView& st, int dn,
@@ -208,96 +242,139 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
                                               int32_t nbigc, int64_t wave,
                                               int64_t nwaves, WaveLds* L,
                                               LevelAcc& a) {
-  for (int64_t i = wave; i < nbigc; i += nwaves) {
-    const uint64_t ent = bigc[i];
-    const int v = (int)(uint32_t)ent;
-    const int sv = st.S[v];
-    if (sv != -1 && sv != dn) continue;  // reached before this level
-    const int cb = g.CP[v] + (int)(ent >> 32) * kPullChunk;
-    const int ce = min(g.CP[v + 1], cb + kPullChunk);
-    bool f = false;
-    uint64_t sum = 0;
-    uint32_t ov = 0;
-    for (int k = cb + lane(); k < ce; k += 64) {
-      const int u = g.IC[k];
-      if (fbit(st.fcur, u)) {
-        f = true;
-        if (SIGMA) {
-          add_carry(sum, ov, st.sigma[u]);
-          if (ovf_any) ov |= st.ovf[u];
-        }
-      }
+  for (int64_t b = wave * 64; b < nbigc; b += nwaves * 64) {
+    const int64_t i = b + lane();
+    bool act = false;
+    uint64_t ent = 0;
+    if (i < nbigc) {
+      ent = bigc[i];
+      const int sv = st.S[(int)(uint32_t)ent];
+      act = (sv == -1 || sv == dn);  // not reached before this level
     }
-    if (__ballot(f)) {
-      if (SIGMA) wave_sum_carry(sum, ov);
-      if (lane() == 0) {
-        if (SIGMA) {
-          const unsigned long long old =
-              atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)sum);
-          if (old + sum < old) ov = 1;
-          if (ov) {
-            st.ovf[v] = 1;
-            a.ovf = 1;
+    uint64_t todo = __ballot(act);
+    while (todo) {
+      const int j = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const uint64_t e = __shfl(ent, j);
+      const int v = (int)(uint32_t)e;
+      const int cb = g.CP[v] + (int)(e >> 32) * kPullChunk;
+      const int ce = min(g.CP[v + 1], cb + kPullChunk);
+      bool f = false;
+      uint64_t sum = 0;
+      uint32_t ov = 0;
+      for (int k = cb + lane(); k < ce; k += 64) {
+        const int u = g.IC[k];
+        if (fbit(st.fcur, u)) {
+          f = true;
+          if (SIGMA) {
+            add_carry(sum, ov, st.sigma[u]);
+            if (ovf_any) ov |= st.ovf[u];
           }
         }
-        const unsigned long long bit = 1ull << (v & 63);
-        const unsigned long long old =
-            atomicOr((unsigned long long*)&st.fnext[v >> 6], bit);
-        if ((old & bit) == 0) {  // first slice of v to find a parent
-          st.S[v] = dn;
-          a.found(g, v);
+      }
+      if (__ballot(f)) {
+        if (SIGMA) wave_sum_carry(sum, ov);
+        if (lane() == 0) {
+          if (SIGMA) {
+            const unsigned long long old =
+                atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)sum);
+            if (old + sum < old) ov = 1;
+            if (ov) {
+              st.ovf[v] = 1;
+              a.ovf = 1;
+            }
+          }
+          const unsigned long long bit = 1ull << (v & 63);
+          const unsigned long long old =
+              atomicOr((unsigned long long*)&st.fnext[v >> 6], bit);
+          if ((old & bit) == 0) {  // first slice of v to find a parent
+            st.S[v] = dn;
+            a.found(g, v);
+          }
         }
       }
     }
   }
   const int64_t nwords = ((int64_t)g.n + 63) >> 6;
-  for (int64_t w = wave; w < nwords; w += nwaves) {
-    const int v = (int)(w * 64) + lane();
-    int beg = 0, deg = 0;
-    if (v < g.n && st.S[v] == -1) {
-      beg = g.CP[v];
-      const int dg = g.CP[v + 1] - beg;
-      deg = dg > kBigIn ? 0 : dg;
-    }
-    const uint32_t incl = wave_incl_scan((uint32_t)deg);
-    const int total = (int)__shfl(incl, 63);
-    if (total == 0) continue;
-    wave_lds_sync();
-    L->excl[lane()] = (int)incl - deg;
-    L->beg[lane()] = beg;
-    if (SIGMA) L->x[lane()] = 0;
-    wave_lds_sync();
-    uint64_t fmask = 0, omask = 0;
-    for (int e = lane(); e < total; e += 64) {
-      const int o = owner_of(L->excl, e);
-      const int k = L->beg[o] + (e - L->excl[o]);
-      const int u = g.IC[k];
-      if (fbit(st.fcur, u)) {
-        fmask |= 1ull << o;
+  for (int64_t sb = wave * 64; sb < nwords; sb += nwaves * 64) {
+    const int64_t wl = sb + lane();
+    // reached (any frontier so far, the current one included) or not a
+    // light column: nothing to do for that vertex
+    const uint64_t done_l = wl < nwords ? (st.vis[wl] | st.fcur[wl]) : ~0ull;
+    uint64_t todo = __ballot(done_l != ~0ull);
+    while (todo) {
+      const int j = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const int64_t w = sb + j;
+      const uint64_t done = __shfl(done_l, j);
+      const int v = (int)(w * 64) + lane();
+      int beg = 0, deg = 0;
+      if (!((done >> lane()) & 1ull)) {
+        beg = g.CP[v];
+        deg = g.CP[v + 1] - beg;
+      }
+      const uint32_t incl = wave_incl_scan((uint32_t)deg);
+      const int total = (int)__shfl(incl, 63);
+      wave_lds_sync();
+      L->excl[lane()] = (int)incl - deg;
+      L->beg[lane()] = beg;
+      if (SIGMA) L->x[lane()] = 0;
+      wave_lds_sync();
+      uint64_t fmask = 0, omask = 0;
+      for (int e0 = lane(); e0 < total; e0 += 64 * kUnroll) {
+        int u[kUnroll], o[kUnroll];
+#pragma unroll
+        for (int t = 0; t < kUnroll; ++t) {
+          const int e = e0 + 64 * t;
+          o[t] = -1;
+          if (e < total) {
+            o[t] = owner_of(L->excl, e);
+            u[t] = g.IC[L->beg[o[t]] + (e - L->excl[o[t]])];
+          }
+        }
+        bool par[kUnroll];
+#pragma unroll
+        for (int t = 0; t < kUnroll; ++t) par[t] = o[t] >= 0 && fbit(st.fcur, u[t]);
         if (SIGMA) {
-          const uint64_t xu = st.sigma[u];
-          const unsigned long long old =
-              atomicAdd((unsigned long long*)&L->x[o], (unsigned long long)xu);
-          if (old + xu < old || (ovf_any && st.ovf[u])) omask |= 1ull << o;
+          uint64_t xs[kUnroll];
+          uint32_t us[kUnroll];
+#pragma unroll
+          for (int t = 0; t < kUnroll; ++t) {
+            xs[t] = par[t] ? st.sigma[u[t]] : 0ull;
+            us[t] = (par[t] && ovf_any) ? (uint32_t)st.ovf[u[t]] : 0u;
+          }
+#pragma unroll
+          for (int t = 0; t < kUnroll; ++t) {
+            if (par[t]) {
+              fmask |= 1ull << o[t];
+              const unsigned long long old =
+                  atomicAdd((unsigned long long*)&L->x[o[t]], (unsigned long long)xs[t]);
+              if (old + xs[t] < old || us[t]) omask |= 1ull << o[t];
+            }
+          }
+        } else {
+#pragma unroll
+          for (int t = 0; t < kUnroll; ++t)
+            if (par[t]) fmask |= 1ull << o[t];
         }
       }
-    }
-    fmask = wave_or_u64(fmask);
-    if (fmask == 0) continue;
-    if (SIGMA) omask = wave_or_u64(omask);
-    wave_lds_sync();
-    if ((fmask >> lane()) & 1ull) {
-      st.S[v] = dn;
-      if (SIGMA) {
-        st.sigma[v] = L->x[lane()];
-        if ((omask >> lane()) & 1ull) {
-          st.ovf[v] = 1;
-          a.ovf = 1;
+      fmask = wave_or_u64(fmask);
+      if (fmask == 0) continue;
+      if (SIGMA) omask = wave_or_u64(omask);
+      wave_lds_sync();
+      if ((fmask >> lane()) & 1ull) {
+        st.S[v] = dn;
+        if (SIGMA) {
+          st.sigma[v] = L->x[lane()];
+          if ((omask >> lane()) & 1ull) {
+            st.ovf[v] = 1;
+            a.ovf = 1;
+          }
         }
+        a.found(g, v);
       }
-      a.found(g, v);
+      if (lane() == 0) atomicOr((unsigned long long*)&st.fnext[w], fmask);
     }
-    if (lane() == 0) atomicOr((unsigned long long*)&st.fnext[w], fmask);
   }
 }
 
@@ -382,6 +459,7 @@ __device__ __forceinline__ void init_state(const PersistParams& p, int64_t gtid,
     p.fb[0][w] = (w == (r >> 6)) ? (1ull << (r & 63)) : 0ull;
     p.fb[1][w] = 0ull;
     p.fb[2][w] = 0ull;
+    p.vis[w] = p.skip[w];
   }
   if (gtid < 3 * kStripes) {
     CtrStripe* c = &p.cset[gtid / kStripes].s[gtid % kStripes];
@@ -445,9 +523,15 @@ __device__ __forceinline__ void run_level(const PersistParams& p, int d,
   const GraphView& g = p.g;
   const int64_t nwords = ((int64_t)g.n + 63) >> 6;
   const int64_t wave = gtid >> 6, nwaves = gstride >> 6;
-  // buffers of depth d+2 were last used by depth d-1: clear them now
+  // buffers of depth d+2 were last used by depth d-1: clear them now; and
+  // fold the current frontier into the reached bitmap
   uint64_t* fclr = p.fb[(d + 2) % 3];
-  for (int64_t w = gtid; w < nwords; w += gstride) fclr[w] = 0ull;
+  const uint64_t* fcur = p.fb[d % 3];
+  for (int64_t w = gtid; w < nwords; w += gstride) {
+    fclr[w] = 0ull;
+    const uint64_t f = fcur[w];
+    if (f) p.vis[w] |= f;
+  }
   if (gtid < kStripes) {
     CtrStripe* z = &p.cset[(d + 2) % 3].s[gtid];
     z->qm = 0;
@@ -455,7 +539,7 @@ __device__ __forceinline__ void run_level(const PersistParams& p, int d,
     z->ovf = 0;
   }
   StateView st{p.S, p.sigma, p.ovf, p.fb[d % 3], p.fb[(d + 1) % 3],
-               nullptr, nullptr, nullptr, &p.cset[(d + 1) % 3]};
+               nullptr, nullptr, nullptr, &p.cset[(d + 1) % 3], p.vis};
   if (pushing) {
     push_lb_level(g, st, d + 1, ovf_any, p.bigo, p.nbigo, wave, nwaves, L, acc);
   } else if (p.sigma_on) {
@@ -494,7 +578,10 @@ __global__ __launch_bounds__(256) void k_chain_init(PersistParams p) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
   init_state(p, gtid, gstride);
-  if (gtid == 0) p.loop[0] = first_state(p);
+  if (gtid == 0) {
+    p.loop[0] = first_state(p);
+    p.stats->levels = -1;  // set by the level that finds the frontier empty
+  }
 }
 
 // Chained loop: level d.  Reads slot d & 1 and the counters of depth d (both
