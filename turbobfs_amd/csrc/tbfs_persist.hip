@@ -147,9 +147,9 @@ constexpr int kUnroll = 4;  // independent edges in flight per lane
 
 // Push level, driven by the frontier bitmap: (a) static slices of the long
 // out-lists whose vertex is in the frontier, (b) every other frontier vertex:
-// a wavefront loads 64 frontier words at once, and for each nonzero one
-// walks the out-lists of its set bits load-balanced, kUnroll edges per lane
-// in flight.
+// frontier words are dealt to wavefronts round-robin; a wavefront loads 64 of
+// its words at once, and for each nonzero one walks the out-lists of its set
+// bits load-balanced, kUnroll edges per lane in flight.
 __device__ __forceinline__ void push_lb_level(const GraphView& g,
                                               const StateView& st, int dn,
                                               uint32_t ovf_any,
@@ -157,11 +157,13 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
                                               int32_t nbigo, int64_t wave,
                                               int64_t nwaves, WaveLds* L,
                                               LevelAcc& a) {
-  for (int64_t b = wave * 64; b < nbigo; b += nwaves * 64) {
-    const int64_t i = b + lane();
+  // this wave's share: entries wave, wave + nwaves, ...; 64 examined at once
+  const int64_t nbo = nbigo > wave ? (nbigo - wave + nwaves - 1) / nwaves : 0;
+  for (int64_t b = 0; b < nbo; b += 64) {
+    const int64_t i = wave + (b + lane()) * nwaves;
     bool act = false;
     uint64_t ent = 0;
-    if (i < nbigo) {
+    if (b + lane() < nbo) {
       ent = bigo[i];
       act = fbit(st.fcur, (int)(uint32_t)ent);
     }
@@ -179,15 +181,17 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
     }
   }
   const int64_t nwords = ((int64_t)g.n + 63) >> 6;
-  for (int64_t sb = wave * 64; sb < nwords; sb += nwaves * 64) {
-    const int64_t wl = sb + lane();
-    const uint64_t bits_l = wl < nwords ? st.fcur[wl] : 0ull;
+  // this wave's words: wave, wave + nwaves, ...; their bits loaded 64 at once
+  const int64_t nmine = nwords > wave ? (nwords - wave + nwaves - 1) / nwaves : 0;
+  for (int64_t b = 0; b < nmine; b += 64) {
+    const int64_t wl = wave + (b + lane()) * nwaves;
+    const uint64_t bits_l = (b + lane() < nmine) ? st.fcur[wl] : 0ull;
     uint64_t todo = __ballot(bits_l != 0);
     while (todo) {
       const int j = __ffsll((unsigned long long)todo) - 1;
       todo &= todo - 1;
       const uint64_t bits = __shfl(bits_l, j);
-      const int u = (int)((sb + j) * 64) + lane();
+      const int u = (int)((wave + (b + j) * nwaves) * 64) + lane();
       int beg = 0, deg = 0;
       uint64_t x = 0;
       uint32_t ou = 0;
@@ -230,10 +234,11 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
 }
 
 // Pull level: (a) static slices of the high in-degree columns, reduced with
-// global atomics; (b) the other unreached columns: a wavefront loads 64 words
-// of the reached-or-skipped bitmap at once, and for each word with a clear
-// bit walks the in-lists of its clear bits load-balanced (kUnroll edges per
-// lane in flight), summing parents' counts per column in LDS.
+// global atomics; (b) the other unreached columns: words are dealt to
+// wavefronts round-robin; a wavefront loads 64 of its words of the
+// reached-or-skipped bitmap at once, and for each word with a clear bit walks
+// the in-lists of its clear bits load-balanced (kUnroll edges per lane in
+// flight), summing parents' counts per column in LDS.
 template <bool SIGMA>
 __device__ __forceinline__ void pull_lb_level(const GraphView& g,
                                               const StateView& st, int dn,
@@ -242,11 +247,12 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
                                               int32_t nbigc, int64_t wave,
                                               int64_t nwaves, WaveLds* L,
                                               LevelAcc& a) {
-  for (int64_t b = wave * 64; b < nbigc; b += nwaves * 64) {
-    const int64_t i = b + lane();
+  const int64_t nbc = nbigc > wave ? (nbigc - wave + nwaves - 1) / nwaves : 0;
+  for (int64_t b = 0; b < nbc; b += 64) {
+    const int64_t i = wave + (b + lane()) * nwaves;
     bool act = false;
     uint64_t ent = 0;
-    if (i < nbigc) {
+    if (b + lane() < nbc) {
       ent = bigc[i];
       const int sv = st.S[(int)(uint32_t)ent];
       act = (sv == -1 || sv == dn);  // not reached before this level
@@ -296,16 +302,17 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
     }
   }
   const int64_t nwords = ((int64_t)g.n + 63) >> 6;
-  for (int64_t sb = wave * 64; sb < nwords; sb += nwaves * 64) {
-    const int64_t wl = sb + lane();
+  const int64_t nmine = nwords > wave ? (nwords - wave + nwaves - 1) / nwaves : 0;
+  for (int64_t b = 0; b < nmine; b += 64) {
+    const int64_t wl = wave + (b + lane()) * nwaves;
     // reached (any frontier so far, the current one included) or not a
     // light column: nothing to do for that vertex
-    const uint64_t done_l = wl < nwords ? (st.vis[wl] | st.fcur[wl]) : ~0ull;
+    const uint64_t done_l = (b + lane() < nmine) ? (st.vis[wl] | st.fcur[wl]) : ~0ull;
     uint64_t todo = __ballot(done_l != ~0ull);
     while (todo) {
       const int j = __ffsll((unsigned long long)todo) - 1;
       todo &= todo - 1;
-      const int64_t w = sb + j;
+      const int64_t w = wave + (b + j) * nwaves;
       const uint64_t done = __shfl(done_l, j);
       const int v = (int)(w * 64) + lane();
       int beg = 0, deg = 0;
