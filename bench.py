@@ -93,6 +93,7 @@ def main():
                     choices=["auto", "auto_chain", "auto_host", "auto_sc", "auto_wa", "pull_sc",
                              "pull_wa", "push", "cooc"])
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--fb-lds", type=int, default=0, help="-1: no LDS-staged frontier bitmap")
     ap.add_argument("--grid", type=int, default=0, help="workgroups of the persistent kernel (0: all co-resident)")
     ap.add_argument("--alpha", type=float, default=0.0)
     ap.add_argument("--beta", type=float, default=0.0)
@@ -133,6 +134,8 @@ def main():
         kw["grid"] = args.grid
     if args.block:
         kw["block"] = args.block
+    if args.fb_lds:
+        kw["fb_lds"] = args.fb_lds
     if args.alpha:
         kw["alpha"] = args.alpha
     if args.beta:

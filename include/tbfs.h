@@ -78,7 +78,9 @@ typedef struct tbfs_opts {
                      the persistent loop takes 256 or 1024 */
   int32_t grid;   /* workgroups (0: default) */
   int32_t pull;   /* TBFS_PULL_*: AUTO's bottom-up kernel */
-  int32_t reserved[7];
+  int32_t fb_lds; /* chained loop: 0 stages the frontier bitmap in LDS for
+                     pull levels when it fits (n <= ~1.1M), -1 never */
+  int32_t reserved[6];
 } tbfs_opts;
 
 typedef struct tbfs_stats {

@@ -106,6 +106,8 @@ MODES = [
     ("pers_auto_g64", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=PERS, grid=64)),
     ("pers_auto_b1024", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=PERS, block=1024)),
     ("chain_auto", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED)),
+    ("chain_auto_no_lds", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED,
+                               fb_lds=-1)),
     ("chain_push", dict(mode=L.MODE_PUSH, pull=L.PULL_LB, loop=L.LOOP_CHAINED)),
     ("chain_pull_depth_only", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED,
                                    alpha=1e9, sigma=0)),

@@ -55,7 +55,8 @@ class Opts(C.Structure):
         ("block", C.c_int32),
         ("grid", C.c_int32),
         ("pull", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("fb_lds", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 

@@ -56,6 +56,8 @@ struct tbfs_ctx {
   tbfs::LoopState* loop = nullptr;  // [2] chained-loop state slots
   tbfs::LoopState* loop_h = nullptr;
   int chain_k = 8;                  // levels enqueued per host check
+  size_t fbl_limit = 0;             // bytes of LDS for a staged frontier bitmap
+  int cus = 0;
   int32_t last_n = 0;
 };
 

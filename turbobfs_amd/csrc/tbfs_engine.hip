@@ -424,6 +424,14 @@ int ctx_create(int32_t n_cap, tbfs_ctx** out) {
     (void)hipGetLastError();
     c->persist_grid_max1024 = 0;
   }
+  if (kern::chain_fbl_limit(&c->fbl_limit) != hipSuccess) {
+    (void)hipGetLastError();
+    c->fbl_limit = 0;
+  }
+  if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) {
+    (void)hipGetLastError();
+    c->cus = 256;
+  }
   *out = c;
   return TBFS_OK;
 fail:
@@ -543,7 +551,15 @@ static int enqueue_persistent(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   return TBFS_OK;
 }
 
-static int chain_grid(const tbfs_graph* g, const tbfs_opts& o) {
+// LDS bytes of the staged frontier bitmap for the chained levels (0: none).
+static size_t chain_fbl(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o) {
+  const size_t bytes = (((size_t)g->n + 63) / 64) * sizeof(uint64_t);
+  if (o.fb_lds < 0 || c->fbl_limit == 0 || bytes > c->fbl_limit) return 0;
+  return (bytes + 15) & ~(size_t)15;
+}
+
+static int chain_grid(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o) {
+  if (chain_fbl(c, g, o)) return o.grid > 0 ? std::min(o.grid, 65536) : c->cus;
   const int64_t nwords = ((int64_t)g->n + 63) / 64;
   // about four frontier words per wavefront: fewer workgroups to dispatch
   // per level and fewer counter atomics, still >> 256 CUs' worth of waves
@@ -557,10 +573,11 @@ static int chain_grid(const tbfs_graph* g, const tbfs_opts& o) {
 static int enqueue_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
                            const tbfs_opts& o, RunStats* stats_slot, int levels) {
   const PersistParams p = make_params(c, g, root, o, stats_slot);
-  const int grid = chain_grid(g, o);
+  const int grid = chain_grid(c, g, o);
+  const size_t fbl = chain_fbl(c, g, o);
   hipStream_t s = c->stream;
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
-  for (int d = 0; d < levels; ++d) HIPCHK(kern::chain_level(s, grid, p, d));
+  for (int d = 0; d < levels; ++d) HIPCHK(kern::chain_level(s, grid, p, d, fbl));
   return TBFS_OK;
 }
 
@@ -569,12 +586,13 @@ static int enqueue_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
 static int run_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
                        const tbfs_opts& o, RunStats* stats_slot) {
   const PersistParams p = make_params(c, g, root, o, stats_slot);
-  const int grid = chain_grid(g, o);
+  const int grid = chain_grid(c, g, o);
+  const size_t fbl = chain_fbl(c, g, o);
   hipStream_t s = c->stream;
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
   int32_t d = 0;
   for (;;) {
-    for (int i = 0; i < c->chain_k; ++i) HIPCHK(kern::chain_level(s, grid, p, d + i));
+    for (int i = 0; i < c->chain_k; ++i) HIPCHK(kern::chain_level(s, grid, p, d + i, fbl));
     d += c->chain_k;
     HIPCHK(hipMemcpyAsync(c->loop_h, c->loop + (d & 1), sizeof(LoopState),
                           hipMemcpyDeviceToHost, s));
@@ -797,7 +815,7 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
       rc = run_chained(c, g, roots[count - 1], o, c->stats + count - 1);
       if (rc) return rc;
     }
-    c->chain_k = std::max(4, std::min(4096, maxlev + 2));
+    c->chain_k = std::max(4, std::min(4096, maxlev + 1));
   }
   c->last_n = g->n;
   if (st) {

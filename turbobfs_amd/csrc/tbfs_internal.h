@@ -187,7 +187,12 @@ hipError_t persist(hipStream_t s, int grid, int block, const PersistParams& p);
 // Chained loop: one init kernel, then one kernel per level (level index d),
 // each deciding its own direction from the device-side counters.
 hipError_t chain_init(hipStream_t s, int grid, const PersistParams& p);
-hipError_t chain_level(hipStream_t s, int grid, const PersistParams& p, int32_t d);
+// fbl_bytes > 0: 1024-thread workgroups staging the frontier bitmap
+// (fbl_bytes of dynamic LDS) for pull levels; 0: kLbBlock-thread workgroups.
+hipError_t chain_level(hipStream_t s, int grid, const PersistParams& p, int32_t d,
+                       size_t fbl_bytes);
+// Largest bitmap (bytes) the LDS-staging variant can take; 0 if unavailable.
+hipError_t chain_fbl_limit(size_t* bytes);
 }  // namespace kern
 
 constexpr int kBarWords = 512;  // grid barrier: top count, generation, 8 group counts

@@ -246,7 +246,8 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
                                               const uint64_t* bigc,
                                               int32_t nbigc, int64_t wave,
                                               int64_t nwaves, WaveLds* L,
-                                              LevelAcc& a) {
+                                              LevelAcc& a,
+                                              const uint64_t* fpar) {
   const int64_t nbc = nbigc > wave ? (nbigc - wave + nwaves - 1) / nwaves : 0;
   for (int64_t b = 0; b < nbc; b += 64) {
     const int64_t i = wave + (b + lane()) * nwaves;
@@ -270,7 +271,7 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
       uint32_t ov = 0;
       for (int k = cb + lane(); k < ce; k += 64) {
         const int u = g.IC[k];
-        if (fbit(st.fcur, u)) {
+        if (fbit(fpar, u)) {
           f = true;
           if (SIGMA) {
             add_carry(sum, ov, st.sigma[u]);
@@ -341,7 +342,7 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
         }
         bool par[kUnroll];
 #pragma unroll
-        for (int t = 0; t < kUnroll; ++t) par[t] = o[t] >= 0 && fbit(st.fcur, u[t]);
+        for (int t = 0; t < kUnroll; ++t) par[t] = o[t] >= 0 && fbit(fpar, u[t]);
         if (SIGMA) {
           uint64_t xs[kUnroll];
           uint32_t us[kUnroll];
@@ -412,7 +413,7 @@ __global__ __launch_bounds__(kLbBlock) void k_pull_lb(GraphView g, StateView st,
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   LevelAcc a;
   pull_lb_level<SIGMA>(g, st, d + 1, (uint32_t)ovf_any, bigc, nbigc, wave,
-                       nwaves, &lds[threadIdx.x >> 6], a);
+                       nwaves, &lds[threadIdx.x >> 6], a, st.fcur);
   flush_level(st.cset, a);
 }
 
@@ -523,10 +524,14 @@ __device__ __forceinline__ void write_stats(const PersistParams& p,
 }
 
 // Housekeeping for depth d+2 plus the level's work (frontier depth d).
+// fb_lds: when not null, pull levels test parents against a copy of the
+// frontier bitmap staged in this workgroup's LDS (random bitmap lookups stay
+// out of the L2); it must hold ceil(n/64) words.
 __device__ __forceinline__ void run_level(const PersistParams& p, int d,
                                           bool pushing, uint32_t ovf_any,
                                           int64_t gtid, int64_t gstride,
-                                          WaveLds* L, LevelAcc& acc) {
+                                          WaveLds* L, LevelAcc& acc,
+                                          uint64_t* fb_lds) {
   const GraphView& g = p.g;
   const int64_t nwords = ((int64_t)g.n + 63) >> 6;
   const int64_t wave = gtid >> 6, nwaves = gstride >> 6;
@@ -549,10 +554,21 @@ __device__ __forceinline__ void run_level(const PersistParams& p, int d,
                nullptr, nullptr, nullptr, &p.cset[(d + 1) % 3], p.vis};
   if (pushing) {
     push_lb_level(g, st, d + 1, ovf_any, p.bigo, p.nbigo, wave, nwaves, L, acc);
-  } else if (p.sigma_on) {
-    pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc);
   } else {
-    pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc);
+    const uint64_t* fpar = st.fcur;
+    if (fb_lds) {
+      const uint4* src = reinterpret_cast<const uint4*>(st.fcur);
+      uint4* dst = reinterpret_cast<uint4*>(fb_lds);
+      const int64_t n16 = nwords >> 1;
+      for (int64_t i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+      if ((nwords & 1) && threadIdx.x == 0) fb_lds[nwords - 1] = st.fcur[nwords - 1];
+      __syncthreads();
+      fpar = fb_lds;
+    }
+    if (p.sigma_on)
+      pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc, fpar);
+    else
+      pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc, fpar);
   }
   flush_level(st.cset, acc);
 }
@@ -573,7 +589,7 @@ __global__ __launch_bounds__(BLOCK) void k_persist(PersistParams p) {
   for (;;) {
     const LevelCtr c = read_set(&p.cset[s.d % 3], true);
     if (!step_state(p, &c, s)) break;
-    run_level(p, s.d, s.pushing != 0, s.ovf_any, gtid, gstride, L, acc);
+    run_level(p, s.d, s.pushing != 0, s.ovf_any, gtid, gstride, L, acc, nullptr);
     ++s.d;
     grid_barrier(p.bar, ++bk);
   }
@@ -592,9 +608,12 @@ __global__ __launch_bounds__(256) void k_chain_init(PersistParams p) {
 }
 
 // Chained loop: level d.  Reads slot d & 1 and the counters of depth d (both
-// final: written by earlier kernels), writes slot (d + 1) & 1.
-__global__ __launch_bounds__(kLbBlock) void k_chain_level(PersistParams p, int32_t d) {
-  __shared__ WaveLds lds[kLbBlock / 64];
+// final: written by earlier kernels), writes slot (d + 1) & 1.  FBL: pull
+// levels stage the frontier bitmap in dynamic LDS (one workgroup per CU).
+template <int BLOCK, bool FBL>
+__global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t d) {
+  __shared__ WaveLds lds[BLOCK / 64];
+  extern __shared__ uint64_t fb_dyn[];
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
   LoopState s = p.loop[d & 1];
@@ -614,7 +633,8 @@ __global__ __launch_bounds__(kLbBlock) void k_chain_level(PersistParams p, int32
   nx.d = d + 1;
   if (gtid == 0) p.loop[(d + 1) & 1] = nx;
   LevelAcc acc;
-  run_level(p, d, s.pushing != 0, s.ovf_any, gtid, gstride, &lds[threadIdx.x >> 6], acc);
+  run_level(p, d, s.pushing != 0, s.ovf_any, gtid, gstride, &lds[threadIdx.x >> 6], acc,
+            FBL ? fb_dyn : nullptr);
 }
 
 }  // namespace
@@ -670,9 +690,28 @@ hipError_t chain_init(hipStream_t s, int grid, const PersistParams& p) {
   return hipGetLastError();
 }
 
-hipError_t chain_level(hipStream_t s, int grid, const PersistParams& p, int32_t d) {
-  hipLaunchKernelGGL(k_chain_level, dim3(grid), dim3(kLbBlock), 0, s, p, d);
+hipError_t chain_level(hipStream_t s, int grid, const PersistParams& p, int32_t d,
+                       size_t fbl_bytes) {
+  if (fbl_bytes)
+    hipLaunchKernelGGL((k_chain_level<1024, true>), dim3(grid), dim3(1024), fbl_bytes, s, p, d);
+  else
+    hipLaunchKernelGGL((k_chain_level<kLbBlock, false>), dim3(grid), dim3(kLbBlock), 0, s, p, d);
   return hipGetLastError();
+}
+
+hipError_t chain_fbl_limit(size_t* bytes) {
+  // LDS left for the staged bitmap next to the kernel's static LDS
+  hipFuncAttributes fa;
+  hipError_t e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_chain_level<1024, true>));
+  if (e != hipSuccess) return e;
+  const size_t lds_cu = 160 * 1024;
+  size_t avail = lds_cu > fa.sharedSizeBytes ? lds_cu - fa.sharedSizeBytes : 0;
+  avail &= ~(size_t)255;
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_chain_level<1024, true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail);
+  if (e != hipSuccess) return e;
+  *bytes = avail;
+  return hipSuccess;
 }
 
 }  // namespace kern
