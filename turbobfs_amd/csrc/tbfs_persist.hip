@@ -32,6 +32,8 @@ namespace {
 
 using namespace dev;
 
+constexpr int kUnroll = 4;  // independent edges in flight per lane
+
 struct WaveLds {
   int32_t excl[64];  // exclusive prefix of per-lane work
   int32_t beg[64];   // first edge index of each lane's slice
@@ -118,32 +120,31 @@ __device__ __forceinline__ LevelCtr read_set(const CtrSet* set, bool agent) {
 }
 
 // One edge u -> v of a push level, u at depth d with count x (overflow ou).
+// "Reached at depth <= d" is read from the reached bitmap or the current
+// frontier (both small and cached); otherwise v is reached at depth d+1, so
+// the count is added at once and, independently, v's bit is claimed in the
+// next frontier: the lane that sets the bit records the depth.  The two
+// device-scope atomics do not wait on each other.
 __device__ __forceinline__ void push_edge(const GraphView& g, const StateView& st,
                                           int dn, int v, uint64_t x, bool ou,
                                           LevelAcc& a) {
-  int s = st.S[v];
-  bool nw = false;
-  if (s == -1) {
-    const int old = atomicCAS(&st.S[v], -1, dn);
-    nw = (old == -1);
-    s = dn;  // old is -1 or dn: only this level writes S now
+  const uint64_t bit = 1ull << (v & 63);
+  if ((st.vis[v >> 6] | st.fcur[v >> 6]) & bit) return;
+  const unsigned long long old =
+      atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)x);
+  const unsigned long long oldb =
+      atomicOr((unsigned long long*)&st.fnext[v >> 6], (unsigned long long)bit);
+  if (old + x < old || ou) {
+    st.ovf[v] = 1;
+    a.ovf = 1;
   }
-  if (s == dn) {
-    const unsigned long long old =
-        atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)x);
-    if (old + x < old || ou) {
-      st.ovf[v] = 1;
-      a.ovf = 1;
-    }
-  }
-  if (nw) {
-    atomicOr((unsigned long long*)&st.fnext[v >> 6], 1ull << (v & 63));
+  if ((oldb & bit) == 0) {
+    st.S[v] = dn;
     a.found(g, v);
   }
 }
 
 // ----------------------------------------------------------------------------
-constexpr int kUnroll = 4;  // independent edges in flight per lane
 
 // Push level, driven by the frontier bitmap: (a) static slices of the long
 // out-lists whose vertex is in the frontier, (b) every other frontier vertex:
@@ -177,7 +178,14 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
       const int ce = min(g.RP[u + 1], cb + kPullChunk);
       const uint64_t x = st.sigma[u];
       const bool ou = ovf_any && st.ovf[u];
-      for (int k = cb + lane(); k < ce; k += 64) push_edge(g, st, dn, g.JR[k], x, ou, a);
+      for (int k0 = cb + lane(); k0 < ce; k0 += 64 * kUnroll) {
+        int v[kUnroll];
+#pragma unroll
+        for (int t = 0; t < kUnroll; ++t) v[t] = (k0 + 64 * t < ce) ? g.JR[k0 + 64 * t] : -1;
+#pragma unroll
+        for (int t = 0; t < kUnroll; ++t)
+          if (v[t] >= 0) push_edge(g, st, dn, v[t], x, ou, a);
+      }
     }
   }
   const int64_t nwords = ((int64_t)g.n + 63) >> 6;
