@@ -639,7 +639,7 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
   }
   HIPCHK(hipEventRecord(c->ev0, s));
   HIPCHK(kern::init(s, fill_grid, n, root, rp2[1] - rp2[0], c->S, c->sigma,
-                    c->ovf, c->fb[0], c->fb[1], c->fb[2], c->q[0], c->vis, g->skip));
+                    c->ovf, c->fb[0], c->fb[1], c->fb[2], c->q[0], c->vis));
 
   int64_t qsize = 1;
   int64_t qe = (rp2[1] - rp2[0] + kPushChunk - 1) / kPushChunk;
@@ -663,7 +663,7 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
     }
     const int nxt = (cur + 1) % 3;
     StateView sv{c->S, c->sigma, c->ovf, c->fb[cur], c->fb[nxt],
-                 c->q[d & 1], c->q[(d + 1) & 1], c->ctr, c->cset, c->vis};
+                 c->q[d & 1], c->q[(d + 1) & 1], c->ctr, c->cset, c->vis, g->skip};
     HIPCHK(kern::level_prologue(s, grid_for(nwords, 256, 4096), c->fb[nxt],
                                 nwords, c->ctr, c->cset, c->fb[cur], c->vis));
     bool striped = false;  // load-balanced kernels count into the stripes

@@ -16,8 +16,9 @@
 //                                 > kBigIn (pulled through bigc slices)
 //   fb[3] uint64[ceil(n/64)]      frontier bitmaps (bit v&63 of word v>>6);
 //                                 current / next / being cleared, rotating
-//   vis uint64[ceil(n/64)]        skip | every frontier so far: a clear bit is
-//                                 an unreached column of the light pull path
+//   vis uint64[ceil(n/64)]        every frontier so far (reached vertices);
+//                                 vis | skip | current frontier has a clear
+//                                 bit exactly for the light pull path's work
 //   q[2] uint64[q_cap]            frontier work queues of the per-level
 //                                 reference-mapping kernels (current / next):
 //                                 one entry per kPushChunk-entry slice of a
@@ -96,7 +97,8 @@ struct StateView {
   uint64_t* qnext;       // queue of depth d+1
   LevelCtr* ctr;         // counters of depth d+1 (must be zero on entry)
   CtrSet* cset;          // the same, striped (load-balanced kernels)
-  uint64_t* vis;         // reached-or-skipped bitmap (load-balanced pull)
+  uint64_t* vis;         // reached bitmap (every frontier so far)
+  const uint64_t* skip;  // static skip bitmap of the graph
 };
 
 // Loop state carried from one level to the next by the device-resident
@@ -122,7 +124,7 @@ struct PersistParams {
   uint64_t* sigma;
   uint8_t* ovf;
   uint64_t* fb[3];
-  uint64_t* vis;          // reached-or-skipped bitmap
+  uint64_t* vis;          // reached bitmap (every frontier so far)
   const uint64_t* skip;   // static: in-degree 0 or split column (graph)
   CtrSet* cset;     // [3] striped level counters
   unsigned* bar;    // kBarWords barrier words (zero at launch)
@@ -146,7 +148,7 @@ namespace kern {
 hipError_t init(hipStream_t s, int grid, int32_t n, int32_t root,
                 int32_t root_out_deg, int32_t* S, uint64_t* sigma,
                 uint8_t* ovf, uint64_t* fb0, uint64_t* fb1, uint64_t* fb2,
-                uint64_t* q0, uint64_t* vis, const uint64_t* skip);
+                uint64_t* q0, uint64_t* vis);
 hipError_t level_prologue(hipStream_t s, int grid, uint64_t* fnext,
                           int32_t nwords, LevelCtr* ctr, CtrSet* cset,
                           const uint64_t* fcur, uint64_t* vis);

@@ -32,7 +32,7 @@ using namespace dev;
 __global__ void k_init(int32_t n, int32_t root, int32_t root_out_deg,
                        int32_t* S, uint64_t* sigma, uint8_t* ovf,
                        uint64_t* fb0, uint64_t* fb1, uint64_t* fb2,
-                       uint64_t* q0, uint64_t* vis, const uint64_t* skip) {
+                       uint64_t* q0, uint64_t* vis) {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = tid; i < n; i += stride) {
@@ -45,7 +45,7 @@ __global__ void k_init(int32_t n, int32_t root, int32_t root_out_deg,
     fb0[w] = (w == (root >> 6)) ? (1ull << (root & 63)) : 0ull;
     fb1[w] = 0ull;
     fb2[w] = 0ull;
-    vis[w] = skip[w];
+    vis[w] = 0ull;
   }
   const int64_t nch = ((int64_t)root_out_deg + kPushChunk - 1) / kPushChunk;
   for (int64_t c = tid; c < nch; c += stride)
@@ -327,9 +327,9 @@ namespace kern {
 hipError_t init(hipStream_t s, int grid, int32_t n, int32_t root,
                 int32_t root_out_deg, int32_t* S, uint64_t* sigma,
                 uint8_t* ovf, uint64_t* fb0, uint64_t* fb1, uint64_t* fb2,
-                uint64_t* q0, uint64_t* vis, const uint64_t* skip) {
+                uint64_t* q0, uint64_t* vis) {
   hipLaunchKernelGGL(k_init, dim3(grid), dim3(256), 0, s, n, root,
-                     root_out_deg, S, sigma, ovf, fb0, fb1, fb2, q0, vis, skip);
+                     root_out_deg, S, sigma, ovf, fb0, fb1, fb2, q0, vis);
   return hipGetLastError();
 }
 

@@ -316,7 +316,8 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
     const int64_t wl = wave + (b + lane()) * nwaves;
     // reached (any frontier so far, the current one included) or not a
     // light column: nothing to do for that vertex
-    const uint64_t done_l = (b + lane() < nmine) ? (st.vis[wl] | st.fcur[wl]) : ~0ull;
+    const uint64_t done_l =
+        (b + lane() < nmine) ? (st.vis[wl] | st.skip[wl] | st.fcur[wl]) : ~0ull;
     uint64_t todo = __ballot(done_l != ~0ull);
     while (todo) {
       const int j = __ffsll((unsigned long long)todo) - 1;
@@ -475,7 +476,7 @@ __device__ __forceinline__ void init_state(const PersistParams& p, int64_t gtid,
     p.fb[0][w] = (w == (r >> 6)) ? (1ull << (r & 63)) : 0ull;
     p.fb[1][w] = 0ull;
     p.fb[2][w] = 0ull;
-    p.vis[w] = p.skip[w];
+    p.vis[w] = 0ull;
   }
   if (gtid < 3 * kStripes) {
     CtrStripe* c = &p.cset[gtid / kStripes].s[gtid % kStripes];
@@ -559,7 +560,7 @@ __device__ __forceinline__ void run_level(const PersistParams& p, int d,
     z->ovf = 0;
   }
   StateView st{p.S, p.sigma, p.ovf, p.fb[d % 3], p.fb[(d + 1) % 3],
-               nullptr, nullptr, nullptr, &p.cset[(d + 1) % 3], p.vis};
+               nullptr, nullptr, nullptr, &p.cset[(d + 1) % 3], p.vis, p.skip};
   if (pushing) {
     push_lb_level(g, st, d + 1, ovf_any, p.bigo, p.nbigo, wave, nwaves, L, acc);
   } else {
