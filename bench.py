@@ -29,6 +29,20 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per BFS measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
+# this bench (scaled by the counters' reading of the calibration copy)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bfs_rmat20_traffic.json")
+
+
+def measured_traffic(scale: int, mode: str):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("scale") != scale or t.get("mode") != mode:
+        return None
+    return t.get("hbm_bytes_per_bfs")
 METRIC = "GTEPS on 1 MI355X + batched sources/s & p99 latency vs per-source loop, exact"
 
 
@@ -89,8 +103,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--sources", type=int, default=64)
-    ap.add_argument("--mode", default="auto",
-                    choices=["auto", "auto_chain", "auto_host", "auto_sc", "auto_wa", "pull_sc",
+    ap.add_argument("--mode", default="auto_chain",
+                    choices=["auto_persistent", "auto_chain", "auto_host", "auto_sc", "auto_wa", "pull_sc",
                              "pull_wa", "push", "cooc"])
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--fb-lds", type=int, default=0, help="-1: no LDS-staged frontier bitmap")
@@ -119,7 +133,7 @@ def main():
     sources = graphs.sample_sources(graphs.nonzero_degree_vertices(g), args.sources, seed=2)
     H, P = _lib.LOOP_HOST, _lib.LOOP_PERSISTENT
     modes = {
-        "auto": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, loop=P),
+        "auto_persistent": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, loop=P),
         "auto_host": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, loop=H),
         "auto_chain": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, loop=_lib.LOOP_CHAINED),
         "auto_sc": dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_SC, loop=H),
@@ -193,9 +207,13 @@ def main():
         "copy_kernel_gbs": (2 * calib_bytes / (copy_ms * 1e-3) / 1e9) if copy_ms else None,
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_persist (one launch = one whole BFS)" if args.mode in ("auto", "push")
-                      else "whole BFS (all level kernels of one source)",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(args.scale, args.mode),
+            "kernel": {"auto_chain": "k_chain_level (the level launches of one BFS; "
+                                     "time = device time of the BFS incl. its init kernel)",
+                       "auto_persistent": "k_persist (one launch = one whole BFS)",
+                       "push": "k_persist (one launch = one whole BFS)"}.get(
+                           args.mode, "whole BFS (all level kernels of one source)"),
         },
     }
     if not args.no_cpu:

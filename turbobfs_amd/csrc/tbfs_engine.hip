@@ -815,7 +815,9 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
       rc = run_chained(c, g, roots[count - 1], o, c->stats + count - 1);
       if (rc) return rc;
     }
-    c->chain_k = std::max(4, std::min(4096, maxlev + 1));
+    // one spare level: a source one level deeper than any seen so far still
+    // finishes without a re-run (a spare launch costs ~4 us, a re-run a BFS)
+    c->chain_k = std::max(c->chain_k, std::min(4096, maxlev + 2));
   }
   c->last_n = g->n;
   if (st) {
