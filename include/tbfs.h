@@ -56,7 +56,9 @@ extern "C" {
 /* level loop (tbfs_opts.loop) */
 #define TBFS_LOOP_HOST 0       /* host launches each level, reads its counters */
 #define TBFS_LOOP_PERSISTENT 1 /* one cooperative launch per BFS, levels
-                                  separated by a device-wide barrier */
+                                  separated by a device-wide barrier; when
+                                  the runtime declines the cooperative
+                                  launch the BFS runs the chained loop */
 #define TBFS_LOOP_CHAINED 2    /* one kernel per level, enqueued back to back;
                                   each decides its direction from the device
                                   counters, the host checks for the end once
@@ -94,7 +96,7 @@ typedef struct tbfs_stats {
   double ms;           /* device time of the BFS, first to last kernel */
 } tbfs_stats;
 
-/* Fill *o with the defaults: AUTO, load-balanced kernels, persistent loop,
+/* Fill *o with the defaults: AUTO, load-balanced kernels, chained loop,
  * sigma on, alpha 14, beta 24. */
 void tbfs_opts_default(tbfs_opts *o);
 
@@ -134,7 +136,7 @@ int tbfs_bfs(tbfs_ctx *c, const tbfs_graph *g, int32_t root,
              const tbfs_opts *o, tbfs_stats *st);
 
 /* Run `count` BFS back to back (roots[i]), with no host synchronisation in
- * between when the persistent loop applies.  Results of the last BFS stay in
+ * between when the persistent or chained loop applies.  Results of the last BFS stay in
  * the context; st (may be NULL) receives count entries, each with its own
  * device time. */
 int tbfs_bfs_many(tbfs_ctx *c, const tbfs_graph *g, const int32_t *roots,

@@ -170,7 +170,7 @@ int tdbu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int* sigma_h,
   tbfs_opts_default(&o);
   o.mode = TBFS_MODE_AUTO;
   o.pull = pull;
-  o.loop = pull == TBFS_PULL_LB ? TBFS_LOOP_PERSISTENT : TBFS_LOOP_HOST;
+  o.loop = pull == TBFS_PULL_LB ? TBFS_LOOP_CHAINED : TBFS_LOOP_HOST;
   DropinResult res;
   int rc = run_dropin(name, CP_h, I_h, nz, n, r, repetition, o, false, true, &res);
   if (rc) return rc;

@@ -539,15 +539,26 @@ static PersistParams make_params(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   return p;
 }
 
-// Enqueue one device-resident BFS (no host synchronisation).
+// Enqueue one device-resident BFS (no host synchronisation).  *refused is
+// set when the runtime declines the cooperative launch (for instance while
+// another process on the device holds the cooperative-launch resources):
+// nothing was enqueued then, and the caller runs the chained loop instead.
 static int enqueue_persistent(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
-                              const tbfs_opts& o, RunStats* stats_slot) {
+                              const tbfs_opts& o, RunStats* stats_slot,
+                              bool* refused) {
   const PersistParams p = make_params(c, g, root, o, stats_slot);
   const int block = o.block == 1024 ? 1024 : 256;
   int grid = block == 1024 ? c->persist_grid_max1024 : c->persist_grid_max;
   if (o.grid > 0 && o.grid < grid) grid = o.grid;
   HIPCHK(hipMemsetAsync(c->bar, 0, kBarWords * sizeof(unsigned), c->stream));
-  HIPCHK(kern::persist(c->stream, grid, block, p));
+  const hipError_t e = kern::persist(c->stream, grid, block, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    if (std::getenv("TBFS_VERBOSE"))
+      std::fprintf(stderr, "turbobfs: cooperative launch declined (%s); "
+                           "using the chained level loop\n", hipGetErrorString(e));
+    *refused = true;
+  }
   return TBFS_OK;
 }
 
@@ -765,7 +776,7 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
   }
   int rc = ctx_reserve_queue(c, g);
   if (rc) return rc;
-  const int loop = pick_loop(c, o);
+  int loop = pick_loop(c, o);
   if (loop == TBFS_LOOP_HOST) {
     for (int i = 0; i < count; ++i) {
       rc = run_host_loop(c, g, roots[i], o, st ? &st[i] : nullptr);
@@ -783,9 +794,12 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
   const int budget = c->chain_k;
   HIPCHK(hipEventRecord(c->evs[0], s));
   for (int i = 0; i < count; ++i) {
-    rc = loop == TBFS_LOOP_PERSISTENT
-             ? enqueue_persistent(c, g, roots[i], o, c->stats + i)
-             : enqueue_chained(c, g, roots[i], o, c->stats + i, budget);
+    if (loop == TBFS_LOOP_PERSISTENT) {
+      bool refused = false;
+      rc = enqueue_persistent(c, g, roots[i], o, c->stats + i, &refused);
+      if (refused) loop = TBFS_LOOP_CHAINED;  // this source and the rest
+    }
+    if (loop == TBFS_LOOP_CHAINED) rc = enqueue_chained(c, g, roots[i], o, c->stats + i, budget);
     if (rc) return rc;
     HIPCHK(hipEventRecord(c->evs[i + 1], s));
   }
@@ -851,13 +865,12 @@ void tbfs_opts_default(tbfs_opts* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
   o->mode = TBFS_MODE_AUTO;
-  o->loop = TBFS_LOOP_HOST;
   o->sigma = 1;
   o->bbreak = 0;
   o->alpha = 14.0f;
   o->beta = 24.0f;
   o->pull = TBFS_PULL_LB;
-  o->loop = TBFS_LOOP_PERSISTENT;
+  o->loop = TBFS_LOOP_CHAINED;
 }
 
 const char* tbfs_status_str(int s) {
