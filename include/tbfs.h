@@ -82,7 +82,9 @@ typedef struct tbfs_opts {
   int32_t pull;   /* TBFS_PULL_*: AUTO's bottom-up kernel */
   int32_t fb_lds; /* chained loop: 0 stages the frontier bitmap in LDS for
                      pull levels when it fits (n <= ~1.1M), -1 never */
-  int32_t reserved[6];
+  float batch_alpha; /* batches: expand top-down while the frontier's
+                        out-degree sum * batch_alpha < nz * words (0: 4) */
+  int32_t reserved[5];
 } tbfs_opts;
 
 typedef struct tbfs_stats {
@@ -142,6 +144,15 @@ int tbfs_bfs(tbfs_ctx *c, const tbfs_graph *g, int32_t root,
 int tbfs_bfs_many(tbfs_ctx *c, const tbfs_graph *g, const int32_t *roots,
                   int32_t count, const tbfs_opts *o, tbfs_stats *st);
 
+/* One BFS whose depth-0 frontier is every vertex in roots[0..nroots-1]
+ * (repeats allowed): S[v] = distance to the nearest root, sigma[v] = number
+ * of shortest paths from any root.  Packing many graphs as diagonal blocks of
+ * one CSC with one root per block gives each block exactly its own
+ * single-source result in one pass.  Runs the chained loop (mode AUTO or
+ * PUSH); results stay in the context as for tbfs_bfs. */
+int tbfs_bfs_multi(tbfs_ctx *c, const tbfs_graph *g, const int32_t *roots,
+                   int32_t nroots, const tbfs_opts *o, tbfs_stats *st);
+
 /* Copy the last BFS's results to host memory (any pointer may be NULL;
  * n must equal that graph's vertex count). */
 int tbfs_ctx_read(tbfs_ctx *c, int32_t n, int32_t *S, uint64_t *sigma,
@@ -172,6 +183,37 @@ int tbfs_device_info(int32_t dev, char *name, int32_t name_len, int32_t *cus,
 /* Device pointers of the context's S / sigma / ovf buffers. */
 int tbfs_ctx_device_ptrs(tbfs_ctx *c, int32_t **S, uint64_t **sigma,
                          uint8_t **ovf);
+
+/* ---- batched sources ---------------------------------------------------
+ * A batch runs up to 1024 BFS from different roots on one graph at once: the
+ * frontiers are bit-packed, 64 sources per 64-bit word, and every level is
+ * one pass over the in-lists for all sources.  Each source's S and sigma are
+ * exactly those of tbfs_bfs from that root.  A batch object owns its own HIP
+ * stream and device state for graphs of up to n_cap vertices and up to b_cap
+ * sources (b_cap <= 1024); device memory ~ n_cap * b_cap * 12 bytes (S and
+ * sigma of every (vertex, source) pair) plus 56 bytes per vertex per 64
+ * sources. */
+typedef struct tbfs_batch tbfs_batch;
+
+int tbfs_batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch **out);
+int tbfs_batch_destroy(tbfs_batch *b);
+void *tbfs_batch_stream(tbfs_batch *b);
+
+/* BFS from roots[0..count-1] (1 <= count <= b_cap; roots may repeat) on the
+ * CSC graph g.  opts may be NULL; only opts->grid (workgroups per level
+ * kernel, 0: default) and opts->batch_alpha are used.  st (may be NULL)
+ * receives one entry for the whole batch: levels = max over sources of their
+ * level count, push_levels / pull_levels = levels expanded top-down /
+ * bottom-up, reached and edges summed over sources, ms = device time of the
+ * batch. */
+int tbfs_batch_run(tbfs_batch *b, const tbfs_graph *g, const int32_t *roots,
+                   int32_t count, const tbfs_opts *opts, tbfs_stats *st);
+
+/* Results of sources first .. first+count-1 of the last batch, one row of n
+ * entries per source (S[i*n + v] is the depth of v from roots[first+i]; any
+ * pointer may be NULL; n must equal the graph's vertex count). */
+int tbfs_batch_read(tbfs_batch *b, int32_t first, int32_t count, int32_t n,
+                    int32_t *S, uint64_t *sigma, uint8_t *ovf);
 
 #ifdef __cplusplus
 }

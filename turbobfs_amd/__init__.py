@@ -8,6 +8,7 @@ workload generators.
 from . import _lib, graphs  # noqa: F401
 from ._lib import LIB_PATH, TbfsError, device_count, device_info, load  # noqa: F401
 from .bfs import (  # noqa: F401
+    Batch,
     Context,
     Graph,
     Result,
@@ -22,7 +23,7 @@ from .bfs import (  # noqa: F401
 )
 
 __all__ = [
-    "Context", "Graph", "Result", "default_opts", "graphs", "load", "LIB_PATH",
+    "Batch", "Context", "Graph", "Result", "default_opts", "graphs", "load", "LIB_PATH",
     "device_count", "device_info",
     "TbfsError",
     "bfs_gpu_td_csc_sc", "bfs_gpu_td_csc_wa", "bfs_gpu_bu_csc_sc",

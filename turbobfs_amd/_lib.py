@@ -56,7 +56,8 @@ class Opts(C.Structure):
         ("grid", C.c_int32),
         ("pull", C.c_int32),
         ("fb_lds", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("batch_alpha", C.c_float),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -95,6 +96,7 @@ SIGNATURES = {
     "tbfs_ctx_stream": (C.c_void_p, [C.c_void_p]),
     "tbfs_bfs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(Opts), C.POINTER(Stats)]),
     "tbfs_bfs_many": (C.c_int, [C.c_void_p, C.c_void_p, _i32p, C.c_int32, C.POINTER(Opts), C.POINTER(Stats)]),
+    "tbfs_bfs_multi": (C.c_int, [C.c_void_p, C.c_void_p, _i32p, C.c_int32, C.POINTER(Opts), C.POINTER(Stats)]),
     "tbfs_ctx_read": (C.c_int, [C.c_void_p, C.c_int32, _i32p, _u64p, _u8p]),
     "tbfs_ctx_device_ptrs": (C.c_int, [C.c_void_p, C.POINTER(_i32p), C.POINTER(_u64p), C.POINTER(_u8p)]),
     "tbfs_ctx_mark": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -103,6 +105,11 @@ SIGNATURES = {
     "tbfs_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
     "tbfs_device_info": (C.c_int, [C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "tbfs_batch_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "tbfs_batch_destroy": (C.c_int, [C.c_void_p]),
+    "tbfs_batch_stream": (C.c_void_p, [C.c_void_p]),
+    "tbfs_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, _i32p, C.c_int32, C.POINTER(Opts), C.POINTER(Stats)]),
+    "tbfs_batch_read": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _i32p, _u64p, _u8p]),
     # generators (tbfs_gen.h)
     "tbfs_gen_er": (C.c_int, [C.c_int32, C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
     "tbfs_gen_rmat": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),

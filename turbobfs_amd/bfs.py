@@ -180,6 +180,17 @@ class Context:
                                         st), "tbfs_bfs_many")
         return [{f: getattr(x, f) for f, _ in Stats._fields_ if f != "pad"} for x in st]
 
+    def run_multi(self, g: Graph, roots, opts: Opts | None = None) -> dict:
+        """One BFS from all roots at once (depth 0 = every root); results stay
+        on the device (read with read())."""
+        roots = np.ascontiguousarray(roots, dtype=np.int32)
+        st = Stats()
+        o = opts if opts is not None else default_opts()
+        check(_lib.load().tbfs_bfs_multi(self._h, g.handle, _ptr(roots, _i32p),
+                                         int(roots.shape[0]), C.byref(o), C.byref(st)),
+              "tbfs_bfs_multi")
+        return {f: getattr(st, f) for f, _ in Stats._fields_ if f != "pad"}
+
     def mark(self, slot: int) -> None:
         """Record timing marker `slot` on this context's stream."""
         check(_lib.load().tbfs_ctx_mark(self._h, int(slot)), "tbfs_ctx_mark")
@@ -215,6 +226,56 @@ class Context:
     def close(self):
         if self._h:
             _lib.load().tbfs_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many sources per BFS: up to b_cap (<= 1024) roots on one graph at once,
+    frontiers bit-packed 64 sources per word.  Each source's result equals
+    Context.bfs from that root."""
+
+    def __init__(self, n_cap: int, b_cap: int):
+        self._h = C.c_void_p()
+        check(_lib.load().tbfs_batch_create(int(n_cap), int(b_cap), C.byref(self._h)),
+              "tbfs_batch_create")
+        self.n_cap = int(n_cap)
+        self.b_cap = int(b_cap)
+        self._n = 0
+        self._count = 0
+
+    def run(self, g: Graph, roots, opts: Opts | None = None) -> dict:
+        roots = np.ascontiguousarray(roots, dtype=np.int32)
+        st = Stats()
+        o = opts if opts is not None else default_opts()
+        check(_lib.load().tbfs_batch_run(self._h, g.handle, _ptr(roots, _i32p),
+                                         int(roots.shape[0]), C.byref(o), C.byref(st)),
+              "tbfs_batch_run")
+        self._n = g.n
+        self._count = int(roots.shape[0])
+        return {f: getattr(st, f) for f, _ in Stats._fields_ if f != "pad"}
+
+    def read(self, first: int = 0, count: int | None = None):
+        """(S, sigma, ovf) of sources first .. first+count-1, shape (count, n)."""
+        if count is None:
+            count = self._count - first
+        n = self._n
+        S = np.empty((count, n), dtype=np.int32)
+        sigma = np.empty((count, n), dtype=np.uint64)
+        ovf = np.empty((count, n), dtype=np.uint8)
+        check(_lib.load().tbfs_batch_read(self._h, int(first), int(count), int(n),
+                                          _ptr(S, _i32p), _ptr(sigma, _u64p), _ptr(ovf, _u8p)),
+              "tbfs_batch_read")
+        return S, sigma, ovf
+
+    def close(self):
+        if self._h:
+            _lib.load().tbfs_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

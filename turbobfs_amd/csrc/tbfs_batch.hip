@@ -1,0 +1,904 @@
+// tbfs_batch.hip -- many sources in one BFS: frontier words of 64 sources.
+//
+// A batch of up to B = 64*W sources (W = 1, 2, 4, 8 or 16) runs as one
+// level-synchronous BFS whose per-vertex state is W 64-bit words; bit j of
+// word w belongs to source 64*w + j.  Layout in HBM (batch object):
+//   F[3]    uint64[n][W]  frontier words of depth d (current), d+1 (next) and
+//                         d-1 (being cleared), rotating with d
+//   seen[2] uint64[n][W]  OR of the frontiers of depths < d (slot d & 1); bits
+//                         of unused sources (index >= count) start set
+//   ovfw    uint64[n][W]  per-source "path count >= 2^64" bits
+//   S       int32[n][B]   per-source depth   } valid where the pair's seen
+//   sigma   uint64[n][B]  per-source count   } bit is set (never-reached
+//                                              pairs are not written; reads
+//                                              give -1 / 0 for them)
+// S and sigma keep the source index fastest: the 64 lanes of a wavefront are
+// the 64 sources of one word, so a vertex's counts for a word are 512
+// contiguous bytes and every gather of a parent's counts is coalesced.
+//
+// Every level is a pull over static slices of the in-lists (kBatchSlice
+// entries at most).  A wavefront takes one (word w, slice of vertex v) item:
+// it loads v's unseen sources of word w, then, 64 in-list entries at a time
+// with one lane per entry u, ANDs u's frontier word with them; the nonzero
+// results mark shortest-path edges u -> v into depth d+1 for those sources.
+// For each such edge (taken in turn, all loads of a group of 16 in flight)
+// the lanes switch to one lane per source and add sigma[u][64w + lane] where
+// the edge's mask has the lane's bit.  Items run word-major, so the waves in
+// flight share one word and the caches hold that word's frontier array and
+// the count segments of its hubs.
+// A vertex whose in-list is one slice is written by that one wavefront with
+// plain stores (every reached (v, source) entry is written once, at its
+// discovery); longer in-lists are split over slices whose partial sums are
+// added with atomics into rows zeroed at batch start.
+//
+// Small frontiers expand top-down instead (push while the frontier's
+// out-degree sum times alpha stays below nz * W): the frontier words push
+// their bits along the out-lists with atomic ORs (the lane that sets a bit
+// zeroes that pair's count and writes its depth), and a second kernel walks
+// the same edges again adding each parent's count atomically into the pairs
+// now marked.
+//
+// The loop is chained: one kernel per level (plus the count kernel, which
+// returns at once after a pull level), enqueued back to back; each level reads the counters of its
+// frontier (written by the previous kernels), chooses its direction, stops
+// when the frontier is empty, and clears the buffers of depth d-1.
+//
+// Semantics per source: exactly the single-source BFS of the reference's
+// sequential oracle (TurboBFS_CSC_TD/bfstdcsc_seq.c:27-96): depth and number
+// of shortest paths, with exact 64-bit counts and overflow flags.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "tbfs.h"
+#include "tbfs_device.h"
+#include "tbfs_engine.h"
+#include "tbfs_internal.h"
+
+namespace tbfs {
+namespace {
+
+using namespace dev;
+
+constexpr int kBatchSlice = 256;  // in-list entries per pull slice
+constexpr int kBatchBlock = 256;  // threads per workgroup (4 wavefronts)
+constexpr int kBatchMaxW = 16;    // words per vertex: up to 1024 sources
+constexpr float kBatchAlpha = 32.0f;  // default push / pull switch
+
+// Level counters, striped over kStripes cache lines like the single-source
+// ones (a workgroup adds into stripe blockIdx & (kStripes - 1)).
+struct alignas(128) BStripe {
+  unsigned long long q;      // (v, w) words that became nonzero
+  unsigned long long bits;   // (v, source) pairs discovered
+  unsigned long long edges;  // sum of in-degrees over those pairs
+  unsigned long long mout;   // sum of out-degrees over the new words (push work)
+  unsigned int ovf;          // some new count is >= 2^64
+  unsigned int pad0;
+  unsigned long long pad1[11];
+};
+struct BCtrSet {
+  BStripe s[kStripes];
+};
+
+// Chained-loop state, two slots alternating by level parity.
+struct alignas(64) BLoop {
+  int32_t d;        // depth of the current frontier
+  int32_t done;     // the frontier of depth d is empty
+  uint32_t ovf_any; // some count so far is >= 2^64 (parents' flags matter)
+  int32_t pushing;  // level d - 1 expanded top-down (its counts come from the fix pass)
+  int32_t push_levels;
+  int32_t pull_levels;
+  unsigned long long reached;  // (v, source) pairs reached before depth d
+  unsigned long long edges;    // their in-degrees
+  unsigned long long pad[4];
+};
+
+struct BatchParams {
+  int32_t n;
+  int32_t nz;
+  int32_t nsrc;
+  const int32_t* CP;       // in-lists
+  const int32_t* IC;
+  const int32_t* RP;       // out-lists (alias CP / IC when symmetric)
+  const int32_t* JR;
+  const uint64_t* slices;  // pull slices: v | slice << 32
+  int32_t nslices;
+  const uint64_t* bigo;    // out-lists longer than kBigIn, in kPullChunk slices
+  int32_t nbigo;
+  float alpha;             // push while m_out * alpha < nz * W
+  uint64_t* F[3];
+  uint64_t* seen[2];
+  uint64_t* ovfw;
+  uint64_t* sigma;
+  int32_t* S;
+  BCtrSet* cset;  // [3]
+  BLoop* loop;    // [2]
+  RunStats* stats;
+  const int32_t* roots;
+};
+
+__device__ __forceinline__ uint64_t rl64(uint64_t x, int l) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Bits of word w that belong to one of the nsrc sources of the batch.
+__device__ __forceinline__ uint64_t word_valid(int nsrc, int w) {
+  const int k = nsrc - 64 * w;
+  return k >= 64 ? ~0ull : (k <= 0 ? 0ull : ((1ull << k) - 1ull));
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_batch_init(BatchParams p) {
+  constexpr int B = 64 * W;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nw = (int64_t)p.n * W;
+  for (int64_t i = gtid; i < nw; i += gstride) {
+    const uint64_t pad = ~word_valid(p.nsrc, (int)(i & (W - 1)));
+    p.F[0][i] = 0ull;
+    p.F[1][i] = 0ull;
+    p.F[2][i] = 0ull;
+    p.seen[0][i] = pad;
+    p.seen[1][i] = pad;
+    p.ovfw[i] = 0ull;
+  }
+  // rows of split in-lists receive atomic adds: zero them
+  for (int64_t t = gtid; t < (int64_t)p.nslices * W; t += gstride) {
+    const uint64_t e = p.slices[t / W];
+    const int v = (int)(uint32_t)e;
+    if ((e >> 32) != 0 || p.CP[v + 1] - p.CP[v] <= kBatchSlice) continue;
+    uint64_t* row = p.sigma + (int64_t)v * B + 64 * (t % W);
+    for (int j = 0; j < 64; ++j) row[j] = 0ull;
+  }
+  const int nctr = (int)(3 * sizeof(BCtrSet) / sizeof(unsigned long long));
+  unsigned long long* cw = reinterpret_cast<unsigned long long*>(p.cset);
+  for (int64_t i = gtid; i < nctr; i += gstride) cw[i] = 0ull;
+  if (gtid == 0) {
+    p.loop[0] = BLoop{};
+    p.loop[1] = BLoop{};
+    RunStats rs{};
+    rs.levels = -1;  // set by the level that finds the frontier empty
+    *p.stats = rs;
+  }
+}
+
+// Depth-0 frontier: one bit per source at its root.
+template <int W>
+__global__ __launch_bounds__(256) void k_batch_seed(BatchParams p) {
+  constexpr int B = 64 * W;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= p.nsrc) return;
+  const int r = p.roots[j];
+  const uint64_t bit = 1ull << (j & 63);
+  const unsigned long long old =
+      atomicOr((unsigned long long*)&p.F[0][(int64_t)r * W + (j >> 6)], (unsigned long long)bit);
+  p.S[(int64_t)r * B + j] = 0;
+  p.sigma[(int64_t)r * B + j] = 1ull;
+  BStripe& c = p.cset[0].s[j & (kStripes - 1)];
+  if (old == 0) {
+    atomicAdd(&c.q, 1ull);
+    atomicAdd(&c.mout, (unsigned long long)(p.RP[r + 1] - p.RP[r]));
+  }
+  atomicAdd(&c.bits, 1ull);
+  atomicAdd(&c.edges, (unsigned long long)(p.CP[r + 1] - p.CP[r]));
+}
+
+// Per-lane counters of one wavefront (summed over the wave when flushed).
+struct BAcc {
+  unsigned long long q = 0, bits = 0, edges = 0, mout = 0;
+  unsigned int ovf = 0;
+};
+
+__device__ __forceinline__ void flush_acc(BStripe& c, BAcc a) {
+  a.q = wave_sum_u64(a.q);
+  a.bits = wave_sum_u64(a.bits);
+  a.edges = wave_sum_u64(a.edges);
+  a.mout = wave_sum_u64(a.mout);
+  a.ovf = __ballot(a.ovf != 0) != 0;
+  if (lane() == 0) {
+    if (a.q) atomicAdd(&c.q, a.q);
+    if (a.bits) atomicAdd(&c.bits, a.bits);
+    if (a.edges) atomicAdd(&c.edges, a.edges);
+    if (a.mout) atomicAdd(&c.mout, a.mout);
+    if (a.ovf) atomicOr(&c.ovf, 1u);
+  }
+}
+
+// Pull over (word, slice) items in word-major order: the waves in flight
+// share one word, so the caches hold that word's frontier array and the count
+// segments of its hubs.  A wavefront first tests 64 consecutive items at once
+// (one per lane), then expands the active ones (some source of the word
+// still unseen at v) in turn: it discovers those sources (next frontier
+// word) and writes their depths and counts.
+template <int W>
+__device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, const uint64_t* Fc,
+                                           uint64_t* Fn, const uint64_t* sd, bool pov,
+                                           int64_t gtid, int64_t gstride, BAcc& a) {
+  constexpr int B = 64 * W;
+  const int ln = lane();
+  const int64_t nwaves = gstride >> 6;
+  const int64_t items = (int64_t)p.nslices * W;
+  for (int64_t base = (gtid >> 6) * 64; base < items; base += nwaves * 64) {
+    const int64_t t = base + ln;
+    int w = 0, v = 0, k0 = 0, k1 = 0, deg = 0;
+    uint64_t tgt = 0;
+    if (t < items) {
+      w = (int)(t / p.nslices);
+      const uint64_t e = p.slices[t - (int64_t)w * p.nslices];
+      v = (int)(uint32_t)e;
+      const int beg0 = p.CP[v], end0 = p.CP[v + 1];
+      k0 = beg0 + (int)(e >> 32) * kBatchSlice;
+      k1 = min(end0, k0 + kBatchSlice);
+      deg = end0 - beg0;
+      const int64_t iw = (int64_t)v * W + w;
+      tgt = ~(sd[iw] | Fc[iw]);
+    }
+    uint64_t live = __ballot(tgt != 0);
+    while (live) {
+      const int src = __builtin_ctzll(live);
+      live &= live - 1;
+      const int wi = __builtin_amdgcn_readlane(w, src);
+      const int vi = __builtin_amdgcn_readlane(v, src);
+      const int ka = __builtin_amdgcn_readlane(k0, src);
+      const int kb = __builtin_amdgcn_readlane(k1, src);
+      const int dg = __builtin_amdgcn_readlane(deg, src);
+      const uint64_t un = rl64(tgt, src);
+      const uint64_t* sgw = p.sigma + 64 * wi + ln;
+      uint64_t acc = 0;
+      bool hit = false, ovb = false;
+      for (int c0 = ka; c0 < kb; c0 += 64) {
+        const int k = c0 + ln;
+        int u = 0;
+        uint64_t mm = 0;
+        if (k < kb) {
+          u = p.IC[k];
+          mm = Fc[(int64_t)u * W + wi] & un;  // sources for which u -> v is a parent edge
+        }
+        uint64_t bal = __ballot(mm != 0);
+        // parent edges in groups of G: all count loads of a group are issued
+        // before any is added
+        constexpr int G = 16;
+        while (bal) {
+          uint64_t x[G];
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            x[g] = 0ull;
+            if (bal) {
+              const int pe = __builtin_ctzll(bal);
+              bal &= bal - 1;
+              const int ue = __builtin_amdgcn_readlane(u, pe);
+              const uint64_t me = rl64(mm, pe);
+              if ((me >> ln) & 1ull) {
+                x[g] = sgw[(int64_t)ue * B];
+                hit = true;
+                if (pov && ((p.ovfw[(int64_t)ue * W + wi] >> ln) & 1ull)) ovb = true;
+              }
+            }
+          }
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            const uint64_t y = acc + x[g];
+            if (y < acc) ovb = true;
+            acc = y;
+          }
+        }
+      }
+      const uint64_t nb = __ballot(hit);
+      if (nb == 0) continue;
+      uint64_t ob = __ballot(ovb);
+      const int64_t iw = (int64_t)vi * W + wi;
+      const int64_t ix = (int64_t)vi * B + 64 * wi + ln;
+      if (dg <= kBatchSlice) {  // the only slice of v: plain stores
+        if (hit) {
+          p.S[ix] = d + 1;
+          p.sigma[ix] = acc;
+        }
+        if (ln == 0) {
+          Fn[iw] = nb;
+          if (ob) p.ovfw[iw] |= ob;
+        }
+        if (ln == 0) {
+          a.q += 1;
+          a.bits += (unsigned long long)__popcll(nb);
+          a.edges += (unsigned long long)__popcll(nb) * (unsigned long long)dg;
+          a.mout += (unsigned long long)(p.RP[vi + 1] - p.RP[vi]);
+        }
+      } else {  // one of several slices: partial sums into a zeroed row
+        bool cy = false;
+        if (hit) {
+          const unsigned long long old =
+              atomicAdd((unsigned long long*)&p.sigma[ix], (unsigned long long)acc);
+          cy = old + acc < old;
+          p.S[ix] = d + 1;
+        }
+        ob |= __ballot(cy);
+        if (ln == 0) {
+          if (ob) atomicOr((unsigned long long*)&p.ovfw[iw], (unsigned long long)ob);
+          {
+            const unsigned long long old =
+                atomicOr((unsigned long long*)&Fn[iw], (unsigned long long)nb);
+            const uint64_t newly = nb & ~(uint64_t)old;
+            a.bits += (unsigned long long)__popcll(newly);
+            a.edges += (unsigned long long)__popcll(newly) * (unsigned long long)dg;
+            if (old == 0) {
+              a.q += 1;
+              a.mout += (unsigned long long)(p.RP[vi + 1] - p.RP[vi]);
+            }
+          }
+        }
+      }
+      if (ob && ln == 0) a.ovf = 1;
+    }
+  }
+}
+
+// Top-down edge u -> v for word w, f = frontier word of u.
+//   ADD = false (mark): the sources of f unvisited at v join the next
+//     frontier (atomic OR); the lane that sets a bit first zeroes that pair's
+//     count and writes its depth.
+//   ADD = true (after every mark of the level has landed): the sources of f
+//     now marked at v (exactly the shortest-path edges u -> v into depth
+//     d + 1) get u's count added atomically, with carry and inherited
+//     overflow flags.
+template <int W, bool ADD>
+__device__ __forceinline__ void push_edge(const BatchParams& p, const uint64_t* Fc, uint64_t* Fn,
+                                          const uint64_t* sd, int32_t d, int u, int v, int w,
+                                          uint64_t f, bool pov, BAcc& a) {
+  constexpr int B = 64 * W;
+  const int64_t iv = (int64_t)v * W + w;
+  if (!ADD) {
+    const uint64_t m = f & ~(sd[iv] | Fc[iv]);
+    if (!m) return;
+    const unsigned long long old = atomicOr((unsigned long long*)&Fn[iv], (unsigned long long)m);
+    uint64_t newly = m & ~(uint64_t)old;
+    if (!newly) return;
+    const unsigned long long c = (unsigned long long)__popcll(newly);
+    a.bits += c;
+    a.edges += c * (unsigned long long)(p.CP[v + 1] - p.CP[v]);
+    if (old == 0) {
+      a.q += 1;
+      a.mout += (unsigned long long)(p.RP[v + 1] - p.RP[v]);
+    }
+    while (newly) {
+      const int sb = __builtin_ctzll(newly);
+      newly &= newly - 1;
+      const int64_t ix = (int64_t)v * B + 64 * w + sb;
+      p.sigma[ix] = 0ull;
+      p.S[ix] = d + 1;
+    }
+  } else {
+    uint64_t m = f & Fn[iv];
+    if (!m) return;
+    const uint64_t po = pov ? p.ovfw[(int64_t)u * W + w] : 0ull;
+    uint64_t ob = 0;
+    while (m) {
+      const int sb = __builtin_ctzll(m);
+      m &= m - 1;
+      const uint64_t x = p.sigma[(int64_t)u * B + 64 * w + sb];
+      const unsigned long long old =
+          atomicAdd((unsigned long long*)&p.sigma[(int64_t)v * B + 64 * w + sb], (unsigned long long)x);
+      if (old + x < old || ((po >> sb) & 1ull)) ob |= 1ull << sb;
+    }
+    if (ob) {
+      atomicOr((unsigned long long*)&p.ovfw[iv], (unsigned long long)ob);
+      a.ovf = 1;
+    }
+  }
+}
+
+struct PushLds {
+  int32_t excl[64];
+  int32_t beg[64];
+  int32_t u[64];
+  int32_t w[64];
+  uint64_t f[64];
+};
+
+// Top-down pass over every nonzero frontier word (u, w): u's out-list edges.
+// Lists up to kBigIn entries are split over the lanes of a wavefront by edge
+// count (prefix scan + owner search); longer lists go by their static
+// kPullChunk slices, one (slice, word) item per wavefront.
+template <int W, bool ADD>
+__device__ __forceinline__ void batch_push(const BatchParams& p, const uint64_t* Fc,
+                                           uint64_t* Fn, const uint64_t* sd, int32_t d, bool pov,
+                                           int64_t gtid, int64_t gstride, PushLds& L, BAcc& a) {
+  const int ln = lane();
+  const int64_t nwaves = gstride >> 6;
+  const int64_t nW = (int64_t)p.n * W;
+  for (int64_t base = (gtid >> 6) * 64; base < nW; base += nwaves * 64) {
+    const int64_t i = base + ln;
+    uint64_t f = 0;
+    int od = 0, rb = 0;
+    if (i < nW) {
+      f = Fc[i];
+      if (f) {
+        const int u = (int)(i / W);
+        rb = p.RP[u];
+        od = p.RP[u + 1] - rb;
+        if (od > kBigIn) od = 0;  // served by the static slices below
+      }
+    }
+    const uint32_t incl = wave_incl_scan((uint32_t)od);
+    const int total = (int)__shfl(incl, 63);
+    if (total == 0) continue;
+    L.excl[ln] = (int)incl - od;
+    L.beg[ln] = rb;
+    L.u[ln] = (int)(i / W);
+    L.w[ln] = (int)(i % W);
+    L.f[ln] = f;
+    wave_lds_sync();
+    for (int e0 = 0; e0 < total; e0 += 64) {
+      const int e = e0 + ln;
+      if (e < total) {
+        const int o = owner_of(L.excl, e);
+        const int v = p.JR[L.beg[o] + (e - L.excl[o])];
+        push_edge<W, ADD>(p, Fc, Fn, sd, d, L.u[o], v, L.w[o], L.f[o], pov, a);
+      }
+    }
+    wave_lds_sync();
+  }
+  const int64_t hubs = (int64_t)p.nbigo * W;
+  for (int64_t t = gtid >> 6; t < hubs; t += nwaves) {
+    const int w = (int)(t % W);
+    const uint64_t e = p.bigo[t / W];
+    const int u = (int)(uint32_t)e;
+    const uint64_t f = Fc[(int64_t)u * W + w];
+    if (!f) continue;
+    const int kb = p.RP[u] + (int)(e >> 32) * kPullChunk;
+    const int ke = min(p.RP[u + 1], kb + kPullChunk);
+    for (int k = kb + ln; k < ke; k += 64)
+      push_edge<W, ADD>(p, Fc, Fn, sd, d, u, p.JR[k], w, f, pov, a);
+  }
+}
+
+// Level d: expand the frontier of depth d into depth d + 1.
+template <int W>
+__global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int32_t d) {
+  __shared__ PushLds lds[kBatchBlock / 64];
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+
+  BLoop st = p.loop[d & 1];
+  if (st.done) {
+    if (gtid == 0) p.loop[(d + 1) & 1] = st;
+    return;
+  }
+  unsigned long long q = 0, bits = 0, edges = 0, mout = 0;
+  unsigned int ovf = 0;
+  {
+    const BCtrSet& cs = p.cset[d % 3];
+#pragma unroll
+    for (int k = 0; k < kStripes; ++k) {
+      q += cs.s[k].q;
+      bits += cs.s[k].bits;
+      edges += cs.s[k].edges;
+      mout += cs.s[k].mout;
+      ovf |= cs.s[k].ovf;
+    }
+  }
+  st.reached += bits;
+  st.edges += edges;
+  st.ovf_any |= ovf;
+  if (q == 0) {
+    st.done = 1;
+    if (gtid == 0) {
+      p.loop[(d + 1) & 1] = st;
+      RunStats rs{};
+      rs.levels = d;
+      rs.push_levels = st.push_levels;
+      rs.pull_levels = st.pull_levels;
+      rs.ovf_any = st.ovf_any;
+      rs.reached = st.reached;
+      rs.edges = st.edges;
+      *p.stats = rs;
+    }
+    return;
+  }
+  const bool push = (double)mout * (double)p.alpha < (double)p.nz * (double)W;
+  if (gtid == 0) {
+    BLoop nx = st;
+    nx.d = d + 1;
+    nx.pushing = push ? 1 : 0;
+    nx.push_levels += push ? 1 : 0;
+    nx.pull_levels += push ? 0 : 1;
+    p.loop[(d + 1) & 1] = nx;
+  }
+
+  const uint64_t* Fc = p.F[d % 3];
+  uint64_t* Fn = p.F[(d + 1) % 3];
+  uint64_t* Fp = p.F[(d + 2) % 3];
+  const uint64_t* sd = p.seen[d & 1];
+  uint64_t* sn = p.seen[(d + 1) & 1];
+
+  // housekeeping for level d + 1: seen through depth d, clear depth d - 1
+  // (slot (d+1)&1 holds the OR through depth d-2: unchanged where neither
+  // frontier d-1 nor frontier d has a bit)
+  const int64_t nw = (int64_t)p.n * W;
+  for (int64_t i = gtid; i < nw; i += gstride) {
+    const uint64_t fc = Fc[i], fp = Fp[i];
+    if (fc | fp) {
+      sn[i] = sd[i] | fc;
+      if (fp) Fp[i] = 0ull;
+    }
+  }
+  if (gtid < kStripes) {
+    BStripe& c = p.cset[(d + 2) % 3].s[gtid];
+    c.q = 0;
+    c.bits = 0;
+    c.edges = 0;
+    c.mout = 0;
+    c.ovf = 0;
+  }
+
+  BAcc a;
+  if (push)
+    batch_push<W, false>(p, Fc, Fn, sd, d, false, gtid, gstride, lds[threadIdx.x >> 6], a);
+  else
+    batch_pull<W>(p, d, Fc, Fn, sd, st.ovf_any != 0, gtid, gstride, a);
+  flush_acc(p.cset[(d + 1) % 3].s[blockIdx.x & (kStripes - 1)], a);
+}
+
+// After a push level d: every edge of the frontier adds its count into the
+// pairs the level marked (their counts were zeroed by the marking lanes).
+// Nothing to do after a pull level.
+template <int W>
+__global__ __launch_bounds__(kBatchBlock) void k_batch_scatter(BatchParams p, int32_t d) {
+  __shared__ PushLds lds[kBatchBlock / 64];
+  const BLoop nx = p.loop[(d + 1) & 1];
+  if (nx.done || nx.d != d + 1 || !nx.pushing) return;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  // seen[d & 1] and F[d % 3] are still those of level d (level d + 1 has not run)
+  BAcc a;
+  batch_push<W, true>(p, p.F[d % 3], p.F[(d + 1) % 3], p.seen[d & 1], d, nx.ovf_any != 0, gtid,
+                      gstride, lds[threadIdx.x >> 6], a);
+  flush_acc(p.cset[(d + 1) % 3].s[blockIdx.x & (kStripes - 1)], a);
+}
+
+// Per-source rows out of the [n][B] layout: sources first .. first+count-1
+// (count <= 64) into So[count][n], sgo[count][n], ovo[count][n].  Pairs whose
+// bit is clear in `seen` (never reached) read S = -1, sigma = 0: the level
+// kernels never write them.
+__global__ __launch_bounds__(256) void k_batch_extract(
+    const int32_t* __restrict__ S, const uint64_t* __restrict__ sigma,
+    const uint64_t* __restrict__ ovfw, const uint64_t* __restrict__ seen, int32_t n,
+    int32_t B, int32_t W, int32_t first, int32_t count, int32_t* So, uint64_t* sgo,
+    uint8_t* ovo) {
+  __shared__ int32_t ts[64][65];
+  __shared__ uint64_t tg[64][65];
+  __shared__ uint8_t to[64][65];
+  const int64_t v0 = (int64_t)blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {  // r: vertex row of the tile, tx: source
+    const int64_t v = v0 + r;
+    if (v < n && tx < count) {
+      const int j = first + tx;
+      const int64_t iw = v * W + (j >> 6);
+      const bool reached = (seen[iw] >> (j & 63)) & 1ull;
+      ts[r][tx] = reached ? S[v * B + j] : -1;
+      tg[r][tx] = reached ? sigma[v * B + j] : 0ull;
+      to[r][tx] = reached ? (uint8_t)((ovfw[iw] >> (j & 63)) & 1ull) : (uint8_t)0;
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < count; r += 4) {  // r: source, tx: vertex
+    const int64_t v = v0 + tx;
+    if (v < n) {
+      So[(int64_t)r * n + v] = ts[tx][r];
+      sgo[(int64_t)r * n + v] = tg[tx][r];
+      ovo[(int64_t)r * n + v] = to[tx][r];
+    }
+  }
+}
+
+}  // namespace
+}  // namespace tbfs
+
+// ----------------------------------------------------------------------------
+// host side
+using namespace tbfs;
+
+struct tbfs_batch {
+  int32_t n_cap = 0;
+  int32_t b_cap = 0;
+  int32_t w_cap = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint64_t* F[3] = {nullptr, nullptr, nullptr};
+  uint64_t* seen[2] = {nullptr, nullptr};
+  uint64_t* ovfw = nullptr;
+  uint64_t* sigma = nullptr;
+  int32_t* S = nullptr;
+  BCtrSet* cset = nullptr;
+  BLoop* loop = nullptr;
+  RunStats* stats = nullptr;
+  RunStats* stats_h = nullptr;
+  int32_t* roots = nullptr;
+  int32_t* xS = nullptr;  // extraction staging: 64 rows of n_cap
+  uint64_t* xsig = nullptr;
+  uint8_t* xovf = nullptr;
+  int cus = 0;
+  int chain_k = 8;
+  int32_t last_n = 0;
+  int32_t last_count = 0;
+  int32_t last_w = 0;
+  const uint64_t* seen_final = nullptr;  // reached bits of the last batch
+};
+
+namespace tbfs {
+namespace {
+
+int words_for(int32_t count) {
+  int w = 1;
+  while (64 * w < count) w <<= 1;
+  return w;
+}
+
+void batch_free(tbfs_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (int i = 0; i < 3; ++i) dev_free(b->F[i]);
+  for (int i = 0; i < 2; ++i) dev_free(b->seen[i]);
+  dev_free(b->ovfw);
+  dev_free(b->sigma);
+  dev_free(b->S);
+  dev_free(b->cset);
+  dev_free(b->loop);
+  dev_free(b->stats);
+  dev_free(b->roots);
+  dev_free(b->xS);
+  dev_free(b->xsig);
+  dev_free(b->xovf);
+  if (b->stats_h) (void)hipHostFree(b->stats_h);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch** out) {
+  if (!out || n_cap <= 0 || b_cap <= 0 || b_cap > 64 * kBatchMaxW) return TBFS_ERR_INVALID;
+  *out = nullptr;
+  tbfs_batch* b = new (std::nothrow) tbfs_batch();
+  if (!b) return TBFS_ERR_NOMEM;
+  b->n_cap = n_cap;
+  b->b_cap = b_cap;
+  b->w_cap = words_for(b_cap);
+  const size_t nw = (size_t)n_cap * b->w_cap;
+  const size_t nb = nw * 64;
+  int dev = 0;
+  int rc = TBFS_OK;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e != hipSuccess) {
+    rc = hip_fail(e, "batch stream/events", __FILE__, __LINE__);
+    batch_free(b);
+    return rc;
+  }
+  for (int i = 0; i < 3 && !rc; ++i) rc = dev_alloc(&b->F[i], nw);
+  for (int i = 0; i < 2 && !rc; ++i) rc = dev_alloc(&b->seen[i], nw);
+  if (!rc) rc = dev_alloc(&b->ovfw, nw);
+  if (!rc) rc = dev_alloc(&b->sigma, nb);
+  if (!rc) rc = dev_alloc(&b->S, nb);
+  if (!rc) rc = dev_alloc(&b->cset, 3);
+  if (!rc) rc = dev_alloc(&b->loop, 2);
+  if (!rc) rc = dev_alloc(&b->stats, 1);
+  if (!rc) rc = dev_alloc(&b->roots, (size_t)b_cap);
+  if (!rc) {
+    e = hipHostMalloc((void**)&b->stats_h, sizeof(RunStats), hipHostMallocDefault);
+    if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc", __FILE__, __LINE__);
+  }
+  if (rc) {
+    batch_free(b);
+    return rc;
+  }
+  *out = b;
+  return TBFS_OK;
+}
+
+// Pull slices of the in-lists (built on first use, kept with the graph).
+int graph_ensure_batch_slices(tbfs_graph* g) {
+  if (g->bsl) return TBFS_OK;
+  std::vector<int32_t> cp((size_t)g->n + 1);
+  HIPCHK(hipMemcpy(cp.data(), g->CP, cp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<uint64_t> sl;
+  sl.reserve((size_t)g->n);
+  for (int32_t v = 0; v < g->n; ++v) {
+    const int64_t deg = (int64_t)cp[v + 1] - cp[v];
+    const int64_t ns = (deg + kBatchSlice - 1) / kBatchSlice;
+    for (int64_t c = 0; c < ns; ++c) sl.push_back((uint64_t)(uint32_t)v | ((uint64_t)c << 32));
+  }
+  uint64_t* d = nullptr;
+  int rc = dev_alloc(&d, sl.size());
+  if (rc) return rc;
+  if (!sl.empty()) {
+    hipError_t e = hipMemcpy(d, sl.data(), sl.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      dev_free(d);
+      return hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
+    }
+  }
+  g->bsl = d;
+  g->nbsl = (int32_t)sl.size();
+  return TBFS_OK;
+}
+
+template <int W>
+int enqueue_levels(tbfs_batch* b, const BatchParams& p, int d0, int count, int grid) {
+  for (int d = d0; d < d0 + count; ++d) {
+    hipLaunchKernelGGL(k_batch_level<W>, dim3(grid), dim3(kBatchBlock), 0, b->stream, p, d);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_scatter<W>, dim3(grid), dim3(kBatchBlock), 0, b->stream, p, d);
+    HIPCHK(hipGetLastError());
+  }
+  return TBFS_OK;
+}
+
+template <int W>
+int batch_run_w(tbfs_batch* b, const tbfs_graph* g, int32_t count, const tbfs_opts& o,
+                tbfs_stats* st) {
+  BatchParams p{};
+  p.n = g->n;
+  p.nz = g->nz;
+  p.nsrc = count;
+  p.CP = g->CP;
+  p.IC = g->IC;
+  p.RP = g->RP;
+  p.JR = g->JR;
+  p.slices = g->bsl;
+  p.nslices = g->nbsl;
+  p.bigo = g->bigo;
+  p.nbigo = g->nbigo;
+  p.alpha = o.batch_alpha > 0 ? o.batch_alpha : kBatchAlpha;
+  for (int i = 0; i < 3; ++i) p.F[i] = b->F[i];
+  for (int i = 0; i < 2; ++i) p.seen[i] = b->seen[i];
+  p.ovfw = b->ovfw;
+  p.sigma = b->sigma;
+  p.S = b->S;
+  p.cset = b->cset;
+  p.loop = b->loop;
+  p.stats = b->stats;
+  p.roots = b->roots;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_level<W>, kBatchBlock, 0) !=
+          hipSuccess || per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 4;
+  }
+  int grid = o.grid > 0 ? std::min(o.grid, 65536) : std::max(1, b->cus) * per_cu;
+  hipStream_t s = b->stream;
+  HIPCHK(hipEventRecord(b->ev0, s));
+  const int igrid = grid_for((int64_t)g->n * W, 256, std::max(1, b->cus) * 8);
+  hipLaunchKernelGGL(k_batch_init<W>, dim3(igrid), dim3(256), 0, s, p);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_batch_seed<W>, dim3((count + 255) / 256), dim3(256), 0, s, p);
+  HIPCHK(hipGetLastError());
+  int d = 0;
+  int rc = enqueue_levels<W>(b, p, d, b->chain_k, grid);
+  if (rc) return rc;
+  d += b->chain_k;
+  for (;;) {
+    HIPCHK(hipMemcpyAsync(b->stats_h, b->stats, sizeof(RunStats), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (b->stats_h->levels >= 0) break;
+    if (d > g->n + 1) {  // a BFS has at most n levels: the loop state is corrupt
+      std::fprintf(stderr, "turbobfs: batched BFS did not finish within %d levels\n", d);
+      return TBFS_ERR_HIP;
+    }
+    rc = enqueue_levels<W>(b, p, d, b->chain_k, grid);
+    if (rc) return rc;
+    d += b->chain_k;
+  }
+  HIPCHK(hipEventRecord(b->ev1, s));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  const RunStats& rs = *b->stats_h;
+  // one spare level beyond the deepest batch so far
+  b->chain_k = std::max(b->chain_k, std::min(4096, rs.levels + 2));
+  b->last_n = g->n;
+  b->last_count = count;
+  b->last_w = W;
+  b->seen_final = b->seen[rs.levels & 1];  // OR of every frontier
+  if (st) {
+    st->levels = rs.levels;
+    st->push_levels = rs.push_levels;
+    st->pull_levels = rs.pull_levels;
+    st->pad = 0;
+    st->reached = (int64_t)rs.reached;
+    st->edges = (int64_t)rs.edges;
+    st->ovf64 = rs.ovf_any ? -1 : 0;
+    st->ms = (double)ms;
+  }
+  return TBFS_OK;
+}
+
+int batch_run(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32_t count,
+              const tbfs_opts* o_in, tbfs_stats* st) {
+  if (!b || !gc || !roots || count < 1 || count > b->b_cap) return TBFS_ERR_INVALID;
+  tbfs_graph* g = const_cast<tbfs_graph*>(gc);
+  if (!g->CP) return TBFS_ERR_UNSUPPORTED;  // needs in-lists (CSC)
+  if (g->n > b->n_cap) return TBFS_ERR_INVALID;
+  for (int32_t i = 0; i < count; ++i)
+    if (roots[i] < 0 || roots[i] >= g->n) return TBFS_ERR_INVALID;
+  tbfs_opts o;
+  if (o_in) o = *o_in; else tbfs_opts_default(&o);
+  int rc = graph_ensure_batch_slices(g);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(b->roots, roots, (size_t)count * sizeof(int32_t),
+                        hipMemcpyHostToDevice, b->stream));
+  switch (words_for(count)) {
+    case 1: return batch_run_w<1>(b, g, count, o, st);
+    case 2: return batch_run_w<2>(b, g, count, o, st);
+    case 4: return batch_run_w<4>(b, g, count, o, st);
+    case 8: return batch_run_w<8>(b, g, count, o, st);
+    default: return batch_run_w<16>(b, g, count, o, st);
+  }
+}
+
+int batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* S,
+               uint64_t* sigma, uint8_t* ovf) {
+  if (!b || n != b->last_n || n <= 0 || first < 0 || count < 0 ||
+      first + count > b->last_count)
+    return TBFS_ERR_INVALID;
+  if (count == 0) return TBFS_OK;
+  if (!b->xS) {
+    int rc = dev_alloc(&b->xS, (size_t)64 * b->n_cap);
+    if (!rc) rc = dev_alloc(&b->xsig, (size_t)64 * b->n_cap);
+    if (!rc) rc = dev_alloc(&b->xovf, (size_t)64 * b->n_cap);
+    if (rc) return rc;
+  }
+  const int W = b->last_w;
+  const int B = 64 * W;
+  hipStream_t s = b->stream;
+  for (int32_t f = 0; f < count; f += 64) {
+    const int32_t c = std::min(64, count - f);
+    hipLaunchKernelGGL(k_batch_extract, dim3((n + 63) / 64), dim3(256), 0, s, b->S, b->sigma,
+                       b->ovfw, b->seen_final, n, B, W, first + f, c, b->xS, b->xsig, b->xovf);
+    HIPCHK(hipGetLastError());
+    const size_t rows = (size_t)c * n;
+    if (S) HIPCHK(hipMemcpyAsync(S + (size_t)f * n, b->xS, rows * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (sigma) HIPCHK(hipMemcpyAsync(sigma + (size_t)f * n, b->xsig, rows * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    if (ovf) HIPCHK(hipMemcpyAsync(ovf + (size_t)f * n, b->xovf, rows, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return TBFS_OK;
+}
+
+}  // namespace
+}  // namespace tbfs
+
+extern "C" {
+
+int tbfs_batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch** out) {
+  return tbfs::batch_create(n_cap, b_cap, out);
+}
+
+int tbfs_batch_destroy(tbfs_batch* b) {
+  tbfs::batch_free(b);
+  return TBFS_OK;
+}
+
+void* tbfs_batch_stream(tbfs_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+int tbfs_batch_run(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t count,
+                   const tbfs_opts* o, tbfs_stats* st) {
+  return tbfs::batch_run(b, g, roots, count, o, st);
+}
+
+int tbfs_batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* S,
+                    uint64_t* sigma, uint8_t* ovf) {
+  return tbfs::batch_read(b, first, count, n, S, sigma, ovf);
+}
+
+}  // extern "C"

@@ -26,6 +26,8 @@ struct tbfs_graph {
   uint64_t* bigo = nullptr;  // static push slices of long out-lists (may alias bigc)
   int32_t nbigo = 0;
   uint64_t* skip = nullptr;  // static: in-degree 0 or > kBigIn (one bit per vertex)
+  uint64_t* bsl = nullptr;   // batched pull slices of the in-lists (v | slice << 32),
+  int32_t nbsl = 0;          // built on first use
 };
 
 // Per-BFS device state bound to one stream.
@@ -59,10 +61,44 @@ struct tbfs_ctx {
   size_t fbl_limit = 0;             // bytes of LDS for a staged frontier bitmap
   int cus = 0;
   int32_t last_n = 0;
+  int32_t* roots_d = nullptr;       // roots of a multi-root BFS
+  int32_t roots_cap = 0;
 };
 
 namespace tbfs {
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define HIPCHK(call)                                                      \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) return hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+template <typename T>
+inline int dev_alloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+  }
+  return TBFS_OK;
+}
+
+template <typename T>
+inline void dev_free(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+inline int grid_for(int64_t items, int block, int cap) {
+  int64_t g = (items + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
 int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
                      int32_t nz, uint32_t flags, tbfs_graph** out);
 int graph_create_cooc(const int32_t* I, const int32_t* J, int32_t n,
@@ -74,6 +110,8 @@ int run_bfs(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
             const tbfs_opts& o, tbfs_stats* st);
 int run_bfs_many(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots,
                  int count, const tbfs_opts& o, tbfs_stats* st);
+int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots,
+              const tbfs_opts& o, tbfs_stats* st);
 int ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma,
              uint8_t* ovf);
 }  // namespace tbfs

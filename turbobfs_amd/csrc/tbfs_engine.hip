@@ -28,38 +28,7 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 
 }  // namespace tbfs
 
-#define HIPCHK(call)                                               \
-  do {                                                             \
-    hipError_t e_ = (call);                                        \
-    if (e_ != hipSuccess) return hip_fail(e_, #call, __FILE__, __LINE__); \
-  } while (0)
-
 namespace {
-
-template <typename T>
-int dev_alloc(T** p, size_t count) {
-  *p = nullptr;
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
-  }
-  return TBFS_OK;
-}
-
-template <typename T>
-void dev_free(T*& p) {
-  if (p) (void)hipFree((void*)p);
-  p = nullptr;
-}
-
-int grid_for(int64_t items, int block, int cap) {
-  int64_t g = (items + block - 1) / block;
-  if (g < 1) g = 1;
-  if (g > cap) g = cap;
-  return (int)g;
-}
 
 // Host-side CSC validation (O(n + nz)).
 int validate_csc_host(const int32_t* CP, const int32_t* IC, int32_t n,
@@ -87,6 +56,7 @@ void graph_free(tbfs_graph* g) {
   if (g->I != g->IC) dev_free(g->I);
   if (g->bigo != g->bigc) dev_free(g->bigo);
   dev_free(g->skip);
+  dev_free(g->bsl);
   dev_free(g->bigc);
   dev_free(g->CP);
   dev_free(g->IC);
@@ -370,6 +340,7 @@ void ctx_free(tbfs_ctx* c) {
   dev_free(c->stats);
   dev_free(c->loop);
   dev_free(c->cset);
+  dev_free(c->roots_d);
   if (c->cset_h) (void)hipHostFree(c->cset_h);
   if (c->loop_h) (void)hipHostFree(c->loop_h);
   if (c->ctr_h) (void)hipHostFree(c->ctr_h);
@@ -515,8 +486,10 @@ static int pick_loop(const tbfs_ctx* c, const tbfs_opts& o) {
 
 static PersistParams make_params(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
                                  const tbfs_opts& o, RunStats* stats_slot) {
-  PersistParams p;
+  PersistParams p{};
   p.g = GraphView{g->n, g->nz, g->CP, g->IC, g->RP, g->JR};
+  p.nroots = 0;
+  p.roots = nullptr;
   p.S = c->S;
   p.sigma = c->sigma;
   p.ovf = c->ovf;
@@ -594,13 +567,13 @@ static int enqueue_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
 
 // Chained loop for one BFS: init, then chunks of chain_k level kernels; the
 // host reads the loop state once per chunk.  Ends with the stream drained.
-static int run_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
-                       const tbfs_opts& o, RunStats* stats_slot) {
-  const PersistParams p = make_params(c, g, root, o, stats_slot);
+static int run_chained_p(tbfs_ctx* c, const tbfs_graph* g, const PersistParams& p,
+                         const tbfs_opts& o) {
   const int grid = chain_grid(c, g, o);
   const size_t fbl = chain_fbl(c, g, o);
   hipStream_t s = c->stream;
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
+  if (p.nroots > 0) HIPCHK(kern::chain_seed(s, p));
   int32_t d = 0;
   for (;;) {
     for (int i = 0; i < c->chain_k; ++i) HIPCHK(kern::chain_level(s, grid, p, d + i, fbl));
@@ -611,6 +584,11 @@ static int run_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
     if (c->loop_h->done) break;
   }
   return TBFS_OK;
+}
+
+static int run_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
+                       const tbfs_opts& o, RunStats* stats_slot) {
+  return run_chained_p(c, g, make_params(c, g, root, o, stats_slot), o);
 }
 
 static void fill_stats(tbfs_stats* st, const RunStats& rs, float ms) {
@@ -840,6 +818,50 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
   return TBFS_OK;
 }
 
+// One BFS whose depth 0 is every root in roots[0 .. nroots-1]: S = distance
+// to the nearest root, sigma = number of shortest paths from any root.  With
+// one root per connected component (block-diagonal packing of many graphs)
+// every component gets exactly its single-source result.  Chained loop.
+int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots,
+              const tbfs_opts& o_in, tbfs_stats* st) {
+  if (!c || !g || !roots || nroots < 1) return TBFS_ERR_INVALID;
+  tbfs_opts o = o_in;
+  if (o.mode != TBFS_MODE_AUTO && o.mode != TBFS_MODE_PUSH) return TBFS_ERR_UNSUPPORTED;
+  o.pull = TBFS_PULL_LB;
+  o.loop = TBFS_LOOP_CHAINED;
+  for (int i = 0; i < nroots; ++i) {
+    const int rc = check_run(c, g, roots[i], o);
+    if (rc) return rc;
+  }
+  int rc = ctx_reserve_stats(c, 1);
+  if (rc) return rc;
+  if (c->roots_cap < nroots) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dev_free(c->roots_d);
+    c->roots_cap = 0;
+    rc = dev_alloc(&c->roots_d, (size_t)nroots);
+    if (rc) return rc;
+    c->roots_cap = nroots;
+  }
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(c->roots_d, roots, (size_t)nroots * sizeof(int32_t),
+                        hipMemcpyHostToDevice, s));
+  PersistParams p = make_params(c, g, -1, o, c->stats);
+  p.nroots = nroots;
+  p.roots = c->roots_d;
+  HIPCHK(hipEventRecord(c->ev0, s));
+  rc = run_chained_p(c, g, p, o);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(c->ev1, s));
+  HIPCHK(hipMemcpyAsync(c->stats_h, c->stats, sizeof(RunStats), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_n = g->n;
+  if (st) fill_stats(st, c->stats_h[0], ms);
+  return TBFS_OK;
+}
+
 int run_bfs(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs_opts& o,
             tbfs_stats* st) {
   return run_bfs_many(c, g, &root, 1, o, st);
@@ -934,6 +956,13 @@ int tbfs_bfs_many(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots,
   tbfs_opts def;
   tbfs_opts_default(&def);
   return run_bfs_many(c, g, roots, count, o ? *o : def, st);
+}
+
+int tbfs_bfs_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots,
+                   int32_t nroots, const tbfs_opts* o, tbfs_stats* st) {
+  tbfs_opts def;
+  tbfs_opts_default(&def);
+  return run_multi(c, g, roots, nroots, o ? *o : def, st);
 }
 
 int tbfs_ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma,

@@ -133,7 +133,9 @@ struct PersistParams {
   int32_t nbigc;
   const uint64_t* bigo;
   int32_t nbigo;
-  int32_t root;
+  int32_t root;     // single root; -1 when the roots come from `roots`
+  int32_t nroots;   // > 0: multi-root BFS from roots[0 .. nroots-1] (chained loop)
+  const int32_t* roots;
   int32_t mode;     // TBFS_MODE_AUTO / _PUSH / _PULL_SC (all pulls are load-balanced)
   int32_t sigma_on;
   float alpha;
@@ -189,6 +191,7 @@ hipError_t persist(hipStream_t s, int grid, int block, const PersistParams& p);
 // Chained loop: one init kernel, then one kernel per level (level index d),
 // each deciding its own direction from the device-side counters.
 hipError_t chain_init(hipStream_t s, int grid, const PersistParams& p);
+hipError_t chain_seed(hipStream_t s, const PersistParams& p);
 // fbl_bytes > 0: 1024-thread workgroups staging the frontier bitmap
 // (fbl_bytes of dynamic LDS) for pull levels; 0: kLbBlock-thread workgroups.
 hipError_t chain_level(hipStream_t s, int grid, const PersistParams& p, int32_t d,

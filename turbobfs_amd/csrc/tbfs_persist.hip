@@ -466,7 +466,7 @@ __device__ __forceinline__ void init_state(const PersistParams& p, int64_t gtid,
   const GraphView& g = p.g;
   const int32_t n = g.n;
   const int64_t nwords = ((int64_t)n + 63) >> 6;
-  const int r = p.root;
+  const int r = p.nroots > 0 ? -1 : p.root;  // multi-root: seeded afterwards
   for (int64_t i = gtid; i < n; i += gstride) {
     p.S[i] = (i == r) ? 0 : -1;
     p.sigma[i] = (i == r) ? 1ull : 0ull;
@@ -480,7 +480,7 @@ __device__ __forceinline__ void init_state(const PersistParams& p, int64_t gtid,
   }
   if (gtid < 3 * kStripes) {
     CtrStripe* c = &p.cset[gtid / kStripes].s[gtid % kStripes];
-    const bool root_slot = gtid == 0;
+    const bool root_slot = gtid == 0 && r >= 0;
     c->qm = root_slot ? ((1ull << 32) + outdeg(g, r)) : 0ull;
     c->min = root_slot ? indeg(g, r) : 0ull;
     c->ovf = 0;
@@ -616,6 +616,23 @@ __global__ __launch_bounds__(256) void k_chain_init(PersistParams p) {
   }
 }
 
+// Chained loop, several roots: depth 0 is every distinct root (runs after
+// k_chain_init, which seeds none when p.nroots > 0).
+__global__ __launch_bounds__(256) void k_chain_seed(PersistParams p) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= p.nroots) return;
+  const int r = p.roots[j];
+  const uint64_t bit = 1ull << (r & 63);
+  const unsigned long long old =
+      atomicOr((unsigned long long*)&p.fb[0][r >> 6], (unsigned long long)bit);
+  if (old & bit) return;  // repeated root
+  p.S[r] = 0;
+  p.sigma[r] = 1ull;
+  CtrStripe* c = &p.cset[0].s[j & (kStripes - 1)];
+  atomicAdd(&c->qm, (1ull << 32) + (unsigned long long)outdeg(p.g, r));
+  atomicAdd(&c->min, (unsigned long long)indeg(p.g, r));
+}
+
 // Chained loop: level d.  Reads slot d & 1 and the counters of depth d (both
 // final: written by earlier kernels), writes slot (d + 1) & 1.  FBL: pull
 // levels stage the frontier bitmap in dynamic LDS (one workgroup per CU).
@@ -696,6 +713,11 @@ hipError_t persist(hipStream_t s, int grid, int block, const PersistParams& p) {
 
 hipError_t chain_init(hipStream_t s, int grid, const PersistParams& p) {
   hipLaunchKernelGGL(k_chain_init, dim3(grid), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t chain_seed(hipStream_t s, const PersistParams& p) {
+  hipLaunchKernelGGL(k_chain_seed, dim3((p.nroots + 255) / 256), dim3(256), 0, s, p);
   return hipGetLastError();
 }
 
