@@ -83,8 +83,12 @@ typedef struct tbfs_opts {
   int32_t fb_lds; /* chained loop: 0 stages the frontier bitmap in LDS for
                      pull levels when it fits (n <= ~1.1M), -1 never */
   float batch_alpha; /* batches: expand top-down while the frontier's
-                        out-degree sum * batch_alpha < nz * words (0: 4) */
-  int32_t reserved[5];
+                        out-degree sum * batch_alpha < nz * words (0: 32) */
+  int32_t batch_count_bits; /* batches: 0 = 32-bit counts, again with 64-bit
+                               ones if some count reaches 2^32; 64 = 64-bit
+                               counts only; 32 = 32-bit only (a count that
+                               reaches 2^32 is then flagged as overflowed) */
+  int32_t reserved[4];
 } tbfs_opts;
 
 typedef struct tbfs_stats {
@@ -204,8 +208,10 @@ void *tbfs_batch_stream(tbfs_batch *b);
  * kernel, 0: default) and opts->batch_alpha are used.  st (may be NULL)
  * receives one entry for the whole batch: levels = max over sources of their
  * level count, push_levels / pull_levels = levels expanded top-down /
- * bottom-up, reached and edges summed over sources, ms = device time of the
- * batch. */
+ * bottom-up, pad = count width of the final run (32 or 64), reached and
+ * edges summed over sources, ovf64 = -1 if some count reached 2^64 (or 2^32
+ * with batch_count_bits = 32), ms = device time of the batch (both runs when
+ * the counts had to be widened). */
 int tbfs_batch_run(tbfs_batch *b, const tbfs_graph *g, const int32_t *roots,
                    int32_t count, const tbfs_opts *opts, tbfs_stats *st);
 

@@ -116,10 +116,44 @@ def test_batch_star_and_path(gpu):
 
 
 def test_batch_grid_counts_past_2_64(gpu):
+    # 32-bit counts overflow first: the batch runs again with 64-bit counts
     g = graphs.grid(48, 48, 0.0, seed=0, relabel=False)
     roots = [0, 47, 48 * 47, 48 * 48 - 1, 1000, 5] * 20
     st = _check_batch(g, roots)
     assert st["ovf64"] != 0
+    assert st["count_bits"] == 64
+
+
+def test_batch_count_widths(gpu):
+    """Forced 64-bit counts give the same results as the default; forced
+    32-bit counts give every count modulo 2^32 with the flag raised exactly
+    where the count reaches 2^32."""
+    g = graphs.rmat(13, 16, 0.57, 0.19, 0.19, seed=6, relabel=True)
+    roots = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 80, seed=6)
+    st = _check_batch(g, roots)
+    assert st["count_bits"] == 32
+    b = tb.Batch(g.n, 80)
+    gr = tb.Graph(g.CP, g.IC, symmetric=True)
+    st64 = b.run(gr, roots, tb.default_opts(batch_count_bits=64))
+    assert st64["count_bits"] == 64
+    S64, sig64, ovf64 = b.read()
+    b.run(gr, roots, tb.default_opts(batch_count_bits=32))
+    S32, sig32, ovf32 = b.read()
+    assert np.array_equal(S64, S32) and np.array_equal(sig64, sig32)
+    eS, esig, eovf, _, _ = _expected(g, roots)
+    assert np.array_equal(S64, eS) and np.array_equal(sig64, esig)
+
+    grid = graphs.grid(40, 40, 0.0, seed=0, relabel=False)
+    r = np.array([0, 39, 40 * 40 - 1], np.int32)
+    bg = tb.Batch(grid.n, 64)
+    st = bg.run(tb.Graph(grid.CP, grid.IC, symmetric=True), r,
+                tb.default_opts(batch_count_bits=32))
+    assert st["count_bits"] == 32 and st["ovf64"] != 0
+    S, sig, ovf = bg.read()
+    eS, esig, eovf, _, _ = _expected(grid, r)
+    assert np.array_equal(S, eS)
+    assert np.array_equal(sig, esig & np.uint64(0xFFFFFFFF))
+    assert np.array_equal(ovf, (eovf.astype(bool) | (esig >= np.uint64(2**32))).astype(np.uint8))
 
 
 def test_batch_perturbed_grid(gpu):

@@ -57,7 +57,8 @@ class Opts(C.Structure):
         ("pull", C.c_int32),
         ("fb_lds", C.c_int32),
         ("batch_alpha", C.c_float),
-        ("reserved", C.c_int32 * 5),
+        ("batch_count_bits", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 

@@ -258,7 +258,9 @@ class Batch:
               "tbfs_batch_run")
         self._n = g.n
         self._count = int(roots.shape[0])
-        return {f: getattr(st, f) for f, _ in Stats._fields_ if f != "pad"}
+        out = {f: getattr(st, f) for f, _ in Stats._fields_ if f != "pad"}
+        out["count_bits"] = int(st.pad)
+        return out
 
     def read(self, first: int = 0, count: int | None = None):
         """(S, sigma, ovf) of sources first .. first+count-1, shape (count, n)."""

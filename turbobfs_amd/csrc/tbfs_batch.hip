@@ -3,18 +3,22 @@
 // A batch of up to B = 64*W sources (W = 1, 2, 4, 8 or 16) runs as one
 // level-synchronous BFS whose per-vertex state is W 64-bit words; bit j of
 // word w belongs to source 64*w + j.  Layout in HBM (batch object):
-//   F[3]    uint64[n][W]  frontier words of depth d (current), d+1 (next) and
+//   F[3]    uint64[W][n]  frontier words of depth d (current), d+1 (next) and
 //                         d-1 (being cleared), rotating with d
-//   seen[2] uint64[n][W]  OR of the frontiers of depths < d (slot d & 1); bits
+//   seen[2] uint64[W][n]  OR of the frontiers of depths < d (slot d & 1); bits
 //                         of unused sources (index >= count) start set
-//   ovfw    uint64[n][W]  per-source "path count >= 2^64" bits
-//   S       int32[n][B]   per-source depth   } valid where the pair's seen
-//   sigma   uint64[n][B]  per-source count   } bit is set (never-reached
+//   ovfw    uint64[W][n]  per-source "path count >= 2^bits(CT)" bits
+// Counts are uint32 first (CT = uint32_t: half the bytes of every gather);
+// a batch in which some count reaches 2^32 runs again with CT = uint64_t.
+//   S       int32[W][n][64] per-source depth } valid where the pair's seen
+//   sigma   CT[W][n][64]    per-source count } bit is set (never-reached
 //                                              pairs are not written; reads
 //                                              give -1 / 0 for them)
-// S and sigma keep the source index fastest: the 64 lanes of a wavefront are
-// the 64 sources of one word, so a vertex's counts for a word are 512
-// contiguous bytes and every gather of a parent's counts is coalesced.
+// Word-major throughout: a word's frontier array is n * 8 bytes (it stays in
+// L2 while the waves work on that word), and S / sigma keep the 64 sources of
+// a word fastest: the 64 lanes of a wavefront are the 64 sources of one word,
+// so a vertex's counts for a word are 256 / 512 contiguous bytes and every
+// gather of a parent's counts is coalesced.
 //
 // Every level is a pull over static slices of the in-lists (kBatchSlice
 // entries at most).  A wavefront takes one (word w, slice of vertex v) item:
@@ -114,7 +118,7 @@ struct BatchParams {
   uint64_t* F[3];
   uint64_t* seen[2];
   uint64_t* ovfw;
-  uint64_t* sigma;
+  void* sigma;    // CT[n][B]: uint32_t or uint64_t counts
   int32_t* S;
   BCtrSet* cset;  // [3]
   BLoop* loop;    // [2]
@@ -134,14 +138,20 @@ __device__ __forceinline__ uint64_t word_valid(int nsrc, int w) {
   return k >= 64 ? ~0ull : (k <= 0 ? 0ull : ((1ull << k) - 1ull));
 }
 
-template <int W>
+__device__ __forceinline__ uint32_t atomic_add_ct(uint32_t* a, uint32_t x) {
+  return atomicAdd((unsigned int*)a, (unsigned int)x);
+}
+__device__ __forceinline__ uint64_t atomic_add_ct(uint64_t* a, uint64_t x) {
+  return atomicAdd((unsigned long long*)a, (unsigned long long)x);
+}
+
+template <int W, typename CT>
 __global__ __launch_bounds__(256) void k_batch_init(BatchParams p) {
-  constexpr int B = 64 * W;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
   const int64_t nw = (int64_t)p.n * W;
   for (int64_t i = gtid; i < nw; i += gstride) {
-    const uint64_t pad = ~word_valid(p.nsrc, (int)(i & (W - 1)));
+    const uint64_t pad = ~word_valid(p.nsrc, (int)(i / p.n));
     p.F[0][i] = 0ull;
     p.F[1][i] = 0ull;
     p.F[2][i] = 0ull;
@@ -151,11 +161,12 @@ __global__ __launch_bounds__(256) void k_batch_init(BatchParams p) {
   }
   // rows of split in-lists receive atomic adds: zero them
   for (int64_t t = gtid; t < (int64_t)p.nslices * W; t += gstride) {
-    const uint64_t e = p.slices[t / W];
+    const int w = (int)(t / p.nslices);
+    const uint64_t e = p.slices[t - (int64_t)w * p.nslices];
     const int v = (int)(uint32_t)e;
     if ((e >> 32) != 0 || p.CP[v + 1] - p.CP[v] <= kBatchSlice) continue;
-    uint64_t* row = p.sigma + (int64_t)v * B + 64 * (t % W);
-    for (int j = 0; j < 64; ++j) row[j] = 0ull;
+    CT* row = (CT*)p.sigma + ((int64_t)w * p.n + v) * 64;
+    for (int j = 0; j < 64; ++j) row[j] = 0;
   }
   const int nctr = (int)(3 * sizeof(BCtrSet) / sizeof(unsigned long long));
   unsigned long long* cw = reinterpret_cast<unsigned long long*>(p.cset);
@@ -170,17 +181,17 @@ __global__ __launch_bounds__(256) void k_batch_init(BatchParams p) {
 }
 
 // Depth-0 frontier: one bit per source at its root.
-template <int W>
+template <int W, typename CT>
 __global__ __launch_bounds__(256) void k_batch_seed(BatchParams p) {
-  constexpr int B = 64 * W;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= p.nsrc) return;
   const int r = p.roots[j];
   const uint64_t bit = 1ull << (j & 63);
   const unsigned long long old =
-      atomicOr((unsigned long long*)&p.F[0][(int64_t)r * W + (j >> 6)], (unsigned long long)bit);
-  p.S[(int64_t)r * B + j] = 0;
-  p.sigma[(int64_t)r * B + j] = 1ull;
+      atomicOr((unsigned long long*)&p.F[0][(int64_t)(j >> 6) * p.n + r], (unsigned long long)bit);
+  const int64_t ix = ((int64_t)(j >> 6) * p.n + r) * 64 + (j & 63);
+  p.S[ix] = 0;
+  ((CT*)p.sigma)[ix] = 1;
   BStripe& c = p.cset[0].s[j & (kStripes - 1)];
   if (old == 0) {
     atomicAdd(&c.q, 1ull);
@@ -217,11 +228,10 @@ __device__ __forceinline__ void flush_acc(BStripe& c, BAcc a) {
 // (one per lane), then expands the active ones (some source of the word
 // still unseen at v) in turn: it discovers those sources (next frontier
 // word) and writes their depths and counts.
-template <int W>
+template <int W, typename CT>
 __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, const uint64_t* Fc,
                                            uint64_t* Fn, const uint64_t* sd, bool pov,
                                            int64_t gtid, int64_t gstride, BAcc& a) {
-  constexpr int B = 64 * W;
   const int ln = lane();
   const int64_t nwaves = gstride >> 6;
   const int64_t items = (int64_t)p.nslices * W;
@@ -237,7 +247,7 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
       k0 = beg0 + (int)(e >> 32) * kBatchSlice;
       k1 = min(end0, k0 + kBatchSlice);
       deg = end0 - beg0;
-      const int64_t iw = (int64_t)v * W + w;
+      const int64_t iw = (int64_t)w * p.n + v;
       tgt = ~(sd[iw] | Fc[iw]);
     }
     uint64_t live = __ballot(tgt != 0);
@@ -250,8 +260,10 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
       const int kb = __builtin_amdgcn_readlane(k1, src);
       const int dg = __builtin_amdgcn_readlane(deg, src);
       const uint64_t un = rl64(tgt, src);
-      const uint64_t* sgw = p.sigma + 64 * wi + ln;
-      uint64_t acc = 0;
+      const CT* sgw = (const CT*)p.sigma + (int64_t)wi * p.n * 64 + ln;
+      const uint64_t* Fcw = Fc + (int64_t)wi * p.n;
+      const uint64_t* ovw = p.ovfw + (int64_t)wi * p.n;
+      CT acc = 0;
       bool hit = false, ovb = false;
       for (int c0 = ka; c0 < kb; c0 += 64) {
         const int k = c0 + ln;
@@ -259,32 +271,32 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
         uint64_t mm = 0;
         if (k < kb) {
           u = p.IC[k];
-          mm = Fc[(int64_t)u * W + wi] & un;  // sources for which u -> v is a parent edge
+          mm = Fcw[u] & un;  // sources for which u -> v is a parent edge
         }
         uint64_t bal = __ballot(mm != 0);
         // parent edges in groups of G: all count loads of a group are issued
         // before any is added
         constexpr int G = 16;
         while (bal) {
-          uint64_t x[G];
+          CT x[G];
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            x[g] = 0ull;
+            x[g] = 0;
             if (bal) {
               const int pe = __builtin_ctzll(bal);
               bal &= bal - 1;
               const int ue = __builtin_amdgcn_readlane(u, pe);
               const uint64_t me = rl64(mm, pe);
               if ((me >> ln) & 1ull) {
-                x[g] = sgw[(int64_t)ue * B];
+                x[g] = sgw[(int64_t)ue * 64];
                 hit = true;
-                if (pov && ((p.ovfw[(int64_t)ue * W + wi] >> ln) & 1ull)) ovb = true;
+                if (pov && ((ovw[ue] >> ln) & 1ull)) ovb = true;
               }
             }
           }
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            const uint64_t y = acc + x[g];
+            const CT y = acc + x[g];
             if (y < acc) ovb = true;
             acc = y;
           }
@@ -293,12 +305,12 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
       const uint64_t nb = __ballot(hit);
       if (nb == 0) continue;
       uint64_t ob = __ballot(ovb);
-      const int64_t iw = (int64_t)vi * W + wi;
-      const int64_t ix = (int64_t)vi * B + 64 * wi + ln;
+      const int64_t iw = (int64_t)wi * p.n + vi;
+      const int64_t ix = iw * 64 + ln;
       if (dg <= kBatchSlice) {  // the only slice of v: plain stores
         if (hit) {
           p.S[ix] = d + 1;
-          p.sigma[ix] = acc;
+          ((CT*)p.sigma)[ix] = acc;
         }
         if (ln == 0) {
           Fn[iw] = nb;
@@ -313,9 +325,8 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
       } else {  // one of several slices: partial sums into a zeroed row
         bool cy = false;
         if (hit) {
-          const unsigned long long old =
-              atomicAdd((unsigned long long*)&p.sigma[ix], (unsigned long long)acc);
-          cy = old + acc < old;
+          const CT old = atomic_add_ct((CT*)p.sigma + ix, acc);
+          cy = (CT)(old + acc) < old;
           p.S[ix] = d + 1;
         }
         ob |= __ballot(cy);
@@ -347,12 +358,11 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
 //     now marked at v (exactly the shortest-path edges u -> v into depth
 //     d + 1) get u's count added atomically, with carry and inherited
 //     overflow flags.
-template <int W, bool ADD>
+template <int W, bool ADD, typename CT>
 __device__ __forceinline__ void push_edge(const BatchParams& p, const uint64_t* Fc, uint64_t* Fn,
                                           const uint64_t* sd, int32_t d, int u, int v, int w,
                                           uint64_t f, bool pov, BAcc& a) {
-  constexpr int B = 64 * W;
-  const int64_t iv = (int64_t)v * W + w;
+  const int64_t iv = (int64_t)w * p.n + v;
   if (!ADD) {
     const uint64_t m = f & ~(sd[iv] | Fc[iv]);
     if (!m) return;
@@ -369,22 +379,22 @@ __device__ __forceinline__ void push_edge(const BatchParams& p, const uint64_t* 
     while (newly) {
       const int sb = __builtin_ctzll(newly);
       newly &= newly - 1;
-      const int64_t ix = (int64_t)v * B + 64 * w + sb;
-      p.sigma[ix] = 0ull;
+      const int64_t ix = iv * 64 + sb;
+      ((CT*)p.sigma)[ix] = 0;
       p.S[ix] = d + 1;
     }
   } else {
     uint64_t m = f & Fn[iv];
     if (!m) return;
-    const uint64_t po = pov ? p.ovfw[(int64_t)u * W + w] : 0ull;
+    const int64_t iu = (int64_t)w * p.n + u;
+    const uint64_t po = pov ? p.ovfw[iu] : 0ull;
     uint64_t ob = 0;
     while (m) {
       const int sb = __builtin_ctzll(m);
       m &= m - 1;
-      const uint64_t x = p.sigma[(int64_t)u * B + 64 * w + sb];
-      const unsigned long long old =
-          atomicAdd((unsigned long long*)&p.sigma[(int64_t)v * B + 64 * w + sb], (unsigned long long)x);
-      if (old + x < old || ((po >> sb) & 1ull)) ob |= 1ull << sb;
+      const CT x = ((const CT*)p.sigma)[iu * 64 + sb];
+      const CT old = atomic_add_ct((CT*)p.sigma + iv * 64 + sb, x);
+      if ((CT)(old + x) < old || ((po >> sb) & 1ull)) ob |= 1ull << sb;
     }
     if (ob) {
       atomicOr((unsigned long long*)&p.ovfw[iv], (unsigned long long)ob);
@@ -405,7 +415,7 @@ struct PushLds {
 // Lists up to kBigIn entries are split over the lanes of a wavefront by edge
 // count (prefix scan + owner search); longer lists go by their static
 // kPullChunk slices, one (slice, word) item per wavefront.
-template <int W, bool ADD>
+template <int W, bool ADD, typename CT>
 __device__ __forceinline__ void batch_push(const BatchParams& p, const uint64_t* Fc,
                                            uint64_t* Fn, const uint64_t* sd, int32_t d, bool pov,
                                            int64_t gtid, int64_t gstride, PushLds& L, BAcc& a) {
@@ -419,7 +429,7 @@ __device__ __forceinline__ void batch_push(const BatchParams& p, const uint64_t*
     if (i < nW) {
       f = Fc[i];
       if (f) {
-        const int u = (int)(i / W);
+        const int u = (int)(i % p.n);
         rb = p.RP[u];
         od = p.RP[u + 1] - rb;
         if (od > kBigIn) od = 0;  // served by the static slices below
@@ -430,8 +440,8 @@ __device__ __forceinline__ void batch_push(const BatchParams& p, const uint64_t*
     if (total == 0) continue;
     L.excl[ln] = (int)incl - od;
     L.beg[ln] = rb;
-    L.u[ln] = (int)(i / W);
-    L.w[ln] = (int)(i % W);
+    L.u[ln] = (int)(i % p.n);
+    L.w[ln] = (int)(i / p.n);
     L.f[ln] = f;
     wave_lds_sync();
     for (int e0 = 0; e0 < total; e0 += 64) {
@@ -439,27 +449,27 @@ __device__ __forceinline__ void batch_push(const BatchParams& p, const uint64_t*
       if (e < total) {
         const int o = owner_of(L.excl, e);
         const int v = p.JR[L.beg[o] + (e - L.excl[o])];
-        push_edge<W, ADD>(p, Fc, Fn, sd, d, L.u[o], v, L.w[o], L.f[o], pov, a);
+        push_edge<W, ADD, CT>(p, Fc, Fn, sd, d, L.u[o], v, L.w[o], L.f[o], pov, a);
       }
     }
     wave_lds_sync();
   }
   const int64_t hubs = (int64_t)p.nbigo * W;
   for (int64_t t = gtid >> 6; t < hubs; t += nwaves) {
-    const int w = (int)(t % W);
-    const uint64_t e = p.bigo[t / W];
+    const int w = (int)(t / p.nbigo);
+    const uint64_t e = p.bigo[t - (int64_t)w * p.nbigo];
     const int u = (int)(uint32_t)e;
-    const uint64_t f = Fc[(int64_t)u * W + w];
+    const uint64_t f = Fc[(int64_t)w * p.n + u];
     if (!f) continue;
     const int kb = p.RP[u] + (int)(e >> 32) * kPullChunk;
     const int ke = min(p.RP[u + 1], kb + kPullChunk);
     for (int k = kb + ln; k < ke; k += 64)
-      push_edge<W, ADD>(p, Fc, Fn, sd, d, u, p.JR[k], w, f, pov, a);
+      push_edge<W, ADD, CT>(p, Fc, Fn, sd, d, u, p.JR[k], w, f, pov, a);
   }
 }
 
 // Level d: expand the frontier of depth d into depth d + 1.
-template <int W>
+template <int W, typename CT>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int32_t d) {
   __shared__ PushLds lds[kBatchBlock / 64];
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -539,16 +549,16 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int3
 
   BAcc a;
   if (push)
-    batch_push<W, false>(p, Fc, Fn, sd, d, false, gtid, gstride, lds[threadIdx.x >> 6], a);
+    batch_push<W, false, CT>(p, Fc, Fn, sd, d, false, gtid, gstride, lds[threadIdx.x >> 6], a);
   else
-    batch_pull<W>(p, d, Fc, Fn, sd, st.ovf_any != 0, gtid, gstride, a);
+    batch_pull<W, CT>(p, d, Fc, Fn, sd, st.ovf_any != 0, gtid, gstride, a);
   flush_acc(p.cset[(d + 1) % 3].s[blockIdx.x & (kStripes - 1)], a);
 }
 
 // After a push level d: every edge of the frontier adds its count into the
 // pairs the level marked (their counts were zeroed by the marking lanes).
 // Nothing to do after a pull level.
-template <int W>
+template <int W, typename CT>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_scatter(BatchParams p, int32_t d) {
   __shared__ PushLds lds[kBatchBlock / 64];
   const BLoop nx = p.loop[(d + 1) & 1];
@@ -557,7 +567,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_scatter(BatchParams p, in
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
   // seen[d & 1] and F[d % 3] are still those of level d (level d + 1 has not run)
   BAcc a;
-  batch_push<W, true>(p, p.F[d % 3], p.F[(d + 1) % 3], p.seen[d & 1], d, nx.ovf_any != 0, gtid,
+  batch_push<W, true, CT>(p, p.F[d % 3], p.F[(d + 1) % 3], p.seen[d & 1], d, nx.ovf_any != 0, gtid,
                       gstride, lds[threadIdx.x >> 6], a);
   flush_acc(p.cset[(d + 1) % 3].s[blockIdx.x & (kStripes - 1)], a);
 }
@@ -566,10 +576,11 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_scatter(BatchParams p, in
 // (count <= 64) into So[count][n], sgo[count][n], ovo[count][n].  Pairs whose
 // bit is clear in `seen` (never reached) read S = -1, sigma = 0: the level
 // kernels never write them.
+template <typename CT>
 __global__ __launch_bounds__(256) void k_batch_extract(
-    const int32_t* __restrict__ S, const uint64_t* __restrict__ sigma,
+    const int32_t* __restrict__ S, const CT* __restrict__ sigma,
     const uint64_t* __restrict__ ovfw, const uint64_t* __restrict__ seen, int32_t n,
-    int32_t B, int32_t W, int32_t first, int32_t count, int32_t* So, uint64_t* sgo,
+    int32_t first, int32_t count, int32_t* So, uint64_t* sgo,
     uint8_t* ovo) {
   __shared__ int32_t ts[64][65];
   __shared__ uint64_t tg[64][65];
@@ -580,10 +591,10 @@ __global__ __launch_bounds__(256) void k_batch_extract(
     const int64_t v = v0 + r;
     if (v < n && tx < count) {
       const int j = first + tx;
-      const int64_t iw = v * W + (j >> 6);
+      const int64_t iw = (int64_t)(j >> 6) * n + v;
       const bool reached = (seen[iw] >> (j & 63)) & 1ull;
-      ts[r][tx] = reached ? S[v * B + j] : -1;
-      tg[r][tx] = reached ? sigma[v * B + j] : 0ull;
+      ts[r][tx] = reached ? S[iw * 64 + (j & 63)] : -1;
+      tg[r][tx] = reached ? (uint64_t)sigma[iw * 64 + (j & 63)] : 0ull;
       to[r][tx] = reached ? (uint8_t)((ovfw[iw] >> (j & 63)) & 1ull) : (uint8_t)0;
     }
   }
@@ -614,7 +625,7 @@ struct tbfs_batch {
   uint64_t* F[3] = {nullptr, nullptr, nullptr};
   uint64_t* seen[2] = {nullptr, nullptr};
   uint64_t* ovfw = nullptr;
-  uint64_t* sigma = nullptr;
+  uint64_t* sigma = nullptr;  // viewed as uint32_t[n][B] by 32-bit runs
   int32_t* S = nullptr;
   BCtrSet* cset = nullptr;
   BLoop* loop = nullptr;
@@ -629,6 +640,7 @@ struct tbfs_batch {
   int32_t last_n = 0;
   int32_t last_count = 0;
   int32_t last_w = 0;
+  int last_ct = 8;                       // bytes per count of the last batch
   const uint64_t* seen_final = nullptr;  // reached bits of the last batch
 };
 
@@ -733,20 +745,20 @@ int graph_ensure_batch_slices(tbfs_graph* g) {
   return TBFS_OK;
 }
 
-template <int W>
+template <int W, typename CT>
 int enqueue_levels(tbfs_batch* b, const BatchParams& p, int d0, int count, int grid) {
   for (int d = d0; d < d0 + count; ++d) {
-    hipLaunchKernelGGL(k_batch_level<W>, dim3(grid), dim3(kBatchBlock), 0, b->stream, p, d);
+    hipLaunchKernelGGL((k_batch_level<W, CT>), dim3(grid), dim3(kBatchBlock), 0, b->stream, p, d);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_batch_scatter<W>, dim3(grid), dim3(kBatchBlock), 0, b->stream, p, d);
+    hipLaunchKernelGGL((k_batch_scatter<W, CT>), dim3(grid), dim3(kBatchBlock), 0, b->stream, p, d);
     HIPCHK(hipGetLastError());
   }
   return TBFS_OK;
 }
 
-template <int W>
+template <int W, typename CT>
 int batch_run_w(tbfs_batch* b, const tbfs_graph* g, int32_t count, const tbfs_opts& o,
-                tbfs_stats* st) {
+                RunStats* rs_out, float* ms_out) {
   BatchParams p{};
   p.n = g->n;
   p.nz = g->nz;
@@ -770,7 +782,7 @@ int batch_run_w(tbfs_batch* b, const tbfs_graph* g, int32_t count, const tbfs_op
   p.stats = b->stats;
   p.roots = b->roots;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_level<W>, kBatchBlock, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_level<W, CT>, kBatchBlock, 0) !=
           hipSuccess || per_cu < 1) {
     (void)hipGetLastError();
     per_cu = 4;
@@ -779,12 +791,12 @@ int batch_run_w(tbfs_batch* b, const tbfs_graph* g, int32_t count, const tbfs_op
   hipStream_t s = b->stream;
   HIPCHK(hipEventRecord(b->ev0, s));
   const int igrid = grid_for((int64_t)g->n * W, 256, std::max(1, b->cus) * 8);
-  hipLaunchKernelGGL(k_batch_init<W>, dim3(igrid), dim3(256), 0, s, p);
+  hipLaunchKernelGGL((k_batch_init<W, CT>), dim3(igrid), dim3(256), 0, s, p);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_batch_seed<W>, dim3((count + 255) / 256), dim3(256), 0, s, p);
+  hipLaunchKernelGGL((k_batch_seed<W, CT>), dim3((count + 255) / 256), dim3(256), 0, s, p);
   HIPCHK(hipGetLastError());
   int d = 0;
-  int rc = enqueue_levels<W>(b, p, d, b->chain_k, grid);
+  int rc = enqueue_levels<W, CT>(b, p, d, b->chain_k, grid);
   if (rc) return rc;
   d += b->chain_k;
   for (;;) {
@@ -795,32 +807,34 @@ int batch_run_w(tbfs_batch* b, const tbfs_graph* g, int32_t count, const tbfs_op
       std::fprintf(stderr, "turbobfs: batched BFS did not finish within %d levels\n", d);
       return TBFS_ERR_HIP;
     }
-    rc = enqueue_levels<W>(b, p, d, b->chain_k, grid);
+    rc = enqueue_levels<W, CT>(b, p, d, b->chain_k, grid);
     if (rc) return rc;
     d += b->chain_k;
   }
   HIPCHK(hipEventRecord(b->ev1, s));
   HIPCHK(hipEventSynchronize(b->ev1));
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-  const RunStats& rs = *b->stats_h;
+  HIPCHK(hipEventElapsedTime(ms_out, b->ev0, b->ev1));
+  *rs_out = *b->stats_h;
   // one spare level beyond the deepest batch so far
-  b->chain_k = std::max(b->chain_k, std::min(4096, rs.levels + 2));
+  b->chain_k = std::max(b->chain_k, std::min(4096, rs_out->levels + 2));
   b->last_n = g->n;
   b->last_count = count;
   b->last_w = W;
-  b->seen_final = b->seen[rs.levels & 1];  // OR of every frontier
-  if (st) {
-    st->levels = rs.levels;
-    st->push_levels = rs.push_levels;
-    st->pull_levels = rs.pull_levels;
-    st->pad = 0;
-    st->reached = (int64_t)rs.reached;
-    st->edges = (int64_t)rs.edges;
-    st->ovf64 = rs.ovf_any ? -1 : 0;
-    st->ms = (double)ms;
-  }
+  b->last_ct = (int)sizeof(CT);
+  b->seen_final = b->seen[rs_out->levels & 1];  // OR of every frontier
   return TBFS_OK;
+}
+
+template <typename CT>
+int batch_dispatch(tbfs_batch* b, const tbfs_graph* g, int32_t count, const tbfs_opts& o,
+                   RunStats* rs, float* ms) {
+  switch (words_for(count)) {
+    case 1: return batch_run_w<1, CT>(b, g, count, o, rs, ms);
+    case 2: return batch_run_w<2, CT>(b, g, count, o, rs, ms);
+    case 4: return batch_run_w<4, CT>(b, g, count, o, rs, ms);
+    case 8: return batch_run_w<8, CT>(b, g, count, o, rs, ms);
+    default: return batch_run_w<16, CT>(b, g, count, o, rs, ms);
+  }
 }
 
 int batch_run(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32_t count,
@@ -833,17 +847,37 @@ int batch_run(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32_t
     if (roots[i] < 0 || roots[i] >= g->n) return TBFS_ERR_INVALID;
   tbfs_opts o;
   if (o_in) o = *o_in; else tbfs_opts_default(&o);
+  if (o.batch_count_bits != 0 && o.batch_count_bits != 32 && o.batch_count_bits != 64)
+    return TBFS_ERR_INVALID;
   int rc = graph_ensure_batch_slices(g);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(b->roots, roots, (size_t)count * sizeof(int32_t),
                         hipMemcpyHostToDevice, b->stream));
-  switch (words_for(count)) {
-    case 1: return batch_run_w<1>(b, g, count, o, st);
-    case 2: return batch_run_w<2>(b, g, count, o, st);
-    case 4: return batch_run_w<4>(b, g, count, o, st);
-    case 8: return batch_run_w<8>(b, g, count, o, st);
-    default: return batch_run_w<16>(b, g, count, o, st);
+  // 32-bit counts first (half the bytes of every count gather); if some count
+  // reaches 2^32 the batch runs again with 64-bit counts
+  RunStats rs{};
+  float ms = 0.f, ms64 = 0.f;
+  bool wide = o.batch_count_bits == 64;
+  if (!wide) {
+    rc = batch_dispatch<uint32_t>(b, g, count, o, &rs, &ms);
+    if (rc) return rc;
+    wide = rs.ovf_any != 0 && o.batch_count_bits != 32;
   }
+  if (wide) {
+    rc = batch_dispatch<uint64_t>(b, g, count, o, &rs, &ms64);
+    if (rc) return rc;
+  }
+  if (st) {
+    st->levels = rs.levels;
+    st->push_levels = rs.push_levels;
+    st->pull_levels = rs.pull_levels;
+    st->pad = b->last_ct * 8;  // count width of the final run
+    st->reached = (int64_t)rs.reached;
+    st->edges = (int64_t)rs.edges;
+    st->ovf64 = rs.ovf_any ? -1 : 0;
+    st->ms = (double)(ms + ms64);
+  }
+  return TBFS_OK;
 }
 
 int batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* S,
@@ -858,13 +892,17 @@ int batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* 
     if (!rc) rc = dev_alloc(&b->xovf, (size_t)64 * b->n_cap);
     if (rc) return rc;
   }
-  const int W = b->last_w;
-  const int B = 64 * W;
   hipStream_t s = b->stream;
   for (int32_t f = 0; f < count; f += 64) {
     const int32_t c = std::min(64, count - f);
-    hipLaunchKernelGGL(k_batch_extract, dim3((n + 63) / 64), dim3(256), 0, s, b->S, b->sigma,
-                       b->ovfw, b->seen_final, n, B, W, first + f, c, b->xS, b->xsig, b->xovf);
+    if (b->last_ct == 4)
+      hipLaunchKernelGGL(k_batch_extract<uint32_t>, dim3((n + 63) / 64), dim3(256), 0, s, b->S,
+                         (const uint32_t*)b->sigma, b->ovfw, b->seen_final, n, first + f, c,
+                         b->xS, b->xsig, b->xovf);
+    else
+      hipLaunchKernelGGL(k_batch_extract<uint64_t>, dim3((n + 63) / 64), dim3(256), 0, s, b->S,
+                         (const uint64_t*)b->sigma, b->ovfw, b->seen_final, n, first + f, c,
+                         b->xS, b->xsig, b->xovf);
     HIPCHK(hipGetLastError());
     const size_t rows = (size_t)c * n;
     if (S) HIPCHK(hipMemcpyAsync(S + (size_t)f * n, b->xS, rows * sizeof(int32_t), hipMemcpyDeviceToHost, s));
