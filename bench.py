@@ -9,9 +9,12 @@ the number of in-list entries of the vertices reached (undirected edges
 counted once).  Every timed BFS computes depths S and exact uint64 path
 counts sigma.
 
-Prints ONE JSON line.  Also reports the roofline of the dominant kernel and a
+Prints ONE JSON line.  Also reports the roofline of the dominant kernel, a
 CPU baseline (the reference's own sequential bfs_seq_td_csc, compiled from
-its sources into oracle/_ref, else this project's C restatement).
+its sources into oracle/_ref, else this project's C restatement), and
+(unless --no-batch) the batched-sources throughput against the per-source
+loop (R-MAT scale 18, 1024 sources per batch) and a 512-graph block-diagonal
+multi-root BFS against a per-graph loop.
 """
 from __future__ import annotations
 
@@ -96,6 +99,81 @@ def cpu_baseline(g, sources, budget_s: float):
     }
 
 
+def bench_batched(tb, graphs, reps: int = 3):
+    """Batched sources vs the per-source loop on the same GPU: R-MAT scale 18
+    (ef16, a=.57 b=.19 c=.19, symmetrized, relabelled, seed 4), 1024 sources
+    of nonzero degree (seed 4) in one batch, 64 per frontier word."""
+    g = graphs.rmat(18, 16, 0.57, 0.19, 0.19, seed=4, relabel=True)
+    src = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 1024, seed=4)
+    gr = tb.Graph(g.CP, g.IC, symmetric=True)
+    b = tb.Batch(g.n, len(src))
+    b.run(gr, src)  # warm-up
+    runs = [b.run(gr, src) for _ in range(reps)]
+    ms = float(np.median([r["ms"] for r in runs]))
+    st = runs[0]
+    ctx = tb.Context(g.n)
+    ctx.run_many(gr, src[:64])  # warm-up
+    ctx.mark(0)
+    per = ctx.run_many(gr, src)
+    ctx.mark(1)
+    loop_ms = ctx.elapsed(0, 1)
+    batch_sps = len(src) / (ms * 1e-3)
+    loop_sps = len(src) / (loop_ms * 1e-3)
+    b.close()
+    ctx.close()
+    return {
+        "workload": "R-MAT scale 18 ef16 symmetrized relabelled seed 4, 1024 sources (seed 4) "
+                    "in one batch; every source's S and sigma exact",
+        "n": g.n, "nz": g.nz, "sources": len(src),
+        "sources_per_s": batch_sps,
+        "batch_ms": ms,
+        "gteps": st["edges"] / 2 / (ms * 1e-3) / 1e9,
+        "levels": st["levels"], "push_levels": st["push_levels"],
+        "pull_levels": st["pull_levels"], "count_bits": st["count_bits"],
+        "per_source_loop_sources_per_s": loop_sps,
+        "per_source_loop": "tbfs_bfs_many, chained direction-optimizing loop, same GPU",
+        "speedup_vs_per_source_loop": batch_sps / loop_sps,
+        "per_source_loop_gteps": sum(x["edges"] for x in per) / 2 / (loop_ms * 1e-3) / 1e9,
+    }
+
+
+def bench_multigraph(tb, graphs, reps: int = 3):
+    """512 uneven Watts-Strogatz graphs (n ~ U[1000, 64000], k=10, p=0.1) as
+    diagonal blocks of one CSC, one source each, in one multi-root BFS; against
+    a loop of single-source BFS over the same resident graphs."""
+    rng = np.random.default_rng(5)
+    parts = [graphs.watts_strogatz(int(rng.integers(1000, 64001)), 10, 0.1, seed=i)
+             for i in range(512)]
+    bd, off = graphs.block_diagonal(parts)
+    roots = np.array([off[i] + int(rng.integers(0, parts[i].n)) for i in range(len(parts))],
+                     np.int32)
+    gb = tb.Graph(bd.CP, bd.IC, symmetric=True)
+    ctx = tb.Context(bd.n)
+    ctx.run_multi(gb, roots)
+    runs = [ctx.run_multi(gb, roots) for _ in range(reps)]
+    ms = float(np.median([r["ms"] for r in runs]))
+    ctx.close()
+    grs = [tb.Graph(p.CP, p.IC, symmetric=True) for p in parts]
+    cs = tb.Context(max(p.n for p in parts))
+    for i in range(8):
+        cs.run(grs[i], int(roots[i] - off[i]))
+    t0 = time.perf_counter()
+    for i in range(len(parts)):
+        cs.run(grs[i], int(roots[i] - off[i]))
+    loop_ms = (time.perf_counter() - t0) * 1e3
+    cs.close()
+    return {
+        "workload": "512 Watts-Strogatz graphs n~U[1000,64000] k=10 p=0.1 block-diagonal, "
+                    "one source each",
+        "n": bd.n, "nz": bd.nz, "graphs": len(parts),
+        "multi_root_ms": ms, "graphs_per_s": len(parts) / (ms * 1e-3),
+        "levels": runs[0]["levels"],
+        "per_graph_loop_ms": loop_ms,
+        "per_graph_loop": "tbfs_bfs per graph, resident graphs, host wall time",
+        "speedup_vs_per_graph_loop": loop_ms / ms,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +191,8 @@ def main():
     ap.add_argument("--beta", type=float, default=0.0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-sources and multi-graph measurements")
     ap.add_argument("--calib-mib", type=int, default=512,
                     help="size of the plain copy kernel run first (byte-counter reference)")
     args = ap.parse_args()
@@ -216,6 +296,9 @@ def main():
                            args.mode, "whole BFS (all level kernels of one source)"),
         },
     }
+    if not args.no_batch:
+        line["batched"] = bench_batched(tb, graphs)
+        line["multi_graph"] = bench_multigraph(tb, graphs)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(g, sources, args.cpu_budget)
     print(json.dumps(line))

@@ -193,10 +193,10 @@ int tbfs_ctx_device_ptrs(tbfs_ctx *c, int32_t **S, uint64_t **sigma,
  * frontiers are bit-packed, 64 sources per 64-bit word, and every level is
  * one pass over the in-lists for all sources.  Each source's S and sigma are
  * exactly those of tbfs_bfs from that root.  A batch object owns its own HIP
- * stream and device state for graphs of up to n_cap vertices and up to b_cap
- * sources (b_cap <= 1024); device memory ~ n_cap * b_cap * 12 bytes (S and
- * sigma of every (vertex, source) pair) plus 56 bytes per vertex per 64
- * sources. */
+ * stream and device state sized for n_cap vertices times b_cap sources
+ * rounded up to a power-of-two number of 64-source words (b_cap <= 1024):
+ * 816 bytes per vertex per word.  A batch of `count` sources on a graph of n
+ * vertices fits when n * words(count) <= n_cap * words(b_cap). */
 typedef struct tbfs_batch tbfs_batch;
 
 int tbfs_batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch **out);
@@ -220,6 +220,59 @@ int tbfs_batch_run(tbfs_batch *b, const tbfs_graph *g, const int32_t *roots,
  * pointer may be NULL; n must equal the graph's vertex count). */
 int tbfs_batch_read(tbfs_batch *b, int32_t first, int32_t count, int32_t n,
                     int32_t *S, uint64_t *sigma, uint8_t *ovf);
+
+/* ---- serving -------------------------------------------------------------
+ * A server keeps a set of graphs resident and answers single-source queries
+ * submitted at any time.  A dispatcher thread batches the queries waiting for
+ * a graph (continuous batching: queries that arrive while the device is busy
+ * join the next batch formed for their graph) into batch slots whose device
+ * memory comes from mem_budget; slots run concurrently on their own streams.
+ * Every query's S / sigma / overflow flags are exactly those of tbfs_bfs from
+ * its root, whatever batch it ran in. */
+typedef struct tbfs_server tbfs_server;
+
+typedef struct tbfs_server_opts {
+  int64_t mem_budget; /* device bytes for batch slots (0: min(free/4, 64 GiB)) */
+  int32_t slots;      /* batches in flight at once (0: 4) */
+  int32_t max_batch;  /* sources per batch, <= 1024 (0: 1024) */
+  int32_t reserved[4];
+} tbfs_server_opts;
+
+typedef struct tbfs_query_done {
+  uint64_t ticket;
+  int32_t graph;
+  int32_t root;
+  int32_t status;    /* TBFS_OK or an error code */
+  int32_t batch;     /* sources in the batch that answered it */
+  double submit_us;  /* times on the server's clock (microseconds since */
+  double start_us;   /* creation): submitted, its batch started, its     */
+  double done_us;    /* results were in its host buffers               */
+} tbfs_query_done;
+
+/* graphs[i] stay owned by the caller and must outlive the server. */
+int tbfs_server_create(const tbfs_graph *const *graphs, int32_t ngraphs,
+                       const tbfs_server_opts *opts, tbfs_server **out);
+/* Stops the dispatcher (queries still queued are dropped) and frees slots. */
+int tbfs_server_destroy(tbfs_server *s);
+/* Queue a BFS from root on graphs[graph].  S (n int32), sigma (n uint64) and
+ * ovf (n uint8) are host buffers that receive the result (any may be NULL;
+ * pinned memory from tbfs_host_alloc makes the copies asynchronous); they
+ * must stay valid until the query is reported done. */
+int tbfs_server_submit(tbfs_server *s, int32_t graph, int32_t root, int32_t *S,
+                       uint64_t *sigma, uint8_t *ovf, uint64_t *ticket);
+/* Up to max completed queries, oldest first; waits up to wait_us for one. */
+int tbfs_server_poll(tbfs_server *s, tbfs_query_done *done, int32_t max,
+                     int32_t *ndone, int64_t wait_us);
+/* Block until every submitted query has completed. */
+int tbfs_server_drain(tbfs_server *s);
+/* Sources per batch for graphs[graph] (graph >= 0), batches started and
+ * queries they held so far, and the server clock. */
+int tbfs_server_info(tbfs_server *s, int32_t graph, int32_t *batch_cap,
+                     int64_t *batches, int64_t *batched, double *now_us);
+
+/* Pinned host memory (hipHostMalloc) for result buffers. */
+void *tbfs_host_alloc(int64_t bytes);
+int tbfs_host_free(void *p);
 
 #ifdef __cplusplus
 }

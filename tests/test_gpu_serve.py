@@ -1,0 +1,145 @@
+"""Serving (tbfs_server_*): resident graphs, queries batched as they arrive.
+
+Every query's S / sigma / overflow flags must equal the oracle's single-source
+BFS from its root, and must be the same whether the query ran alone or in a
+mix of concurrent queries on several graphs (tolerance 0).
+"""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+import turbobfs_amd as tb
+from turbobfs_amd import graphs, mtx
+from tests.conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph_set():
+    gs = [
+        (graphs.erdos_renyi(4096, 8.0 / 4095.0, seed=1), True),
+        (graphs.rmat(12, 16, 0.57, 0.19, 0.19, seed=2, relabel=True), True),
+        (graphs.grid(40, 30, 0.1, seed=3, relabel=True), True),
+        (graphs.grid(48, 48, 0.0, seed=0, relabel=False), True),   # counts past 2^64
+        (mtx.read_csc(golden("Tina_AskCal.mtx")), False),          # directed
+    ]
+    return [g for g, _ in gs], [tb.Graph(g.CP, g.IC, symmetric=s) for g, s in gs]
+
+
+def _submit(srv, g, gi, root, store):
+    S = tb.host_array(g.n, np.int32)
+    sig = tb.host_array(g.n, np.uint64)
+    ovf = tb.host_array(g.n, np.uint8)
+    t = srv.submit(gi, root, S, sig, ovf)
+    store[t] = (gi, root, S, sig, ovf)
+    return t
+
+
+def _check(cs, store, done):
+    for d in done:
+        assert d["status"] == 0, d
+        gi, root, S, sig, ovf = store[d["ticket"]]
+        eS, esig, eovf, _ = oracle.textbook(cs[gi].CP, cs[gi].IC, root)
+        assert np.array_equal(S, eS), (gi, root)
+        assert np.array_equal(sig, esig), (gi, root)
+        assert np.array_equal(ovf, eovf), (gi, root)
+        assert d["submit_us"] <= d["start_us"] <= d["done_us"]
+
+
+def test_serve_mixed_stream_exact_and_batched(gpu):
+    cs, gs = _graph_set()
+    srv = tb.Server(gs, mem_budget=1 << 30, slots=3)
+    rng = np.random.default_rng(5)
+    store = {}
+    print("[serve] mixed stream of 600 queries", flush=True)
+
+    def producer():
+        for i in range(600):
+            gi = int(rng.integers(0, len(cs)))
+            _submit(srv, cs[gi], gi, int(rng.integers(0, cs[gi].n)), store)
+            if i % 50 == 0:
+                time.sleep(0.002)
+
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    srv.drain()
+    done = srv.poll(1000)
+    assert len(done) == 600
+    _check(cs, store, done)
+    assert max(d["batch"] for d in done) > 1          # queries were batched
+    info = srv.info()
+    assert info["batched"] == 600 and info["batches"] < 600
+    srv.close()
+
+
+def test_serve_alone_equals_in_mix(gpu):
+    cs, gs = _graph_set()
+    probe = [(1, 5), (3, 0), (4, 10), (0, 77)]
+    alone = {}
+    srv = tb.Server(gs, mem_budget=1 << 30, slots=2)
+    for gi, r in probe:
+        store = {}
+        t = _submit(srv, cs[gi], gi, r, store)
+        srv.drain()
+        (d,) = srv.poll(10)
+        assert d["ticket"] == t and d["batch"] == 1
+        alone[(gi, r)] = store[t][2:]
+    store = {}
+    rng = np.random.default_rng(9)
+    tickets = {}
+    for i in range(300):
+        if i % 60 == 0:
+            gi, r = probe[(i // 60) % len(probe)]
+        else:
+            gi = int(rng.integers(0, len(cs)))
+            r = int(rng.integers(0, cs[gi].n))
+        tickets[_submit(srv, cs[gi], gi, r, store)] = (gi, r)
+    srv.drain()
+    done = srv.poll(1000)
+    _check(cs, store, done)
+    seen = 0
+    for d in done:
+        key = tickets[d["ticket"]]
+        if key in alone and d["batch"] > 1:
+            for a, b in zip(alone[key], store[d["ticket"]][2:]):
+                assert np.array_equal(a, b)
+            seen += 1
+    assert seen >= 1
+    srv.close()
+
+
+def test_serve_small_budget_limits_batches(gpu):
+    cs, gs = _graph_set()
+    # room for one 64-source word of the largest graph per slot
+    nmax = max(g.n for g in cs)
+    srv = tb.Server(gs, mem_budget=2 * nmax * 816 * 2, slots=2)
+    assert srv.info(0)["batch_cap"] == 128
+    store = {}
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        _submit(srv, cs[0], 0, int(rng.integers(0, cs[0].n)), store)
+    srv.drain()
+    done = srv.poll(1000)
+    assert len(done) == 200 and max(d["batch"] for d in done) <= 128
+    _check(cs, store, done)
+    srv.close()
+
+
+def test_serve_invalid(gpu):
+    cs, gs = _graph_set()
+    srv = tb.Server(gs[:2], mem_budget=1 << 28, slots=1)
+    with pytest.raises(tb.TbfsError):
+        srv.submit(2, 0)
+    with pytest.raises(tb.TbfsError):
+        srv.submit(0, cs[0].n)
+    t = srv.submit(0, 3)   # no host buffers: runs, copies nothing
+    srv.drain()
+    (d,) = srv.poll(4)
+    assert d["ticket"] == t and d["status"] == 0
+    srv.close()
+    with pytest.raises(tb.TbfsError):
+        tb.Server(gs, mem_budget=1000)   # not even one word of the largest graph

@@ -12,6 +12,7 @@ from .bfs import (  # noqa: F401
     Context,
     Graph,
     Result,
+    Server,
     bfs_gpu_bu_csc_sc,
     bfs_gpu_bu_csc_wa,
     bfs_gpu_td_cooc_sc,
@@ -20,10 +21,11 @@ from .bfs import (  # noqa: F401
     bfs_gpu_tdbu_csc_sc,
     bfs_gpu_tdbu_csc_wa,
     default_opts,
+    host_array,
 )
 
 __all__ = [
-    "Batch", "Context", "Graph", "Result", "default_opts", "graphs", "load", "LIB_PATH",
+    "Batch", "Context", "Graph", "Server", "host_array", "Result", "default_opts", "graphs", "load", "LIB_PATH",
     "device_count", "device_info",
     "TbfsError",
     "bfs_gpu_td_csc_sc", "bfs_gpu_td_csc_wa", "bfs_gpu_bu_csc_sc",

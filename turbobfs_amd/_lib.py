@@ -75,6 +75,28 @@ class Stats(C.Structure):
     ]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [
+        ("mem_budget", C.c_int64),
+        ("slots", C.c_int32),
+        ("max_batch", C.c_int32),
+        ("reserved", C.c_int32 * 4),
+    ]
+
+
+class QueryDone(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_uint64),
+        ("graph", C.c_int32),
+        ("root", C.c_int32),
+        ("status", C.c_int32),
+        ("batch", C.c_int32),
+        ("submit_us", C.c_double),
+        ("start_us", C.c_double),
+        ("done_us", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every symbol the public headers declare
 SIGNATURES = {
     # drop-in surface (bfstdcsc.h, bfsbucsc.h, bfstdbucsc.h, bfstdcooc.h)
@@ -111,6 +133,15 @@ SIGNATURES = {
     "tbfs_batch_stream": (C.c_void_p, [C.c_void_p]),
     "tbfs_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, _i32p, C.c_int32, C.POINTER(Opts), C.POINTER(Stats)]),
     "tbfs_batch_read": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _i32p, _u64p, _u8p]),
+    "tbfs_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
+    "tbfs_server_destroy": (C.c_int, [C.c_void_p]),
+    "tbfs_server_submit": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _i32p, _u64p, _u8p, C.POINTER(C.c_uint64)]),
+    "tbfs_server_poll": (C.c_int, [C.c_void_p, C.POINTER(QueryDone), C.c_int32, C.POINTER(C.c_int32), C.c_int64]),
+    "tbfs_server_drain": (C.c_int, [C.c_void_p]),
+    "tbfs_server_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "tbfs_host_alloc": (C.c_void_p, [C.c_int64]),
+    "tbfs_host_free": (C.c_int, [C.c_void_p]),
     # generators (tbfs_gen.h)
     "tbfs_gen_er": (C.c_int, [C.c_int32, C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
     "tbfs_gen_rmat": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),

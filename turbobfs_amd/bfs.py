@@ -257,6 +257,7 @@ class Batch:
                                          int(roots.shape[0]), C.byref(o), C.byref(st)),
               "tbfs_batch_run")
         self._n = g.n
+        self._g = g
         self._count = int(roots.shape[0])
         out = {f: getattr(st, f) for f, _ in Stats._fields_ if f != "pad"}
         out["count_bits"] = int(st.pad)
@@ -278,6 +279,80 @@ class Batch:
     def close(self):
         if self._h:
             _lib.load().tbfs_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def host_array(count: int, dtype) -> np.ndarray:
+    """A numpy array over pinned host memory (tbfs_host_alloc); freed when the
+    array is garbage-collected."""
+    dt = np.dtype(dtype)
+    nbytes = max(1, int(count) * dt.itemsize)
+    ptr = _lib.load().tbfs_host_alloc(nbytes)
+    if not ptr:
+        raise MemoryError("tbfs_host_alloc failed")
+    buf = (C.c_uint8 * nbytes).from_address(ptr)
+    arr = np.frombuffer(buf, dtype=dt, count=int(count))
+    import weakref
+    weakref.finalize(buf, _lib.load().tbfs_host_free, C.c_void_p(ptr))
+    return arr
+
+
+class Server:
+    """Resident graphs + a native dispatcher that batches queries as they
+    arrive (continuous batching over concurrent batch slots)."""
+
+    def __init__(self, graphs, mem_budget: int = 0, slots: int = 0, max_batch: int = 0):
+        self._graphs = list(graphs)  # keep the graphs alive
+        arr = (C.c_void_p * len(self._graphs))(*[g.handle for g in self._graphs])
+        o = _lib.ServerOpts()
+        o.mem_budget = int(mem_budget)
+        o.slots = int(slots)
+        o.max_batch = int(max_batch)
+        self._h = C.c_void_p()
+        check(_lib.load().tbfs_server_create(arr, len(self._graphs), C.byref(o), C.byref(self._h)),
+              "tbfs_server_create")
+        self._bufs = {}
+
+    def submit(self, graph: int, root: int, S=None, sigma=None, ovf=None) -> int:
+        """Queue a query; S / sigma / ovf (numpy int32 / uint64 / uint8 of n
+        entries, or None) receive the result and must stay alive until done."""
+        t = C.c_uint64()
+        check(_lib.load().tbfs_server_submit(
+            self._h, int(graph), int(root),
+            _ptr(S, _i32p) if S is not None else None,
+            _ptr(sigma, _u64p) if sigma is not None else None,
+            _ptr(ovf, _u8p) if ovf is not None else None, C.byref(t)), "tbfs_server_submit")
+        return int(t.value)
+
+    def poll(self, max_items: int = 1024, wait_us: int = 0) -> list[dict]:
+        buf = (_lib.QueryDone * max_items)()
+        k = C.c_int32()
+        check(_lib.load().tbfs_server_poll(self._h, buf, int(max_items), C.byref(k), int(wait_us)),
+              "tbfs_server_poll")
+        return [{f: getattr(buf[i], f) for f, _ in _lib.QueryDone._fields_} for i in range(k.value)]
+
+    def drain(self) -> None:
+        check(_lib.load().tbfs_server_drain(self._h), "tbfs_server_drain")
+
+    def info(self, graph: int = -1) -> dict:
+        cap = C.c_int32()
+        nb = C.c_int64()
+        nq = C.c_int64()
+        now = C.c_double()
+        check(_lib.load().tbfs_server_info(self._h, int(graph), C.byref(cap), C.byref(nb),
+                                           C.byref(nq), C.byref(now)), "tbfs_server_info")
+        return {"batch_cap": cap.value, "batches": nb.value, "batched": nq.value,
+                "now_us": now.value}
+
+    def close(self):
+        if self._h:
+            _lib.load().tbfs_server_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -617,37 +617,45 @@ __global__ __launch_bounds__(256) void k_batch_extract(
 using namespace tbfs;
 
 struct tbfs_batch {
-  int32_t n_cap = 0;
-  int32_t b_cap = 0;
-  int32_t w_cap = 0;
+  int64_t cap_words = 0;  // n * W of the largest batch shape the buffers hold
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr;    // start of the current batch
+  hipEvent_t ev_step = nullptr;  // end of the work enqueued so far
   uint64_t* F[3] = {nullptr, nullptr, nullptr};
   uint64_t* seen[2] = {nullptr, nullptr};
   uint64_t* ovfw = nullptr;
-  uint64_t* sigma = nullptr;  // viewed as uint32_t[n][B] by 32-bit runs
+  uint64_t* sigma = nullptr;  // viewed as uint32_t counts by 32-bit runs
   int32_t* S = nullptr;
   BCtrSet* cset = nullptr;
   BLoop* loop = nullptr;
   RunStats* stats = nullptr;
   RunStats* stats_h = nullptr;
-  int32_t* roots = nullptr;
-  int32_t* xS = nullptr;  // extraction staging: 64 rows of n_cap
+  int32_t* roots = nullptr;   // kBatchMaxW * 64 entries
+  int32_t* xS = nullptr;      // extraction staging: xrows rows of xn entries
   uint64_t* xsig = nullptr;
   uint8_t* xovf = nullptr;
+  int64_t xcap = 0;           // entries of the staging buffers
   int cus = 0;
   int chain_k = 8;
-  int32_t last_n = 0;
-  int32_t last_count = 0;
-  int32_t last_w = 0;
-  int last_ct = 8;                       // bytes per count of the last batch
-  const uint64_t* seen_final = nullptr;  // reached bits of the last batch
+  // the batch in flight (or the last one)
+  const tbfs_graph* g = nullptr;  // used only while the batch runs
+  int32_t n = 0;                   // vertices of its graph (rows stay readable)
+  tbfs_opts o{};
+  int32_t count = 0;
+  int W = 0;
+  int ct = 4;        // bytes per count of the current run
+  int grid = 0;
+  int d_next = 0;    // first level not yet enqueued
+  bool active = false;
+  bool done = false;
+  RunStats rs{};
+  float ms = 0.f;
+  const uint64_t* seen_final = nullptr;  // reached bits of the finished batch
 };
 
 namespace tbfs {
-namespace {
 
-int words_for(int32_t count) {
+int batch_words_for(int32_t count) {
   int w = 1;
   while (64 * w < count) w <<= 1;
   return w;
@@ -670,20 +678,18 @@ void batch_free(tbfs_batch* b) {
   dev_free(b->xovf);
   if (b->stats_h) (void)hipHostFree(b->stats_h);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
-  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->ev_step) (void)hipEventDestroy(b->ev_step);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
 
-int batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch** out) {
-  if (!out || n_cap <= 0 || b_cap <= 0 || b_cap > 64 * kBatchMaxW) return TBFS_ERR_INVALID;
+int batch_create_words(int64_t cap_words, tbfs_batch** out) {
+  if (!out || cap_words <= 0) return TBFS_ERR_INVALID;
   *out = nullptr;
   tbfs_batch* b = new (std::nothrow) tbfs_batch();
   if (!b) return TBFS_ERR_NOMEM;
-  b->n_cap = n_cap;
-  b->b_cap = b_cap;
-  b->w_cap = words_for(b_cap);
-  const size_t nw = (size_t)n_cap * b->w_cap;
+  b->cap_words = cap_words;
+  const size_t nw = (size_t)b->cap_words;
   const size_t nb = nw * 64;
   int dev = 0;
   int rc = TBFS_OK;
@@ -691,7 +697,7 @@ int batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch** out) {
   if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
-  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_step, hipEventDefault);
   if (e != hipSuccess) {
     rc = hip_fail(e, "batch stream/events", __FILE__, __LINE__);
     batch_free(b);
@@ -705,7 +711,7 @@ int batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch** out) {
   if (!rc) rc = dev_alloc(&b->cset, 3);
   if (!rc) rc = dev_alloc(&b->loop, 2);
   if (!rc) rc = dev_alloc(&b->stats, 1);
-  if (!rc) rc = dev_alloc(&b->roots, (size_t)b_cap);
+  if (!rc) rc = dev_alloc(&b->roots, (size_t)64 * kBatchMaxW);
   if (!rc) {
     e = hipHostMalloc((void**)&b->stats_h, sizeof(RunStats), hipHostMallocDefault);
     if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc", __FILE__, __LINE__);
@@ -717,6 +723,16 @@ int batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch** out) {
   *out = b;
   return TBFS_OK;
 }
+
+int batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch** out) {
+  if (!out || n_cap <= 0 || b_cap <= 0 || b_cap > 64 * kBatchMaxW) return TBFS_ERR_INVALID;
+  return batch_create_words((int64_t)n_cap * batch_words_for(b_cap), out);
+}
+
+int64_t batch_cap_words(const tbfs_batch* b) { return b->cap_words; }
+hipEvent_t batch_step_event(const tbfs_batch* b) { return b->ev_step; }
+
+namespace {
 
 // Pull slices of the in-lists (built on first use, kept with the graph).
 int graph_ensure_batch_slices(tbfs_graph* g) {
@@ -745,24 +761,12 @@ int graph_ensure_batch_slices(tbfs_graph* g) {
   return TBFS_OK;
 }
 
-template <int W, typename CT>
-int enqueue_levels(tbfs_batch* b, const BatchParams& p, int d0, int count, int grid) {
-  for (int d = d0; d < d0 + count; ++d) {
-    hipLaunchKernelGGL((k_batch_level<W, CT>), dim3(grid), dim3(kBatchBlock), 0, b->stream, p, d);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL((k_batch_scatter<W, CT>), dim3(grid), dim3(kBatchBlock), 0, b->stream, p, d);
-    HIPCHK(hipGetLastError());
-  }
-  return TBFS_OK;
-}
-
-template <int W, typename CT>
-int batch_run_w(tbfs_batch* b, const tbfs_graph* g, int32_t count, const tbfs_opts& o,
-                RunStats* rs_out, float* ms_out) {
+BatchParams make_bparams(const tbfs_batch* b) {
+  const tbfs_graph* g = b->g;
   BatchParams p{};
   p.n = g->n;
   p.nz = g->nz;
-  p.nsrc = count;
+  p.nsrc = b->count;
   p.CP = g->CP;
   p.IC = g->IC;
   p.RP = g->RP;
@@ -771,7 +775,7 @@ int batch_run_w(tbfs_batch* b, const tbfs_graph* g, int32_t count, const tbfs_op
   p.nslices = g->nbsl;
   p.bigo = g->bigo;
   p.nbigo = g->nbigo;
-  p.alpha = o.batch_alpha > 0 ? o.batch_alpha : kBatchAlpha;
+  p.alpha = b->o.batch_alpha > 0 ? b->o.batch_alpha : kBatchAlpha;
   for (int i = 0; i < 3; ++i) p.F[i] = b->F[i];
   for (int i = 0; i < 2; ++i) p.seen[i] = b->seen[i];
   p.ovfw = b->ovfw;
@@ -781,68 +785,75 @@ int batch_run_w(tbfs_batch* b, const tbfs_graph* g, int32_t count, const tbfs_op
   p.loop = b->loop;
   p.stats = b->stats;
   p.roots = b->roots;
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_level<W, CT>, kBatchBlock, 0) !=
-          hipSuccess || per_cu < 1) {
-    (void)hipGetLastError();
-    per_cu = 4;
-  }
-  int grid = o.grid > 0 ? std::min(o.grid, 65536) : std::max(1, b->cus) * per_cu;
+  return p;
+}
+
+// Enqueue (init + seed, when d0 == 0) and levels d0 .. d0+levels-1 of the
+// current batch, then the stats copy and the step event.
+template <int W, typename CT>
+int enqueue_w(tbfs_batch* b, int d0, int levels) {
+  const BatchParams p = make_bparams(b);
   hipStream_t s = b->stream;
-  HIPCHK(hipEventRecord(b->ev0, s));
-  const int igrid = grid_for((int64_t)g->n * W, 256, std::max(1, b->cus) * 8);
-  hipLaunchKernelGGL((k_batch_init<W, CT>), dim3(igrid), dim3(256), 0, s, p);
-  HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL((k_batch_seed<W, CT>), dim3((count + 255) / 256), dim3(256), 0, s, p);
-  HIPCHK(hipGetLastError());
-  int d = 0;
-  int rc = enqueue_levels<W, CT>(b, p, d, b->chain_k, grid);
-  if (rc) return rc;
-  d += b->chain_k;
-  for (;;) {
-    HIPCHK(hipMemcpyAsync(b->stats_h, b->stats, sizeof(RunStats), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (b->stats_h->levels >= 0) break;
-    if (d > g->n + 1) {  // a BFS has at most n levels: the loop state is corrupt
-      std::fprintf(stderr, "turbobfs: batched BFS did not finish within %d levels\n", d);
-      return TBFS_ERR_HIP;
+  if (b->grid == 0) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_level<W, CT>, kBatchBlock,
+                                                     0) != hipSuccess ||
+        per_cu < 1) {
+      (void)hipGetLastError();
+      per_cu = 4;
     }
-    rc = enqueue_levels<W, CT>(b, p, d, b->chain_k, grid);
-    if (rc) return rc;
-    d += b->chain_k;
+    b->grid = b->o.grid > 0 ? std::min(b->o.grid, 65536) : std::max(1, b->cus) * per_cu;
   }
-  HIPCHK(hipEventRecord(b->ev1, s));
-  HIPCHK(hipEventSynchronize(b->ev1));
-  HIPCHK(hipEventElapsedTime(ms_out, b->ev0, b->ev1));
-  *rs_out = *b->stats_h;
-  // one spare level beyond the deepest batch so far
-  b->chain_k = std::max(b->chain_k, std::min(4096, rs_out->levels + 2));
-  b->last_n = g->n;
-  b->last_count = count;
-  b->last_w = W;
-  b->last_ct = (int)sizeof(CT);
-  b->seen_final = b->seen[rs_out->levels & 1];  // OR of every frontier
+  if (d0 == 0) {
+    const int igrid = grid_for((int64_t)p.n * W, 256, std::max(1, b->cus) * 8);
+    hipLaunchKernelGGL((k_batch_init<W, CT>), dim3(igrid), dim3(256), 0, s, p);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL((k_batch_seed<W, CT>), dim3((p.nsrc + 255) / 256), dim3(256), 0, s, p);
+    HIPCHK(hipGetLastError());
+  }
+  for (int d = d0; d < d0 + levels; ++d) {
+    hipLaunchKernelGGL((k_batch_level<W, CT>), dim3(b->grid), dim3(kBatchBlock), 0, s, p, d);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL((k_batch_scatter<W, CT>), dim3(b->grid), dim3(kBatchBlock), 0, s, p, d);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemcpyAsync(b->stats_h, b->stats, sizeof(RunStats), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(b->ev_step, s));
   return TBFS_OK;
 }
 
 template <typename CT>
-int batch_dispatch(tbfs_batch* b, const tbfs_graph* g, int32_t count, const tbfs_opts& o,
-                   RunStats* rs, float* ms) {
-  switch (words_for(count)) {
-    case 1: return batch_run_w<1, CT>(b, g, count, o, rs, ms);
-    case 2: return batch_run_w<2, CT>(b, g, count, o, rs, ms);
-    case 4: return batch_run_w<4, CT>(b, g, count, o, rs, ms);
-    case 8: return batch_run_w<8, CT>(b, g, count, o, rs, ms);
-    default: return batch_run_w<16, CT>(b, g, count, o, rs, ms);
+int enqueue_ct(tbfs_batch* b, int d0, int levels) {
+  switch (b->W) {
+    case 1: return enqueue_w<1, CT>(b, d0, levels);
+    case 2: return enqueue_w<2, CT>(b, d0, levels);
+    case 4: return enqueue_w<4, CT>(b, d0, levels);
+    case 8: return enqueue_w<8, CT>(b, d0, levels);
+    default: return enqueue_w<16, CT>(b, d0, levels);
   }
 }
 
-int batch_run(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32_t count,
-              const tbfs_opts* o_in, tbfs_stats* st) {
-  if (!b || !gc || !roots || count < 1 || count > b->b_cap) return TBFS_ERR_INVALID;
+int enqueue(tbfs_batch* b, int d0, int levels) {
+  const int rc = b->ct == 4 ? enqueue_ct<uint32_t>(b, d0, levels)
+                            : enqueue_ct<uint64_t>(b, d0, levels);
+  if (!rc) b->d_next = d0 + levels;
+  return rc;
+}
+
+}  // namespace
+
+// Start a batch without waiting: roots are copied, the first chain of levels
+// is enqueued on the batch's stream.  32-bit counts first (half the bytes of
+// every count gather); a batch in which some count reaches 2^32 is run again
+// with 64-bit counts by batch_progress.
+int batch_start(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32_t count,
+                const tbfs_opts* o_in) {
+  if (!b || !gc || !roots || count < 1 || count > 64 * kBatchMaxW) return TBFS_ERR_INVALID;
+  if (b->active) return TBFS_ERR_INVALID;
   tbfs_graph* g = const_cast<tbfs_graph*>(gc);
   if (!g->CP) return TBFS_ERR_UNSUPPORTED;  // needs in-lists (CSC)
-  if (g->n > b->n_cap) return TBFS_ERR_INVALID;
+  const int W = batch_words_for(count);
+  if ((int64_t)g->n * W > b->cap_words) return TBFS_ERR_INVALID;  // too big for this batch object
   for (int32_t i = 0; i < count; ++i)
     if (roots[i] < 0 || roots[i] >= g->n) return TBFS_ERR_INVALID;
   tbfs_opts o;
@@ -851,51 +862,121 @@ int batch_run(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32_t
     return TBFS_ERR_INVALID;
   int rc = graph_ensure_batch_slices(g);
   if (rc) return rc;
+  b->g = g;
+  b->n = g->n;
+  b->o = o;
+  b->count = count;
+  b->W = W;
+  b->ct = o.batch_count_bits == 64 ? 8 : 4;
+  b->grid = 0;
+  b->done = false;
+  b->seen_final = nullptr;
   HIPCHK(hipMemcpyAsync(b->roots, roots, (size_t)count * sizeof(int32_t),
                         hipMemcpyHostToDevice, b->stream));
-  // 32-bit counts first (half the bytes of every count gather); if some count
-  // reaches 2^32 the batch runs again with 64-bit counts
-  RunStats rs{};
-  float ms = 0.f, ms64 = 0.f;
-  bool wide = o.batch_count_bits == 64;
-  if (!wide) {
-    rc = batch_dispatch<uint32_t>(b, g, count, o, &rs, &ms);
-    if (rc) return rc;
-    wide = rs.ovf_any != 0 && o.batch_count_bits != 32;
-  }
-  if (wide) {
-    rc = batch_dispatch<uint64_t>(b, g, count, o, &rs, &ms64);
-    if (rc) return rc;
-  }
-  if (st) {
-    st->levels = rs.levels;
-    st->push_levels = rs.push_levels;
-    st->pull_levels = rs.pull_levels;
-    st->pad = b->last_ct * 8;  // count width of the final run
-    st->reached = (int64_t)rs.reached;
-    st->edges = (int64_t)rs.edges;
-    st->ovf64 = rs.ovf_any ? -1 : 0;
-    st->ms = (double)(ms + ms64);
-  }
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  rc = enqueue(b, 0, b->chain_k);
+  if (rc) return rc;
+  b->active = true;
   return TBFS_OK;
 }
 
-int batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* S,
-               uint64_t* sigma, uint8_t* ovf) {
-  if (!b || n != b->last_n || n <= 0 || first < 0 || count < 0 ||
-      first + count > b->last_count)
-    return TBFS_ERR_INVALID;
-  if (count == 0) return TBFS_OK;
-  if (!b->xS) {
-    int rc = dev_alloc(&b->xS, (size_t)64 * b->n_cap);
-    if (!rc) rc = dev_alloc(&b->xsig, (size_t)64 * b->n_cap);
-    if (!rc) rc = dev_alloc(&b->xovf, (size_t)64 * b->n_cap);
-    if (rc) return rc;
+// Advance the batch in flight without blocking: *finished = true once its
+// results are complete on the device (b->rs, b->ms filled).
+int batch_progress(tbfs_batch* b, bool* finished) {
+  *finished = false;
+  if (!b->active) {
+    *finished = b->done;
+    return b->done ? TBFS_OK : TBFS_ERR_INVALID;
   }
+  const hipError_t q = hipEventQuery(b->ev_step);
+  if (q == hipErrorNotReady) return TBFS_OK;
+  if (q != hipSuccess) {
+    b->active = false;
+    return hip_fail(q, "hipEventQuery", __FILE__, __LINE__);
+  }
+  const RunStats rs = *b->stats_h;
+  if (rs.levels < 0) {  // needs more levels than enqueued
+    if (b->d_next > b->n + 1) {  // a BFS has at most n levels: the state is corrupt
+      std::fprintf(stderr, "turbobfs: batched BFS did not finish within %d levels\n", b->d_next);
+      b->active = false;
+      return TBFS_ERR_HIP;
+    }
+    const int rc = enqueue(b, b->d_next, b->chain_k);
+    if (rc) b->active = false;
+    return rc;
+  }
+  if (b->ct == 4 && rs.ovf_any && b->o.batch_count_bits != 32) {  // widen and run again
+    b->ct = 8;
+    b->grid = 0;
+    const int rc = enqueue(b, 0, b->chain_k);
+    if (rc) b->active = false;
+    return rc;
+  }
+  HIPCHK(hipEventElapsedTime(&b->ms, b->ev0, b->ev_step));
+  b->rs = rs;
+  // one spare level beyond the deepest batch so far
+  b->chain_k = std::max(b->chain_k, std::min(4096, rs.levels + 2));
+  b->seen_final = b->seen[rs.levels & 1];  // OR of every frontier
+  b->active = false;
+  b->done = true;
+  *finished = true;
+  return TBFS_OK;
+}
+
+void batch_fill_stats(const tbfs_batch* b, tbfs_stats* st) {
+  const RunStats& rs = b->rs;
+  st->levels = rs.levels;
+  st->push_levels = rs.push_levels;
+  st->pull_levels = rs.pull_levels;
+  st->pad = b->ct * 8;  // count width of the final run
+  st->reached = (int64_t)rs.reached;
+  st->edges = (int64_t)rs.edges;
+  st->ovf64 = rs.ovf_any ? -1 : 0;
+  st->ms = (double)b->ms;
+}
+
+int batch_run(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t count,
+              const tbfs_opts* o, tbfs_stats* st) {
+  int rc = batch_start(b, g, roots, count, o);
+  if (rc) return rc;
+  for (;;) {
+    bool fin = false;
+    rc = batch_progress(b, &fin);
+    if (rc) return rc;
+    if (fin) break;
+    HIPCHK(hipEventSynchronize(b->ev_step));
+  }
+  if (st) batch_fill_stats(b, st);
+  return TBFS_OK;
+}
+
+// Enqueue the rows of sources first .. first+count-1 of the finished batch
+// into per-source host destinations (row i -> S_dst[i], sig_dst[i],
+// ovf_dst[i]; any may be null), through a device staging buffer of bounded
+// size; ends with the step event recorded (nothing waits here).
+int batch_enqueue_rows(tbfs_batch* b, int32_t first, int32_t count, int32_t* const* S_dst,
+                       uint64_t* const* sig_dst, uint8_t* const* ovf_dst) {
+  if (!b->done || first < 0 || count < 0 || first + count > b->count) return TBFS_ERR_INVALID;
+  const int32_t n = b->n;
+  const int64_t stage_bytes = (int64_t)256 << 20;
+  const int rows = (int)std::max<int64_t>(1, std::min<int64_t>(64, stage_bytes / ((int64_t)n * 13)));
+  const int64_t need = (int64_t)rows * n;
   hipStream_t s = b->stream;
-  for (int32_t f = 0; f < count; f += 64) {
-    const int32_t c = std::min(64, count - f);
-    if (b->last_ct == 4)
+  if (b->xcap < need) {
+    HIPCHK(hipStreamSynchronize(s));
+    dev_free(b->xS);
+    dev_free(b->xsig);
+    dev_free(b->xovf);
+    b->xcap = 0;
+    int rc = dev_alloc(&b->xS, (size_t)need);
+    if (!rc) rc = dev_alloc(&b->xsig, (size_t)need);
+    if (!rc) rc = dev_alloc(&b->xovf, (size_t)need);
+    if (rc) return rc;
+    b->xcap = need;
+  }
+  for (int32_t f = 0; f < count; f += rows) {
+    const int32_t c = std::min(rows, count - f);
+    if (b->ct == 4)
       hipLaunchKernelGGL(k_batch_extract<uint32_t>, dim3((n + 63) / 64), dim3(256), 0, s, b->S,
                          (const uint32_t*)b->sigma, b->ovfw, b->seen_final, n, first + f, c,
                          b->xS, b->xsig, b->xovf);
@@ -904,16 +985,42 @@ int batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* 
                          (const uint64_t*)b->sigma, b->ovfw, b->seen_final, n, first + f, c,
                          b->xS, b->xsig, b->xovf);
     HIPCHK(hipGetLastError());
-    const size_t rows = (size_t)c * n;
-    if (S) HIPCHK(hipMemcpyAsync(S + (size_t)f * n, b->xS, rows * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    if (sigma) HIPCHK(hipMemcpyAsync(sigma + (size_t)f * n, b->xsig, rows * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    if (ovf) HIPCHK(hipMemcpyAsync(ovf + (size_t)f * n, b->xovf, rows, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    for (int32_t r = 0; r < c; ++r) {
+      const size_t off = (size_t)r * n;
+      if (S_dst && S_dst[f + r])
+        HIPCHK(hipMemcpyAsync(S_dst[f + r], b->xS + off, (size_t)n * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, s));
+      if (sig_dst && sig_dst[f + r])
+        HIPCHK(hipMemcpyAsync(sig_dst[f + r], b->xsig + off, (size_t)n * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, s));
+      if (ovf_dst && ovf_dst[f + r])
+        HIPCHK(hipMemcpyAsync(ovf_dst[f + r], b->xovf + off, (size_t)n, hipMemcpyDeviceToHost, s));
+    }
   }
+  HIPCHK(hipEventRecord(b->ev_step, s));
   return TBFS_OK;
 }
 
-}  // namespace
+int batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* S,
+               uint64_t* sigma, uint8_t* ovf) {
+  if (!b || !b->done || n != b->n || n <= 0 || first < 0 || count < 0 ||
+      first + count > b->count)
+    return TBFS_ERR_INVALID;
+  if (count == 0) return TBFS_OK;
+  std::vector<int32_t*> sd((size_t)count);
+  std::vector<uint64_t*> gd((size_t)count);
+  std::vector<uint8_t*> od((size_t)count);
+  for (int32_t i = 0; i < count; ++i) {
+    sd[i] = S ? S + (size_t)i * n : nullptr;
+    gd[i] = sigma ? sigma + (size_t)i * n : nullptr;
+    od[i] = ovf ? ovf + (size_t)i * n : nullptr;
+  }
+  int rc = batch_enqueue_rows(b, first, count, sd.data(), gd.data(), od.data());
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return TBFS_OK;
+}
+
 }  // namespace tbfs
 
 extern "C" {

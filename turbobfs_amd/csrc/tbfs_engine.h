@@ -112,6 +112,24 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots,
                  int count, const tbfs_opts& o, tbfs_stats* st);
 int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots,
               const tbfs_opts& o, tbfs_stats* st);
+
+// batched sources (tbfs_batch.hip)
+int batch_words_for(int32_t count);
+int batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch** out);
+int batch_create_words(int64_t cap_words, tbfs_batch** out);
+void batch_free(tbfs_batch* b);
+int64_t batch_cap_words(const tbfs_batch* b);
+hipEvent_t batch_step_event(const tbfs_batch* b);
+int batch_start(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t count,
+                const tbfs_opts* o);
+int batch_progress(tbfs_batch* b, bool* finished);
+void batch_fill_stats(const tbfs_batch* b, tbfs_stats* st);
+int batch_run(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t count,
+              const tbfs_opts* o, tbfs_stats* st);
+int batch_enqueue_rows(tbfs_batch* b, int32_t first, int32_t count, int32_t* const* S_dst,
+                       uint64_t* const* sig_dst, uint8_t* const* ovf_dst);
+int batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* S,
+               uint64_t* sigma, uint8_t* ovf);
 int ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma,
              uint8_t* ovf);
 }  // namespace tbfs
