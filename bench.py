@@ -13,8 +13,9 @@ Prints ONE JSON line.  Also reports the roofline of the dominant kernel, a
 CPU baseline (the reference's own sequential bfs_seq_td_csc, compiled from
 its sources into oracle/_ref, else this project's C restatement), and
 (unless --no-batch) the batched-sources throughput against the per-source
-loop (R-MAT scale 18, 1024 sources per batch) and a 512-graph block-diagonal
-multi-root BFS against a per-graph loop.
+loop (R-MAT scale 18, 1024 sources per batch), a 512-graph block-diagonal
+multi-root BFS against a per-graph loop, and (unless --no-serve) the query
+server's saturated throughput and p50 / p99 latency under Poisson arrivals.
 """
 from __future__ import annotations
 
@@ -174,6 +175,96 @@ def bench_multigraph(tb, graphs, reps: int = 3):
     }
 
 
+def bench_serving(tb, graphs, seconds: float = 1.5, rates=(0.25, 0.5, 0.8)):
+    """Query serving: 8 resident graphs (R-MAT scales 16..22, ef16,
+    symmetrized, relabelled, seed = scale; a 1024x1024 grid with 10% of its
+    edges removed, seed 3), (graph, source) uniform (seed 5).  First the
+    saturated rate (a burst of queries submitted at once), then Poisson
+    arrivals at fractions of it; latency = submit -> S, sigma, overflow flags
+    in the query's pinned host buffers."""
+    cs = [graphs.rmat(sc, 16, 0.57, 0.19, 0.19, seed=sc, relabel=True) for sc in range(16, 23)]
+    cs.append(graphs.grid(1024, 1024, 0.1, seed=3, relabel=True))
+    names = [f"rmat{sc}" for sc in range(16, 23)] + ["grid1024"]
+    gs = [tb.Graph(g.CP, g.IC, symmetric=True) for g in cs]
+    srv = tb.Server(gs)
+    caps = [srv.info(i)["batch_cap"] for i in range(len(gs))]
+    # result buffers: a ring of pinned rows per graph, reused round-robin
+    rings = []
+    for g in cs:
+        k = int(min(64, max(4, (256 << 20) // (13 * g.n))))
+        rings.append((k, tb.host_array(k * g.n, np.int32), tb.host_array(k * g.n, np.uint64),
+                      tb.host_array(k * g.n, np.uint8)))
+    use = [0] * len(cs)
+
+    def submit(gi, r):
+        k, S, sig, ovf = rings[gi]
+        j = use[gi] % k
+        use[gi] += 1
+        n = cs[gi].n
+        return srv.submit(gi, r, S[j * n:(j + 1) * n], sig[j * n:(j + 1) * n],
+                          ovf[j * n:(j + 1) * n])
+
+    rng = np.random.default_rng(5)
+
+    def draw(count):
+        gi = rng.integers(0, len(cs), size=count)
+        rt = [int(rng.integers(0, cs[int(x)].n)) for x in gi]
+        return [(int(a), b) for a, b in zip(gi, rt)]
+
+    # warm-up: every graph once (first-use slice tables, kernel loading)
+    for gi in range(len(cs)):
+        submit(gi, 0)
+    srv.drain()
+    srv.poll(4096)
+
+    def collect(expect):
+        out = []
+        while len(out) < expect:
+            out += srv.poll(4096, wait_us=100000)
+        return out
+
+    # saturated: everything at once
+    burst = draw(1536)
+    for gi, r in burst:
+        submit(gi, r)
+    done = collect(len(burst))
+    t_first = min(d["submit_us"] for d in done)
+    t_last = max(d["done_us"] for d in done)
+    sat_qps = len(done) / ((t_last - t_first) * 1e-6)
+    lat = np.array([d["done_us"] - d["submit_us"] for d in done]) * 1e-3
+    result = {
+        "workload": "8 resident graphs: R-MAT scales 16-22 (ef16) and a 1024x1024 grid with "
+                    "10% of edges removed; (graph, source) uniform, seed 5; results copied to "
+                    "pinned host buffers",
+        "graphs": {nm: {"n": g.n, "nz": g.nz, "batch_cap": c} for nm, g, c in zip(names, cs, caps)},
+        "saturated": {"queries": len(done), "queries_per_s": sat_qps,
+                      "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                      "mean_batch": float(np.mean([d["batch"] for d in done]))},
+        "poisson": [],
+    }
+    for frac in rates:
+        rate = frac * sat_qps
+        count = max(64, int(rate * seconds))
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=count))
+        qs = draw(count)
+        t0 = time.perf_counter()
+        for (gi, r), ta in zip(qs, arr):
+            while time.perf_counter() - t0 < ta:
+                pass
+            submit(gi, r)
+        done = collect(count)
+        lat = np.array([d["done_us"] - d["submit_us"] for d in done]) * 1e-3
+        span = (max(d["done_us"] for d in done) - min(d["submit_us"] for d in done)) * 1e-6
+        result["poisson"].append({
+            "offered_fraction": frac, "offered_qps": rate, "queries": count,
+            "achieved_qps": count / span,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_batch": float(np.mean([d["batch"] for d in done])),
+        })
+    srv.close()
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +284,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-sources and multi-graph measurements")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--calib-mib", type=int, default=512,
                     help="size of the plain copy kernel run first (byte-counter reference)")
     args = ap.parse_args()
@@ -299,6 +391,8 @@ def main():
     if not args.no_batch:
         line["batched"] = bench_batched(tb, graphs)
         line["multi_graph"] = bench_multigraph(tb, graphs)
+    if not args.no_serve:
+        line["serving"] = bench_serving(tb, graphs)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(g, sources, args.cpu_budget)
     print(json.dumps(line))
