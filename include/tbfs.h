@@ -195,7 +195,7 @@ int tbfs_ctx_device_ptrs(tbfs_ctx *c, int32_t **S, uint64_t **sigma,
  * exactly those of tbfs_bfs from that root.  A batch object owns its own HIP
  * stream and device state sized for n_cap vertices times b_cap sources
  * rounded up to a power-of-two number of 64-source words (b_cap <= 1024):
- * 816 bytes per vertex per word.  A batch of `count` sources on a graph of n
+ * 828 bytes per vertex per word.  A batch of `count` sources on a graph of n
  * vertices fits when n * words(count) <= n_cap * words(b_cap). */
 typedef struct tbfs_batch tbfs_batch;
 

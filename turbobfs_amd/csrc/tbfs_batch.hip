@@ -8,6 +8,9 @@
 //   seen[2] uint64[W][n]  OR of the frontiers of depths < d (slot d & 1); bits
 //                         of unused sources (index >= count) start set
 //   ovfw    uint64[W][n]  per-source "path count >= 2^bits(CT)" bits
+//   lst[3]  uint32[W*n]   word lists of the frontiers (index w * n + v), made
+//                         by push levels; a level whose frontier and previous
+//                         frontier are both listed touches only those words
 // Counts are uint32 first (CT = uint32_t: half the bytes of every gather);
 // a batch in which some count reaches 2^32 runs again with CT = uint64_t.
 //   S       int32[W][n][64] per-source depth } valid where the pair's seen
@@ -30,20 +33,17 @@
 // the edge's mask has the lane's bit.  Items run word-major, so the waves in
 // flight share one word and the caches hold that word's frontier array and
 // the count segments of its hubs.
-// A vertex whose in-list is one slice is written by that one wavefront with
-// plain stores (every reached (v, source) entry is written once, at its
-// discovery); longer in-lists are split over slices whose partial sums are
-// added with atomics into rows zeroed at batch start.
+// Counts start at zero (the batch's init kernel clears them).  A vertex whose
+// in-list is one slice is written by that one wavefront with plain stores;
+// longer in-lists are split over slices whose partial sums are added with
+// atomics.
 //
 // Small frontiers expand top-down instead (push while the frontier's
-// out-degree sum times alpha stays below nz * W): the frontier words push
-// their bits along the out-lists with atomic ORs (the lane that sets a bit
-// zeroes that pair's count and writes its depth), and a second kernel walks
-// the same edges again adding each parent's count atomically into the pairs
-// now marked.
+// out-degree sum times alpha stays below nz * W): each frontier word pushes
+// its bits along the out-lists with atomic ORs (the lane that sets a bit
+// first writes its depth) and adds its counts atomically, in one pass.
 //
-// The loop is chained: one kernel per level (plus the count kernel, which
-// returns at once after a pull level), enqueued back to back; each level reads the counters of its
+// The loop is chained: one kernel per level, enqueued back to back; each level reads the counters of its
 // frontier (written by the previous kernels), chooses its direction, stops
 // when the frontier is empty, and clears the buffers of depth d-1.
 //
@@ -72,7 +72,8 @@ using namespace dev;
 constexpr int kBatchSlice = 256;  // in-list entries per pull slice
 constexpr int kBatchBlock = 256;  // threads per workgroup (4 wavefronts)
 constexpr int kBatchMaxW = 16;    // words per vertex: up to 1024 sources
-constexpr float kBatchAlpha = 32.0f;  // default push / pull switch
+constexpr float kBatchAlpha = 32.0f;
+constexpr int kBatchWideLevels = 16;  // levels on the full-occupancy grid  // default push / pull switch
 
 // Level counters, striped over kStripes cache lines like the single-source
 // ones (a workgroup adds into stripe blockIdx & (kStripes - 1)).
@@ -82,7 +83,7 @@ struct alignas(128) BStripe {
   unsigned long long edges;  // sum of in-degrees over those pairs
   unsigned long long mout;   // sum of out-degrees over the new words (push work)
   unsigned int ovf;          // some new count is >= 2^64
-  unsigned int pad0;
+  unsigned int tail;         // stripe 0 only: entries appended to the word list
   unsigned long long pad1[11];
 };
 struct BCtrSet {
@@ -94,12 +95,16 @@ struct alignas(64) BLoop {
   int32_t d;        // depth of the current frontier
   int32_t done;     // the frontier of depth d is empty
   uint32_t ovf_any; // some count so far is >= 2^64 (parents' flags matter)
-  int32_t pushing;  // level d - 1 expanded top-down (its counts come from the fix pass)
+  int32_t pushing;  // level d - 1 expanded top-down
   int32_t push_levels;
   int32_t pull_levels;
   unsigned long long reached;  // (v, source) pairs reached before depth d
   unsigned long long edges;    // their in-degrees
-  unsigned long long pad[4];
+  int32_t list_cur;   // the word list of frontier d is complete (made by a push level)
+  int32_t list_prev;  // the same for frontier d - 1
+  uint32_t q_prev;    // words in frontier d - 1
+  uint32_t pad0;
+  unsigned long long pad[2];
 };
 
 struct BatchParams {
@@ -116,6 +121,7 @@ struct BatchParams {
   int32_t nbigo;
   float alpha;             // push while m_out * alpha < nz * W
   uint64_t* F[3];
+  uint32_t* lst[3];  // word lists of the frontiers (index w * n + v), rotating like F
   uint64_t* seen[2];
   uint64_t* ovfw;
   void* sigma;    // CT[n][B]: uint32_t or uint64_t counts
@@ -159,20 +165,19 @@ __global__ __launch_bounds__(256) void k_batch_init(BatchParams p) {
     p.seen[1][i] = pad;
     p.ovfw[i] = 0ull;
   }
-  // rows of split in-lists receive atomic adds: zero them
-  for (int64_t t = gtid; t < (int64_t)p.nslices * W; t += gstride) {
-    const int w = (int)(t / p.nslices);
-    const uint64_t e = p.slices[t - (int64_t)w * p.nslices];
-    const int v = (int)(uint32_t)e;
-    if ((e >> 32) != 0 || p.CP[v + 1] - p.CP[v] <= kBatchSlice) continue;
-    CT* row = (CT*)p.sigma + ((int64_t)w * p.n + v) * 64;
-    for (int j = 0; j < 64; ++j) row[j] = 0;
-  }
+  // every count starts at zero: push levels and split in-lists add into them
+  // atomically (pull levels on single-slice vertices store their sums)
+  CT* sg = (CT*)p.sigma;
+  const int64_t ncounts = nw * 64;
+  for (int64_t i = gtid; i < ncounts; i += gstride) sg[i] = 0;
   const int nctr = (int)(3 * sizeof(BCtrSet) / sizeof(unsigned long long));
   unsigned long long* cw = reinterpret_cast<unsigned long long*>(p.cset);
   for (int64_t i = gtid; i < nctr; i += gstride) cw[i] = 0ull;
   if (gtid == 0) {
-    p.loop[0] = BLoop{};
+    BLoop l0{};
+    l0.list_cur = 1;   // the seed kernel lists the depth-0 words
+    l0.list_prev = 1;  // (no depth -1 words)
+    p.loop[0] = l0;
     p.loop[1] = BLoop{};
     RunStats rs{};
     rs.levels = -1;  // set by the level that finds the frontier empty
@@ -196,6 +201,7 @@ __global__ __launch_bounds__(256) void k_batch_seed(BatchParams p) {
   if (old == 0) {
     atomicAdd(&c.q, 1ull);
     atomicAdd(&c.mout, (unsigned long long)(p.RP[r + 1] - p.RP[r]));
+    p.lst[0][atomicAdd(&p.cset[0].s[0].tail, 1u)] = (uint32_t)((j >> 6) * p.n + r);
   }
   atomicAdd(&c.bits, 1ull);
   atomicAdd(&c.edges, (unsigned long long)(p.CP[r + 1] - p.CP[r]));
@@ -220,6 +226,18 @@ __device__ __forceinline__ void flush_acc(BStripe& c, BAcc a) {
     if (a.mout) atomicAdd(&c.mout, a.mout);
     if (a.ovf) atomicOr(&c.ovf, 1u);
   }
+}
+
+// Append word index iw (where flagged) to a word list; one atomic per wave.
+__device__ __forceinline__ void list_append(uint32_t* lst, unsigned int* tail, bool flag,
+                                            uint32_t iw) {
+  const uint64_t m = __ballot(flag);
+  if (!m) return;
+  const int first = __builtin_ctzll(m);
+  unsigned int base = 0;
+  if (lane() == first) base = atomicAdd(tail, (unsigned int)__popcll(m));
+  base = __shfl(base, first);
+  if (flag) lst[base + __popcll(m & lanemask_lt())] = iw;
 }
 
 // Pull over (word, slice) items in word-major order: the waves in flight
@@ -350,57 +368,49 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
   }
 }
 
-// Top-down edge u -> v for word w, f = frontier word of u.
-//   ADD = false (mark): the sources of f unvisited at v join the next
-//     frontier (atomic OR); the lane that sets a bit first zeroes that pair's
-//     count and writes its depth.
-//   ADD = true (after every mark of the level has landed): the sources of f
-//     now marked at v (exactly the shortest-path edges u -> v into depth
-//     d + 1) get u's count added atomically, with carry and inherited
-//     overflow flags.
-template <int W, bool ADD, typename CT>
-__device__ __forceinline__ void push_edge(const BatchParams& p, const uint64_t* Fc, uint64_t* Fn,
+// Top-down edge u -> v for word w, f = frontier word of u: the sources of f
+// unvisited at v (shortest-path edges u -> v into depth d + 1) join the next
+// frontier (atomic OR; the lane that sets a bit first writes its depth) and
+// get u's count added atomically (counts start at zero), with carry and
+// inherited overflow flags.  Returns true when the OR made the word nonzero
+// (the word joins the next frontier's word list).
+template <int W, typename CT>
+__device__ __forceinline__ bool push_edge(const BatchParams& p, const uint64_t* Fc, uint64_t* Fn,
                                           const uint64_t* sd, int32_t d, int u, int v, int w,
                                           uint64_t f, bool pov, BAcc& a) {
   const int64_t iv = (int64_t)w * p.n + v;
-  if (!ADD) {
-    const uint64_t m = f & ~(sd[iv] | Fc[iv]);
-    if (!m) return;
-    const unsigned long long old = atomicOr((unsigned long long*)&Fn[iv], (unsigned long long)m);
-    uint64_t newly = m & ~(uint64_t)old;
-    if (!newly) return;
-    const unsigned long long c = (unsigned long long)__popcll(newly);
-    a.bits += c;
-    a.edges += c * (unsigned long long)(p.CP[v + 1] - p.CP[v]);
-    if (old == 0) {
-      a.q += 1;
-      a.mout += (unsigned long long)(p.RP[v + 1] - p.RP[v]);
-    }
-    while (newly) {
-      const int sb = __builtin_ctzll(newly);
-      newly &= newly - 1;
-      const int64_t ix = iv * 64 + sb;
-      ((CT*)p.sigma)[ix] = 0;
-      p.S[ix] = d + 1;
-    }
-  } else {
-    uint64_t m = f & Fn[iv];
-    if (!m) return;
-    const int64_t iu = (int64_t)w * p.n + u;
-    const uint64_t po = pov ? p.ovfw[iu] : 0ull;
-    uint64_t ob = 0;
-    while (m) {
-      const int sb = __builtin_ctzll(m);
-      m &= m - 1;
-      const CT x = ((const CT*)p.sigma)[iu * 64 + sb];
-      const CT old = atomic_add_ct((CT*)p.sigma + iv * 64 + sb, x);
-      if ((CT)(old + x) < old || ((po >> sb) & 1ull)) ob |= 1ull << sb;
-    }
-    if (ob) {
-      atomicOr((unsigned long long*)&p.ovfw[iv], (unsigned long long)ob);
-      a.ovf = 1;
-    }
+  uint64_t m = f & ~(sd[iv] | Fc[iv]);
+  if (!m) return false;
+  const int64_t iu = (int64_t)w * p.n + u;
+  const unsigned long long old = atomicOr((unsigned long long*)&Fn[iv], (unsigned long long)m);
+  const uint64_t po = pov ? p.ovfw[iu] : 0ull;
+  uint64_t newly = m & ~(uint64_t)old;
+  uint64_t ob = 0;
+  while (m) {
+    const int sb = __builtin_ctzll(m);
+    m &= m - 1;
+    const CT x = ((const CT*)p.sigma)[iu * 64 + sb];
+    const CT prev = atomic_add_ct((CT*)p.sigma + iv * 64 + sb, x);
+    if ((CT)(prev + x) < prev || ((po >> sb) & 1ull)) ob |= 1ull << sb;
   }
+  if (ob) {
+    atomicOr((unsigned long long*)&p.ovfw[iv], (unsigned long long)ob);
+    a.ovf = 1;
+  }
+  if (!newly) return false;
+  const unsigned long long c = (unsigned long long)__popcll(newly);
+  a.bits += c;
+  a.edges += c * (unsigned long long)(p.CP[v + 1] - p.CP[v]);
+  if (old == 0) {
+    a.q += 1;
+    a.mout += (unsigned long long)(p.RP[v + 1] - p.RP[v]);
+  }
+  while (newly) {
+    const int sb = __builtin_ctzll(newly);
+    newly &= newly - 1;
+    p.S[iv * 64 + sb] = d + 1;
+  }
+  return old == 0;
 }
 
 struct PushLds {
@@ -411,22 +421,30 @@ struct PushLds {
   uint64_t f[64];
 };
 
-// Top-down pass over every nonzero frontier word (u, w): u's out-list edges.
-// Lists up to kBigIn entries are split over the lanes of a wavefront by edge
-// count (prefix scan + owner search); longer lists go by their static
-// kPullChunk slices, one (slice, word) item per wavefront.
-template <int W, bool ADD, typename CT>
+// Top-down pass over the frontier words (u, w): u's out-list edges.  The
+// words come from the frontier's word list when it is complete (`nlist` >= 0
+// entries; sparse frontiers cost in proportion to their size), else from a
+// scan of every word.  Lists up to kBigIn entries are split over the lanes of
+// a wavefront by edge count (prefix scan + owner search); longer lists go by
+// their static kPullChunk slices, one (slice, word) item per wavefront.  The
+// words made nonzero are listed for the next level (`nlst`, `ntail`).
+template <int W, typename CT>
 __device__ __forceinline__ void batch_push(const BatchParams& p, const uint64_t* Fc,
                                            uint64_t* Fn, const uint64_t* sd, int32_t d, bool pov,
-                                           int64_t gtid, int64_t gstride, PushLds& L, BAcc& a) {
+                                           const uint32_t* lst, int64_t nlist, uint32_t* nlst,
+                                           unsigned int* ntail, int64_t gtid, int64_t gstride,
+                                           PushLds& L, BAcc& a) {
   const int ln = lane();
   const int64_t nwaves = gstride >> 6;
   const int64_t nW = (int64_t)p.n * W;
-  for (int64_t base = (gtid >> 6) * 64; base < nW; base += nwaves * 64) {
-    const int64_t i = base + ln;
+  const int64_t span = nlist >= 0 ? nlist : nW;
+  for (int64_t base = (gtid >> 6) * 64; base < span; base += nwaves * 64) {
+    const int64_t k = base + ln;
+    int64_t i = 0;
     uint64_t f = 0;
     int od = 0, rb = 0;
-    if (i < nW) {
+    if (k < span) {
+      i = nlist >= 0 ? (int64_t)lst[k] : k;
       f = Fc[i];
       if (f) {
         const int u = (int)(i % p.n);
@@ -438,6 +456,7 @@ __device__ __forceinline__ void batch_push(const BatchParams& p, const uint64_t*
     const uint32_t incl = wave_incl_scan((uint32_t)od);
     const int total = (int)__shfl(incl, 63);
     if (total == 0) continue;
+    wave_lds_sync();
     L.excl[ln] = (int)incl - od;
     L.beg[ln] = rb;
     L.u[ln] = (int)(i % p.n);
@@ -446,13 +465,15 @@ __device__ __forceinline__ void batch_push(const BatchParams& p, const uint64_t*
     wave_lds_sync();
     for (int e0 = 0; e0 < total; e0 += 64) {
       const int e = e0 + ln;
+      bool nw = false;
+      int v = 0, o = 0;
       if (e < total) {
-        const int o = owner_of(L.excl, e);
-        const int v = p.JR[L.beg[o] + (e - L.excl[o])];
-        push_edge<W, ADD, CT>(p, Fc, Fn, sd, d, L.u[o], v, L.w[o], L.f[o], pov, a);
+        o = owner_of(L.excl, e);
+        v = p.JR[L.beg[o] + (e - L.excl[o])];
+        nw = push_edge<W, CT>(p, Fc, Fn, sd, d, L.u[o], v, L.w[o], L.f[o], pov, a);
       }
+      list_append(nlst, ntail, nw, (uint32_t)((int64_t)L.w[o] * p.n + v));
     }
-    wave_lds_sync();
   }
   const int64_t hubs = (int64_t)p.nbigo * W;
   for (int64_t t = gtid >> 6; t < hubs; t += nwaves) {
@@ -463,8 +484,16 @@ __device__ __forceinline__ void batch_push(const BatchParams& p, const uint64_t*
     if (!f) continue;
     const int kb = p.RP[u] + (int)(e >> 32) * kPullChunk;
     const int ke = min(p.RP[u + 1], kb + kPullChunk);
-    for (int k = kb + ln; k < ke; k += 64)
-      push_edge<W, ADD, CT>(p, Fc, Fn, sd, d, u, p.JR[k], w, f, pov, a);
+    for (int k0 = kb; k0 < ke; k0 += 64) {
+      const int k = k0 + ln;
+      bool nw = false;
+      int v = 0;
+      if (k < ke) {
+        v = p.JR[k];
+        nw = push_edge<W, CT>(p, Fc, Fn, sd, d, u, v, w, f, pov, a);
+      }
+      list_append(nlst, ntail, nw, (uint32_t)((int64_t)w * p.n + v));
+    }
   }
 }
 
@@ -518,6 +547,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int3
     nx.pushing = push ? 1 : 0;
     nx.push_levels += push ? 1 : 0;
     nx.pull_levels += push ? 0 : 1;
+    nx.list_cur = push ? 1 : 0;  // a push level lists the words it makes nonzero
+    nx.list_prev = st.list_cur;
+    nx.q_prev = (uint32_t)q;
     p.loop[(d + 1) & 1] = nx;
   }
 
@@ -529,13 +561,28 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int3
 
   // housekeeping for level d + 1: seen through depth d, clear depth d - 1
   // (slot (d+1)&1 holds the OR through depth d-2: unchanged where neither
-  // frontier d-1 nor frontier d has a bit)
-  const int64_t nw = (int64_t)p.n * W;
-  for (int64_t i = gtid; i < nw; i += gstride) {
-    const uint64_t fc = Fc[i], fp = Fp[i];
-    if (fc | fp) {
-      sn[i] = sd[i] | fc;
-      if (fp) Fp[i] = 0ull;
+  // frontier d-1 nor frontier d has a bit); from the word lists when both are
+  // complete, else over every word
+  if (st.list_cur && st.list_prev) {
+    const uint32_t* lp = p.lst[(d + 2) % 3];
+    const uint32_t* lc = p.lst[d % 3];
+    for (int64_t k = gtid; k < (int64_t)st.q_prev; k += gstride) {
+      const uint32_t i = lp[k];
+      sn[i] = sd[i] | Fc[i];
+      Fp[i] = 0ull;
+    }
+    for (int64_t k = gtid; k < (int64_t)q; k += gstride) {
+      const uint32_t i = lc[k];
+      sn[i] = sd[i] | Fc[i];
+    }
+  } else {
+    const int64_t nw = (int64_t)p.n * W;
+    for (int64_t i = gtid; i < nw; i += gstride) {
+      const uint64_t fc = Fc[i], fp = Fp[i];
+      if (fc | fp) {
+        sn[i] = sd[i] | fc;
+        if (fp) Fp[i] = 0ull;
+      }
     }
   }
   if (gtid < kStripes) {
@@ -545,30 +592,17 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int3
     c.edges = 0;
     c.mout = 0;
     c.ovf = 0;
+    c.tail = 0;
   }
 
   BAcc a;
   if (push)
-    batch_push<W, false, CT>(p, Fc, Fn, sd, d, false, gtid, gstride, lds[threadIdx.x >> 6], a);
+    batch_push<W, CT>(p, Fc, Fn, sd, d, st.ovf_any != 0, p.lst[d % 3],
+                             st.list_cur ? (int64_t)q : -1, p.lst[(d + 1) % 3],
+                             &p.cset[(d + 1) % 3].s[0].tail, gtid, gstride,
+                             lds[threadIdx.x >> 6], a);
   else
     batch_pull<W, CT>(p, d, Fc, Fn, sd, st.ovf_any != 0, gtid, gstride, a);
-  flush_acc(p.cset[(d + 1) % 3].s[blockIdx.x & (kStripes - 1)], a);
-}
-
-// After a push level d: every edge of the frontier adds its count into the
-// pairs the level marked (their counts were zeroed by the marking lanes).
-// Nothing to do after a pull level.
-template <int W, typename CT>
-__global__ __launch_bounds__(kBatchBlock) void k_batch_scatter(BatchParams p, int32_t d) {
-  __shared__ PushLds lds[kBatchBlock / 64];
-  const BLoop nx = p.loop[(d + 1) & 1];
-  if (nx.done || nx.d != d + 1 || !nx.pushing) return;
-  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-  // seen[d & 1] and F[d % 3] are still those of level d (level d + 1 has not run)
-  BAcc a;
-  batch_push<W, true, CT>(p, p.F[d % 3], p.F[(d + 1) % 3], p.seen[d & 1], d, nx.ovf_any != 0, gtid,
-                      gstride, lds[threadIdx.x >> 6], a);
   flush_acc(p.cset[(d + 1) % 3].s[blockIdx.x & (kStripes - 1)], a);
 }
 
@@ -622,6 +656,7 @@ struct tbfs_batch {
   hipEvent_t ev0 = nullptr;    // start of the current batch
   hipEvent_t ev_step = nullptr;  // end of the work enqueued so far
   uint64_t* F[3] = {nullptr, nullptr, nullptr};
+  uint32_t* lst[3] = {nullptr, nullptr, nullptr};
   uint64_t* seen[2] = {nullptr, nullptr};
   uint64_t* ovfw = nullptr;
   uint64_t* sigma = nullptr;  // viewed as uint32_t counts by 32-bit runs
@@ -665,6 +700,7 @@ void batch_free(tbfs_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (int i = 0; i < 3; ++i) dev_free(b->F[i]);
+  for (int i = 0; i < 3; ++i) dev_free(b->lst[i]);
   for (int i = 0; i < 2; ++i) dev_free(b->seen[i]);
   dev_free(b->ovfw);
   dev_free(b->sigma);
@@ -684,7 +720,8 @@ void batch_free(tbfs_batch* b) {
 }
 
 int batch_create_words(int64_t cap_words, tbfs_batch** out) {
-  if (!out || cap_words <= 0) return TBFS_ERR_INVALID;
+  // word lists hold 32-bit word indices
+  if (!out || cap_words <= 0 || cap_words > (int64_t)UINT32_MAX) return TBFS_ERR_INVALID;
   *out = nullptr;
   tbfs_batch* b = new (std::nothrow) tbfs_batch();
   if (!b) return TBFS_ERR_NOMEM;
@@ -704,6 +741,7 @@ int batch_create_words(int64_t cap_words, tbfs_batch** out) {
     return rc;
   }
   for (int i = 0; i < 3 && !rc; ++i) rc = dev_alloc(&b->F[i], nw);
+  for (int i = 0; i < 3 && !rc; ++i) rc = dev_alloc(&b->lst[i], nw);
   for (int i = 0; i < 2 && !rc; ++i) rc = dev_alloc(&b->seen[i], nw);
   if (!rc) rc = dev_alloc(&b->ovfw, nw);
   if (!rc) rc = dev_alloc(&b->sigma, nb);
@@ -777,6 +815,7 @@ BatchParams make_bparams(const tbfs_batch* b) {
   p.nbigo = g->nbigo;
   p.alpha = b->o.batch_alpha > 0 ? b->o.batch_alpha : kBatchAlpha;
   for (int i = 0; i < 3; ++i) p.F[i] = b->F[i];
+  for (int i = 0; i < 3; ++i) p.lst[i] = b->lst[i];
   for (int i = 0; i < 2; ++i) p.seen[i] = b->seen[i];
   p.ovfw = b->ovfw;
   p.sigma = b->sigma;
@@ -811,10 +850,13 @@ int enqueue_w(tbfs_batch* b, int d0, int levels) {
     hipLaunchKernelGGL((k_batch_seed<W, CT>), dim3((p.nsrc + 255) / 256), dim3(256), 0, s, p);
     HIPCHK(hipGetLastError());
   }
+  // A BFS still going after kBatchWideLevels levels is in a long thin tail
+  // (high-diameter graph): its levels run on one workgroup per CU, where a
+  // launch costs far less than a full-occupancy grid (any grid is correct).
+  const int narrow = std::max(1, b->cus);
   for (int d = d0; d < d0 + levels; ++d) {
-    hipLaunchKernelGGL((k_batch_level<W, CT>), dim3(b->grid), dim3(kBatchBlock), 0, s, p, d);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL((k_batch_scatter<W, CT>), dim3(b->grid), dim3(kBatchBlock), 0, s, p, d);
+    const int gd = d < kBatchWideLevels ? b->grid : std::min(b->grid, narrow);
+    hipLaunchKernelGGL((k_batch_level<W, CT>), dim3(gd), dim3(kBatchBlock), 0, s, p, d);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(b->stats_h, b->stats, sizeof(RunStats), hipMemcpyDeviceToHost, s));

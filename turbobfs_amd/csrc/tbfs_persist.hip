@@ -151,6 +151,33 @@ __device__ __forceinline__ void push_edge(const GraphView& g, const StateView& s
 // frontier words are dealt to wavefronts round-robin; a wavefront loads 64 of
 // its words at once, and for each nonzero one walks the out-lists of its set
 // bits load-balanced, kUnroll edges per lane in flight.
+// Number of the words wave, wave + nwaves, ... below nwords.
+__device__ __forceinline__ int64_t nmine_of(int64_t nwords, int64_t wave, int64_t nwaves) {
+  return nwords > wave ? (nwords - wave + nwaves - 1) / nwaves : 0;
+}
+
+// Position of the r-th set bit (0-based) of x; x has more than r set bits.
+__device__ __forceinline__ int select_bit(uint64_t x, int r) {
+  int pos = 0;
+  int c = __popc((uint32_t)x);
+  if (r >= c) {
+    r -= c;
+    x >>= 32;
+    pos += 32;
+  }
+  uint32_t w = (uint32_t)x;
+  c = __popc(w & 0xffffu);
+  if (r >= c) { r -= c; w >>= 16; pos += 16; }
+  c = __popc(w & 0xffu);
+  if (r >= c) { r -= c; w >>= 8; pos += 8; }
+  c = __popc(w & 0xfu);
+  if (r >= c) { r -= c; w >>= 4; pos += 4; }
+  c = __popc(w & 0x3u);
+  if (r >= c) { r -= c; w >>= 2; pos += 2; }
+  if (r >= (int)(w & 1u)) pos += 1;
+  return pos;
+}
+
 __device__ __forceinline__ void push_lb_level(const GraphView& g,
                                               const StateView& st, int dn,
                                               uint32_t ovf_any,
@@ -189,21 +216,35 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
     }
   }
   const int64_t nwords = ((int64_t)g.n + 63) >> 6;
-  // this wave's words: wave, wave + nwaves, ...; their bits loaded 64 at once
-  const int64_t nmine = nwords > wave ? (nwords - wave + nwaves - 1) / nwaves : 0;
-  for (int64_t b = 0; b < nmine; b += 64) {
+  // this wave's words: wave, wave + nwaves, ...; their bits loaded 64 at once.
+  // The frontier vertices of those 64 words are numbered across the words
+  // (prefix sum of popcounts) and handed out one per lane, 64 at a time, so a
+  // sparse frontier costs one pass per 64 vertices, not one per word.
+  const int64_t nm = nmine_of(nwords, wave, nwaves);
+  for (int64_t b = 0; b < nm; b += 64) {
     const int64_t wl = wave + (b + lane()) * nwaves;
-    const uint64_t bits_l = (b + lane() < nmine) ? st.fcur[wl] : 0ull;
-    uint64_t todo = __ballot(bits_l != 0);
-    while (todo) {
-      const int j = __ffsll((unsigned long long)todo) - 1;
-      todo &= todo - 1;
-      const uint64_t bits = __shfl(bits_l, j);
-      const int u = (int)((wave + (b + j) * nwaves) * 64) + lane();
+    const uint64_t bits_l = (b + lane() < nm) ? st.fcur[wl] : 0ull;
+    const uint32_t cnt = (uint32_t)__popcll(bits_l);
+    const uint32_t wincl = wave_incl_scan(cnt);
+    const int nv = (int)__shfl(wincl, 63);
+    if (nv == 0) continue;
+    for (int k0 = 0; k0 < nv; k0 += 64) {
+      // vertex k0 + lane: its word (owner lane) and its bit within the word
+      wave_lds_sync();
+      L->excl[lane()] = (int)(wincl - cnt);
+      L->x[lane()] = bits_l;
+      wave_lds_sync();
+      int u = -1;
+      const int k = k0 + lane();
+      if (k < nv) {
+        const int j = owner_of(L->excl, k);
+        const int bit = select_bit(L->x[j], k - L->excl[j]);
+        u = (int)((wave + (b + j) * nwaves) * 64) + bit;
+      }
       int beg = 0, deg = 0;
       uint64_t x = 0;
       uint32_t ou = 0;
-      if ((bits >> lane()) & 1ull) {
+      if (u >= 0) {
         beg = g.RP[u];
         const int dg = g.RP[u + 1] - beg;
         if (dg <= kBigIn) {

@@ -39,8 +39,9 @@ using namespace tbfs;
 namespace {
 
 // Device bytes per 64-source word of one vertex in a batch slot: frontier x3,
-// seen x2, overflow bits (8 bytes each) + 64 depths (4) + 64 counts (8).
-constexpr int64_t kSlotBytesPerWord = 6 * 8 + 64 * (4 + 8);
+// seen x2, overflow bits (8 bytes each), word lists x3 (4 bytes each), 64
+// depths (4) + 64 counts (8).
+constexpr int64_t kSlotBytesPerWord = 6 * 8 + 3 * 4 + 64 * (4 + 8);
 
 struct Query {
   uint64_t ticket = 0;
