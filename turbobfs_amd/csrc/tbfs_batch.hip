@@ -167,9 +167,10 @@ __global__ __launch_bounds__(256) void k_batch_init(BatchParams p) {
   }
   // every count starts at zero: push levels and split in-lists add into them
   // atomically (pull levels on single-slice vertices store their sums)
-  CT* sg = (CT*)p.sigma;
-  const int64_t ncounts = nw * 64;
-  for (int64_t i = gtid; i < ncounts; i += gstride) sg[i] = 0;
+  // (16-byte stores: 64 counts per word make the count array a multiple of 16)
+  uint4* sg = (uint4*)p.sigma;
+  const int64_t n16 = nw * 64 * (int64_t)sizeof(CT) / 16;
+  for (int64_t i = gtid; i < n16; i += gstride) sg[i] = make_uint4(0u, 0u, 0u, 0u);
   const int nctr = (int)(3 * sizeof(BCtrSet) / sizeof(unsigned long long));
   unsigned long long* cw = reinterpret_cast<unsigned long long*>(p.cset);
   for (int64_t i = gtid; i < nctr; i += gstride) cw[i] = 0ull;
