@@ -63,8 +63,11 @@ extern "C" {
                                   each decides its direction from the device
                                   counters, the host checks for the end once
                                   per chunk of levels */
-/* PERSISTENT and CHAINED apply to AUTO / PUSH with TBFS_PULL_LB; other
- * combinations run the host loop. */
+#define TBFS_LOOP_GRAPH 3      /* the chained kernels of one BFS (init,
+                                  root, a budget of levels) captured once as
+                                  a HIP graph and replayed per source */
+/* PERSISTENT, CHAINED and GRAPH apply to AUTO / PUSH with TBFS_PULL_LB;
+ * other combinations run the host loop. */
 
 typedef struct tbfs_graph tbfs_graph;
 typedef struct tbfs_ctx tbfs_ctx;

@@ -111,6 +111,8 @@ MODES = [
     ("chain_push", dict(mode=L.MODE_PUSH, pull=L.PULL_LB, loop=L.LOOP_CHAINED)),
     ("chain_pull_depth_only", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED,
                                    alpha=1e9, sigma=0)),
+    ("graph_auto", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_GRAPH)),
+    ("graph_push", dict(mode=L.MODE_PUSH, pull=L.PULL_LB, loop=L.LOOP_GRAPH)),
 ]
 
 
@@ -183,7 +185,7 @@ def test_hub_columns_split_across_slices(gpu):
     _check_modes(g, [base], symmetric=False)
 
 
-@pytest.mark.parametrize("loop", [L.LOOP_PERSISTENT, L.LOOP_CHAINED, L.LOOP_HOST])
+@pytest.mark.parametrize("loop", [L.LOOP_PERSISTENT, L.LOOP_CHAINED, L.LOOP_HOST, L.LOOP_GRAPH])
 def test_run_many_matches_single(gpu, loop):
     g = graphs.rmat(12, 16, seed=7)
     roots = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 6, seed=1)

@@ -37,6 +37,7 @@ PULL_LB = 2
 LOOP_HOST = 0
 LOOP_PERSISTENT = 1
 LOOP_CHAINED = 2
+LOOP_GRAPH = 3
 
 _i32p = C.POINTER(C.c_int32)
 _u64p = C.POINTER(C.c_uint64)

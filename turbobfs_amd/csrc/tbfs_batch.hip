@@ -241,9 +241,9 @@ __device__ __forceinline__ void list_append(uint32_t* lst, unsigned int* tail, b
   if (flag) lst[base + __popcll(m & lanemask_lt())] = iw;
 }
 
-// Pull over (word, slice) items in word-major order: the waves in flight
-// share one word, so the caches hold that word's frontier array and the count
-// segments of its hubs.  A wavefront first tests 64 consecutive items at once
+// Pull over (word, slice) items, grouped by word: the waves working on a word
+// share its frontier array and the count segments of its hubs in the caches.
+// A wavefront first tests 64 consecutive items at once
 // (one per lane), then expands the active ones (some source of the word
 // still unseen at v) in turn: it discovers those sources (next frontier
 // word) and writes their depths and counts.
@@ -253,14 +253,40 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
                                            int64_t gtid, int64_t gstride, BAcc& a) {
   const int ln = lane();
   const int64_t nwaves = gstride >> 6;
-  const int64_t items = (int64_t)p.nslices * W;
-  for (int64_t base = (gtid >> 6) * 64; base < items; base += nwaves * 64) {
-    const int64_t t = base + ln;
-    int w = 0, v = 0, k0 = 0, k1 = 0, deg = 0;
+  // chunks of 64 items of one word.  With the grid a multiple of 8 the
+  // chunks follow the XCDs (workgroup b runs on XCD b % 8): XCD x takes the
+  // words w = x (mod 8) one after another, or shares word x % W with the
+  // other XCDs of that residue when W < 8, so each XCD's L2 holds the
+  // frontier array and hot count segments of its own word only.
+  const int64_t cpw = ((int64_t)p.nslices + 63) >> 6;
+  const bool xm = (gridDim.x & 7) == 0;
+  const int xcd = blockIdx.x & 7;
+  const int64_t wpb = blockDim.x >> 6;
+  const int nsh = W >= 8 ? 1 : 8 / W;
+  const int64_t nloc = !xm ? (int64_t)W * cpw
+                           : (W >= 8 ? (int64_t)(W / 8) * cpw
+                                     : (cpw + nsh - 1 - xcd / W) / nsh);
+  const int64_t l0 = xm ? (int64_t)(blockIdx.x >> 3) * wpb + (threadIdx.x >> 6) : (gtid >> 6);
+  const int64_t ls = xm ? (int64_t)(gridDim.x >> 3) * wpb : nwaves;
+  for (int64_t li = l0; li < nloc; li += ls) {
+    int w;
+    int64_t c;
+    if (!xm) {
+      w = (int)(li / cpw);
+      c = li - (int64_t)w * cpw;
+    } else if (W >= 8) {
+      const int64_t q = li / cpw;
+      w = xcd + 8 * (int)q;
+      c = li - q * cpw;
+    } else {
+      w = xcd % W;
+      c = li * nsh + xcd / W;
+    }
+    const int64_t jj = c * 64 + ln;  // slice index within word w
+    int v = 0, k0 = 0, k1 = 0, deg = 0;
     uint64_t tgt = 0;
-    if (t < items) {
-      w = (int)(t / p.nslices);
-      const uint64_t e = p.slices[t - (int64_t)w * p.nslices];
+    if (jj < p.nslices) {
+      const uint64_t e = p.slices[jj];
       v = (int)(uint32_t)e;
       const int beg0 = p.CP[v], end0 = p.CP[v + 1];
       k0 = beg0 + (int)(e >> 32) * kBatchSlice;
@@ -273,7 +299,7 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
     while (live) {
       const int src = __builtin_ctzll(live);
       live &= live - 1;
-      const int wi = __builtin_amdgcn_readlane(w, src);
+      const int wi = w;
       const int vi = __builtin_amdgcn_readlane(v, src);
       const int ka = __builtin_amdgcn_readlane(k0, src);
       const int kb = __builtin_amdgcn_readlane(k1, src);
@@ -835,14 +861,15 @@ int enqueue_w(tbfs_batch* b, int d0, int levels) {
   const BatchParams p = make_bparams(b);
   hipStream_t s = b->stream;
   if (b->grid == 0) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_level<W, CT>, kBatchBlock,
-                                                     0) != hipSuccess ||
-        per_cu < 1) {
-      (void)hipGetLastError();
-      per_cu = 4;
-    }
-    b->grid = b->o.grid > 0 ? std::min(b->o.grid, 65536) : std::max(1, b->cus) * per_cu;
+    // about two 64-item chunks per wavefront of the pull (more workgroups
+    // than fit at once: those that finish early make room for the rest),
+    // between one and 32 workgroups per CU, a multiple of 8 (XCD mapping)
+    const int64_t chunks = (int64_t)W * (((int64_t)p.nslices + 63) >> 6);
+    const int64_t cus = std::max(1, b->cus);
+    int64_t gw = (chunks + 2 * (kBatchBlock / 64) - 1) / (2 * (kBatchBlock / 64));
+    gw = std::min(std::max(gw, cus), cus * 32);
+    gw = (gw + 7) & ~(int64_t)7;
+    b->grid = b->o.grid > 0 ? std::min(b->o.grid, 65536) : (int)gw;
   }
   if (d0 == 0) {
     const int igrid = grid_for((int64_t)p.n * W, 256, std::max(1, b->cus) * 8);

@@ -63,6 +63,15 @@ struct tbfs_ctx {
   int32_t last_n = 0;
   int32_t* roots_d = nullptr;       // roots of a multi-root BFS
   int32_t roots_cap = 0;
+  // graph loop: the captured chunk and what it was captured with
+  hipGraphExec_t gexec = nullptr;
+  int gK = 0;
+  int ggrid = 0;
+  size_t gfbl = 0;
+  tbfs::PersistParams gparams{};
+  tbfs::RunStats* gstats = nullptr;  // the chunk's stats slot (device)
+  int32_t* groots_h = nullptr;       // pinned roots, one per queued source
+  int groots_cap = 0;
 };
 
 namespace tbfs {
@@ -112,6 +121,13 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots,
                  int count, const tbfs_opts& o, tbfs_stats* st);
 int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots,
               const tbfs_opts& o, tbfs_stats* st);
+
+// asynchronous single-source runs (query server)
+int ctx_async_begin(tbfs_ctx* c, int count);
+int ctx_async_enqueue(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs_opts& o,
+                      int idx, int32_t* S, uint64_t* sigma, uint8_t* ovf);
+int ctx_async_finish(tbfs_ctx* c, int count);
+int ctx_async_poll(tbfs_ctx* c, int count, bool* ready, std::vector<int>* unfinished);
 
 // batched sources (tbfs_batch.hip)
 int batch_words_for(int32_t count);

@@ -341,6 +341,9 @@ void ctx_free(tbfs_ctx* c) {
   dev_free(c->loop);
   dev_free(c->cset);
   dev_free(c->roots_d);
+  dev_free(c->gstats);
+  if (c->groots_h) (void)hipHostFree(c->groots_h);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->cset_h) (void)hipHostFree(c->cset_h);
   if (c->loop_h) (void)hipHostFree(c->loop_h);
   if (c->ctr_h) (void)hipHostFree(c->ctr_h);
@@ -459,7 +462,7 @@ static int check_run(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   if (root < 0 || root >= g->n) return TBFS_ERR_INVALID;
   if (o.mode < TBFS_MODE_AUTO || o.mode > TBFS_MODE_COOC) return TBFS_ERR_INVALID;
   if (o.pull < TBFS_PULL_SC || o.pull > TBFS_PULL_LB) return TBFS_ERR_INVALID;
-  if (o.loop < TBFS_LOOP_HOST || o.loop > TBFS_LOOP_CHAINED) return TBFS_ERR_INVALID;
+  if (o.loop < TBFS_LOOP_HOST || o.loop > TBFS_LOOP_GRAPH) return TBFS_ERR_INVALID;
   if (o.mode != TBFS_MODE_COOC && !g->CP) return TBFS_ERR_UNSUPPORTED;
   if (o.mode == TBFS_MODE_COOC && !o.sigma) return TBFS_ERR_UNSUPPORTED;
   if (o.block != 0 && (o.block % 64 != 0 || o.block > 1024 || o.block < 64))
@@ -481,6 +484,7 @@ static int pick_loop(const tbfs_ctx* c, const tbfs_opts& o) {
     const int gmax = o.block == 1024 ? c->persist_grid_max1024 : c->persist_grid_max;
     return gmax > 0 ? TBFS_LOOP_PERSISTENT : TBFS_LOOP_CHAINED;
   }
+  if (o.loop == TBFS_LOOP_GRAPH) return TBFS_LOOP_GRAPH;
   return TBFS_LOOP_CHAINED;
 }
 
@@ -542,6 +546,16 @@ static size_t chain_fbl(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts&
   return (bytes + 15) & ~(size_t)15;
 }
 
+// Levels past kChainWideLevels belong to a long thin tail (high-diameter
+// graph): without a staged bitmap they run on one workgroup per CU, where a
+// level costs ~10 us instead of ~16 us on the full grid (any grid is correct).
+constexpr int kChainWideLevels = 16;
+
+static int level_grid(const tbfs_ctx* c, int grid, size_t fbl, int d) {
+  if (fbl || d < kChainWideLevels) return grid;
+  return std::min(grid, std::max(1, c->cus));
+}
+
 static int chain_grid(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o) {
   if (chain_fbl(c, g, o)) return o.grid > 0 ? std::min(o.grid, 65536) : c->cus;
   const int64_t nwords = ((int64_t)g->n + 63) / 64;
@@ -561,7 +575,8 @@ static int enqueue_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   const size_t fbl = chain_fbl(c, g, o);
   hipStream_t s = c->stream;
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
-  for (int d = 0; d < levels; ++d) HIPCHK(kern::chain_level(s, grid, p, d, fbl));
+  for (int d = 0; d < levels; ++d)
+    HIPCHK(kern::chain_level(s, o.grid > 0 ? grid : level_grid(c, grid, fbl, d), p, d, fbl));
   return TBFS_OK;
 }
 
@@ -576,7 +591,9 @@ static int run_chained_p(tbfs_ctx* c, const tbfs_graph* g, const PersistParams& 
   if (p.nroots > 0) HIPCHK(kern::chain_seed(s, p));
   int32_t d = 0;
   for (;;) {
-    for (int i = 0; i < c->chain_k; ++i) HIPCHK(kern::chain_level(s, grid, p, d + i, fbl));
+    for (int i = 0; i < c->chain_k; ++i)
+      HIPCHK(kern::chain_level(s, o.grid > 0 ? grid : level_grid(c, grid, fbl, d + i), p, d + i,
+                               fbl));
     d += c->chain_k;
     HIPCHK(hipMemcpyAsync(c->loop_h, c->loop + (d & 1), sizeof(LoopState),
                           hipMemcpyDeviceToHost, s));
@@ -744,6 +761,70 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
   return TBFS_OK;
 }
 
+// Graph loop: init + seed + K chained level kernels captured once as a HIP
+// graph (the root is read from device memory, the level kernels take their
+// depth from the loop state), then replayed per source.  The capture is kept
+// until the graph, the options or the level budget change.
+static int graph_chunk(tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o, int K) {
+  PersistParams p = make_params(c, g, -1, o, c->gstats);
+  p.nroots = 1;
+  p.roots = c->roots_d;
+  const int grid = chain_grid(c, g, o);
+  const size_t fbl = chain_fbl(c, g, o);
+  if (c->gexec && c->gK == K && c->ggrid == grid && c->gfbl == fbl &&
+      std::memcmp(&p, &c->gparams, sizeof(p)) == 0)
+    return TBFS_OK;
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  hipStream_t s = c->stream;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipError_t e = kern::chain_init(s, grid_for(g->n, 256, 8192), p);
+  if (e == hipSuccess) e = kern::chain_seed(s, p);
+  for (int i = 0; i < K && e == hipSuccess; ++i)
+    e = kern::chain_level(s, o.grid > 0 ? grid : level_grid(c, grid, fbl, i), p, i, fbl);
+  hipGraph_t gr = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(s, &gr);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return hip_fail(e != hipSuccess ? e : e2, "level-loop capture", __FILE__, __LINE__);
+  }
+  e = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  if (e != hipSuccess) {
+    c->gexec = nullptr;
+    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  }
+  c->gK = K;
+  c->ggrid = grid;
+  c->gfbl = fbl;
+  c->gparams = p;
+  return TBFS_OK;
+}
+
+static int ctx_reserve_graph(tbfs_ctx* c, int count) {
+  if (!c->gstats) {
+    int rc = dev_alloc(&c->gstats, 1);
+    if (rc) return rc;
+  }
+  if (c->roots_cap < 1) {
+    int rc = dev_alloc(&c->roots_d, 1);
+    if (rc) return rc;
+    c->roots_cap = 1;
+  }
+  if (c->groots_cap < count) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->groots_h) (void)hipHostFree(c->groots_h);
+    c->groots_h = nullptr;
+    c->groots_cap = 0;
+    HIPCHK(hipHostMalloc((void**)&c->groots_h, (size_t)count * sizeof(int32_t),
+                         hipHostMallocDefault));
+    c->groots_cap = count;
+  }
+  return TBFS_OK;
+}
+
 int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
                  int count, const tbfs_opts& o, tbfs_stats* st) {
   tbfs_graph* g = const_cast<tbfs_graph*>(gc);
@@ -770,8 +851,23 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
   // chain_k kernels each; a source that needs more is found afterwards from
   // its stats slot and run again with host checks.
   const int budget = c->chain_k;
+  if (loop == TBFS_LOOP_GRAPH) {
+    rc = ctx_reserve_graph(c, count);
+    if (!rc) rc = graph_chunk(c, g, o, (budget + 1) & ~1);
+    if (rc) return rc;
+  }
   HIPCHK(hipEventRecord(c->evs[0], s));
   for (int i = 0; i < count; ++i) {
+    if (loop == TBFS_LOOP_GRAPH) {
+      c->groots_h[i] = roots[i];
+      HIPCHK(hipMemcpyAsync(c->roots_d, c->groots_h + i, sizeof(int32_t),
+                            hipMemcpyHostToDevice, s));
+      HIPCHK(hipGraphLaunch(c->gexec, s));
+      HIPCHK(hipMemcpyAsync(c->stats + i, c->gstats, sizeof(RunStats),
+                            hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipEventRecord(c->evs[i + 1], s));
+      continue;
+    }
     if (loop == TBFS_LOOP_PERSISTENT) {
       bool refused = false;
       rc = enqueue_persistent(c, g, roots[i], o, c->stats + i, &refused);
@@ -787,7 +883,7 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
   std::vector<float> ms((size_t)count, 0.f);
   for (int i = 0; i < count; ++i)
     HIPCHK(hipEventElapsedTime(&ms[i], c->evs[i], c->evs[i + 1]));
-  if (loop == TBFS_LOOP_CHAINED) {
+  if (loop == TBFS_LOOP_CHAINED || loop == TBFS_LOOP_GRAPH) {
     bool redo_last = false;
     int maxlev = 0;
     for (int i = 0; i < count; ++i) {
@@ -838,6 +934,9 @@ int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots
   if (c->roots_cap < nroots) {
     HIPCHK(hipStreamSynchronize(c->stream));
     dev_free(c->roots_d);
+  dev_free(c->gstats);
+  if (c->groots_h) (void)hipHostFree(c->groots_h);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     c->roots_cap = 0;
     rc = dev_alloc(&c->roots_d, (size_t)nroots);
     if (rc) return rc;
@@ -859,6 +958,57 @@ int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_n = g->n;
   if (st) fill_stats(st, c->stats_h[0], ms);
+  return TBFS_OK;
+}
+
+// Asynchronous single-source BFS for the query server: each source is a
+// chained BFS (level budget chain_k) followed by the copy of its results to
+// host buffers, all on the context's stream; ctx_async_finish queues the
+// stats copy and an event, ctx_async_poll reports when that event has passed
+// and which sources ran out of levels (they are enqueued again by the caller
+// after the budget grew).
+int ctx_async_begin(tbfs_ctx* c, int count) { return ctx_reserve_stats(c, count); }
+
+int ctx_async_enqueue(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs_opts& o,
+                      int idx, int32_t* S, uint64_t* sigma, uint8_t* ovf) {
+  if (idx < 0 || idx >= c->stats_cap) return TBFS_ERR_INVALID;
+  int rc = check_run(c, g, root, o);
+  if (rc) return rc;
+  rc = enqueue_chained(c, g, root, o, c->stats + idx, c->chain_k);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  const size_t n = (size_t)g->n;
+  if (S) HIPCHK(hipMemcpyAsync(S, c->S, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (sigma) HIPCHK(hipMemcpyAsync(sigma, c->sigma, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  if (ovf) HIPCHK(hipMemcpyAsync(ovf, c->ovf, n, hipMemcpyDeviceToHost, s));
+  c->last_n = g->n;
+  return TBFS_OK;
+}
+
+int ctx_async_finish(tbfs_ctx* c, int count) {
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(c->stats_h, c->stats, (size_t)count * sizeof(RunStats),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(c->ev1, s));
+  return TBFS_OK;
+}
+
+int ctx_async_poll(tbfs_ctx* c, int count, bool* ready, std::vector<int>* unfinished) {
+  *ready = false;
+  const hipError_t e = hipEventQuery(c->ev1);
+  if (e == hipErrorNotReady) return TBFS_OK;
+  if (e != hipSuccess) return hip_fail(e, "hipEventQuery", __FILE__, __LINE__);
+  *ready = true;
+  unfinished->clear();
+  int maxlev = 0;
+  for (int i = 0; i < count; ++i) {
+    if (c->stats_h[i].levels < 0) unfinished->push_back(i);
+    maxlev = std::max(maxlev, c->stats_h[i].levels);
+  }
+  // more levels per source from now on: the deepest seen plus one spare, or
+  // twice the old budget when a source did not finish
+  const int want = unfinished->empty() ? maxlev + 2 : 2 * c->chain_k;
+  c->chain_k = std::max(c->chain_k, std::min(1 << 16, want));
   return TBFS_OK;
 }
 

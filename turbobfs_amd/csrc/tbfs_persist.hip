@@ -677,28 +677,32 @@ __global__ __launch_bounds__(256) void k_chain_seed(PersistParams p) {
 // Chained loop: level d.  Reads slot d & 1 and the counters of depth d (both
 // final: written by earlier kernels), writes slot (d + 1) & 1.  FBL: pull
 // levels stage the frontier bitmap in dynamic LDS (one workgroup per CU).
+// `step` only selects the state slot (its parity equals the depth's): the
+// depth itself comes from the state, so one captured chunk of level kernels
+// serves every stretch of levels starting at an even depth.
 template <int BLOCK, bool FBL>
-__global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t d) {
+__global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t step) {
   __shared__ WaveLds lds[BLOCK / 64];
   extern __shared__ uint64_t fb_dyn[];
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-  LoopState s = p.loop[d & 1];
+  LoopState s = p.loop[step & 1];
   if (s.done) {
-    if (gtid == 0) p.loop[(d + 1) & 1] = s;
+    if (gtid == 0) p.loop[(step + 1) & 1] = s;
     return;
   }
+  const int32_t d = s.d;
   const LevelCtr c = read_set(&p.cset[d % 3], false);
   if (!step_state(p, &c, s)) {
     if (gtid == 0) {
-      p.loop[(d + 1) & 1] = s;
+      p.loop[(step + 1) & 1] = s;
       write_stats(p, s);
     }
     return;
   }
   LoopState nx = s;
   nx.d = d + 1;
-  if (gtid == 0) p.loop[(d + 1) & 1] = nx;
+  if (gtid == 0) p.loop[(step + 1) & 1] = nx;
   LevelAcc acc;
   run_level(p, d, s.pushing != 0, s.ovf_any, gtid, gstride, &lds[threadIdx.x >> 6], acc,
             FBL ? fb_dyn : nullptr);

@@ -11,7 +11,10 @@
 //   - whenever a slot is idle it admits work: the graph whose oldest waiting
 //     query is oldest, with every query waiting for that graph up to the
 //     slot's capacity for it (the largest power-of-two number of 64-source
-//     words whose state fits the slot's memory, at most max_batch sources).
+//     words whose state fits the slot's memory, at most max_batch sources);
+//     those queries run as one batch, or one after another as single-source
+//     BFS on the slot's context when that costs less for so few queries (by
+//     the per-graph costs measured on earlier admissions).
 // Queries that arrive while slots are busy join the next batch formed for
 // their graph; slots run concurrently on their own streams, so batches of
 // different graphs (or of one graph) overlap on the device.  A query's
@@ -20,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -55,11 +59,14 @@ struct Query {
 
 struct Slot {
   tbfs_batch* b = nullptr;
-  int state = 0;  // 0 idle, 1 BFS running, 2 rows being copied
+  tbfs_ctx* c = nullptr;  // single-source runs (small batches)
+  int state = 0;  // 0 idle, 1 batch BFS running, 2 rows being copied, 3 single runs
   int graph = -1;
+  int words = 0;  // 64-source words of the running batch (0: single runs)
   double t_start = 0;
   std::vector<Query> q;
   std::vector<int32_t> roots;  // kept until the batch has finished
+  std::vector<int> redo;
 };
 
 }  // namespace
@@ -67,6 +74,11 @@ struct Slot {
 struct tbfs_server {
   std::vector<const tbfs_graph*> graphs;
   std::vector<int> cap;  // sources per batch for each graph
+  // measured cost per graph (ms, host clock, results included): one
+  // single-source query, and one batch of 1, 2, 4, 8, 16 words (0: unknown)
+  std::vector<double> t_single;
+  std::vector<std::array<double, 5>> t_batch;
+  int32_t nmax = 0;
   std::vector<std::deque<Query>> pending;
   std::deque<tbfs_query_done> completed;
   std::vector<Slot> slots;
@@ -89,7 +101,20 @@ struct tbfs_server {
 
 namespace {
 
+int word_index(int words) {
+  int k = 0;
+  while ((1 << k) < words) ++k;
+  return k;
+}
+
 void complete(tbfs_server* s, Slot& sl, int status, double t_done) {
+  if (status == TBFS_OK && sl.graph >= 0 && !sl.q.empty()) {
+    const double ms = (t_done - sl.t_start) * 1e-3;
+    double& t = sl.words ? s->t_batch[sl.graph][word_index(sl.words)]
+                         : s->t_single[sl.graph];
+    const double x = sl.words ? ms : ms / (double)sl.q.size();
+    t = t > 0 ? 0.7 * t + 0.3 * x : x;
+  }
   std::lock_guard<std::mutex> lk(s->mu);
   for (const Query& q : sl.q) {
     tbfs_query_done d{};
@@ -107,6 +132,7 @@ void complete(tbfs_server* s, Slot& sl, int status, double t_done) {
   sl.q.clear();
   sl.state = 0;
   sl.graph = -1;
+  sl.words = 0;
   s->cv_done.notify_all();
 }
 
@@ -143,13 +169,51 @@ bool step_slot(tbfs_server* s, Slot& sl) {
     complete(s, sl, e == hipSuccess ? TBFS_OK : TBFS_ERR_HIP, s->now_us());
     return true;
   }
+  if (sl.state == 3) {
+    bool ready = false;
+    int rc = ctx_async_poll(sl.c, (int)sl.q.size(), &ready, &sl.redo);
+    if (rc) {
+      complete(s, sl, rc, s->now_us());
+      return true;
+    }
+    if (!ready) return false;
+    if (sl.redo.empty()) {
+      complete(s, sl, TBFS_OK, s->now_us());
+      return true;
+    }
+    // sources that ran out of levels: again, with the larger budget
+    for (int i : sl.redo) {
+      const Query& q = sl.q[i];
+      rc = ctx_async_enqueue(sl.c, s->graphs[sl.graph], q.root, s->bopts, i, q.S, q.sigma, q.ovf);
+      if (rc) break;
+    }
+    if (!rc) rc = ctx_async_finish(sl.c, (int)sl.q.size());
+    if (rc) complete(s, sl, rc, s->now_us());
+    return true;
+  }
   return false;
+}
+
+// Single-source runs or one batch: single runs while k queries cost less
+// that way than one batch of their word count, by the costs measured so far
+// (a graph's batch cost is measured first, then one single query).
+bool use_single(tbfs_server* s, int g, size_t* k) {
+  const int wi = word_index(batch_words_for((int32_t)*k));
+  const double tb = s->t_batch[g][wi];
+  const double t1 = s->t_single[g];
+  if (tb <= 0) return false;
+  if (t1 <= 0) {
+    *k = 1;
+    return true;
+  }
+  return (double)*k * t1 < tb;
 }
 
 // Form a batch for an idle slot; true if one was started.
 bool admit(tbfs_server* s, Slot& sl) {
   std::vector<Query> take;
   int g = -1;
+  bool single = false;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     double oldest = 0;
@@ -163,15 +227,39 @@ bool admit(tbfs_server* s, Slot& sl) {
     }
     if (g < 0) return false;
     std::deque<Query>& pq = s->pending[g];
-    const size_t k = std::min(pq.size(), (size_t)s->cap[g]);
+    size_t k = std::min(pq.size(), (size_t)s->cap[g]);
+    single = use_single(s, g, &k);
     take.assign(pq.begin(), pq.begin() + k);
     pq.erase(pq.begin(), pq.begin() + k);
+  }
+  if (single) {
+    sl.q = std::move(take);
+    sl.graph = g;
+    sl.words = 0;
+    sl.t_start = s->now_us();
+    int rc = TBFS_OK;
+    if (!sl.c) rc = tbfs_ctx_create(s->nmax, &sl.c);
+    if (!rc) rc = ctx_async_begin(sl.c, (int)sl.q.size());
+    for (size_t i = 0; i < sl.q.size() && !rc; ++i) {
+      const Query& q = sl.q[i];
+      rc = ctx_async_enqueue(sl.c, s->graphs[g], q.root, s->bopts, (int)i, q.S, q.sigma, q.ovf);
+    }
+    if (!rc) rc = ctx_async_finish(sl.c, (int)sl.q.size());
+    if (rc) {
+      complete(s, sl, rc, s->now_us());
+      return true;
+    }
+    sl.state = 3;
+    ++s->batches;
+    s->batched += sl.q.size();
+    return true;
   }
   std::vector<int32_t>& roots = sl.roots;
   roots.resize(take.size());
   for (size_t i = 0; i < take.size(); ++i) roots[i] = take[i].root;
   sl.q = std::move(take);
   sl.graph = g;
+  sl.words = batch_words_for((int32_t)roots.size());
   sl.t_start = s->now_us();
   const int rc = batch_start(sl.b, s->graphs[g], roots.data(), (int32_t)roots.size(), &s->bopts);
   if (rc) {
@@ -214,7 +302,10 @@ void server_free(tbfs_server* s) {
     s->cv_work.notify_all();
     s->th.join();
   }
-  for (Slot& sl : s->slots) batch_free(sl.b);
+  for (Slot& sl : s->slots) {
+    batch_free(sl.b);
+    if (sl.c) tbfs_ctx_destroy(sl.c);
+  }
   delete s;
 }
 
@@ -250,6 +341,9 @@ int tbfs_server_create(const tbfs_graph* const* graphs, int32_t ngraphs,
   s->t0 = std::chrono::steady_clock::now();
   s->graphs.assign(graphs, graphs + ngraphs);
   s->pending.resize(ngraphs);
+  s->t_single.assign(ngraphs, 0.0);
+  s->t_batch.assign(ngraphs, std::array<double, 5>{});
+  s->nmax = nmax;
   for (int i = 0; i < ngraphs; ++i) {
     int w = 1;
     while (w < 16 && (int64_t)graphs[i]->n * (w * 2) <= cap_words) w *= 2;
