@@ -53,6 +53,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -73,7 +74,8 @@ constexpr int kBatchSlice = 256;  // in-list entries per pull slice
 constexpr int kBatchBlock = 256;  // threads per workgroup (4 wavefronts)
 constexpr int kBatchMaxW = 16;    // words per vertex: up to 1024 sources
 constexpr float kBatchAlpha = 32.0f;
-constexpr int kBatchWideLevels = 16;  // levels on the full-occupancy grid  // default push / pull switch
+constexpr int kBatchWideLevels = 16;  // levels on the full-occupancy grid
+constexpr int kBatchLbDegree = 12;    // mean in-degree below which pulls share passes  // default push / pull switch
 
 // Level counters, striped over kStripes cache lines like the single-source
 // ones (a workgroup adds into stripe blockIdx & (kStripes - 1)).
@@ -120,6 +122,7 @@ struct BatchParams {
   const uint64_t* bigo;    // out-lists longer than kBigIn, in kPullChunk slices
   int32_t nbigo;
   float alpha;             // push while m_out * alpha < nz * W
+  int32_t pull_lb;         // pull with in-list entries of several items per pass
   uint64_t* F[3];
   uint32_t* lst[3];  // word lists of the frontiers (index w * n + v), rotating like F
   uint64_t* seen[2];
@@ -135,6 +138,12 @@ struct BatchParams {
 __device__ __forceinline__ uint64_t rl64(uint64_t x, int l) {
   const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, l);
   const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t rfl64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
   return ((uint64_t)hi << 32) | lo;
 }
 
@@ -241,14 +250,213 @@ __device__ __forceinline__ void list_append(uint32_t* lst, unsigned int* tail, b
   if (flag) lst[base + __popcll(m & lanemask_lt())] = iw;
 }
 
+// Write the discoveries of one pull item (vertex v, word w): the lanes are
+// the word's sources; `hit` marks the sources found at depth d + 1 with
+// count `acc`, `ovb` those whose count overflowed.
+template <int W, typename CT>
+__device__ __forceinline__ void pull_flush(const BatchParams& p, int32_t d, uint64_t* Fn, int w,
+                                           int v, int dg, CT acc, bool hit, bool ovb, BAcc& a) {
+  const int ln = lane();
+  const uint64_t nb = __ballot(hit);
+  if (nb == 0) return;
+  uint64_t ob = __ballot(ovb);
+  const int64_t iw = (int64_t)w * p.n + v;
+  const int64_t ix = iw * 64 + ln;
+  if (dg <= kBatchSlice) {  // the only slice of v: plain stores
+    if (hit) {
+      p.S[ix] = d + 1;
+      ((CT*)p.sigma)[ix] = acc;
+    }
+    if (ln == 0) {
+      Fn[iw] = nb;
+      if (ob) p.ovfw[iw] |= ob;
+      a.q += 1;
+      a.bits += (unsigned long long)__popcll(nb);
+      a.edges += (unsigned long long)__popcll(nb) * (unsigned long long)dg;
+      a.mout += (unsigned long long)(p.RP[v + 1] - p.RP[v]);
+    }
+  } else {  // one of several slices: partial sums added atomically
+    bool cy = false;
+    if (hit) {
+      const CT old = atomic_add_ct((CT*)p.sigma + ix, acc);
+      cy = (CT)(old + acc) < old;
+      p.S[ix] = d + 1;
+    }
+    ob |= __ballot(cy);
+    if (ln == 0) {
+      if (ob) atomicOr((unsigned long long*)&p.ovfw[iw], (unsigned long long)ob);
+      const unsigned long long old = atomicOr((unsigned long long*)&Fn[iw], (unsigned long long)nb);
+      const uint64_t newly = nb & ~(uint64_t)old;
+      a.bits += (unsigned long long)__popcll(newly);
+      a.edges += (unsigned long long)__popcll(newly) * (unsigned long long)dg;
+      if (old == 0) {
+        a.q += 1;
+        a.mout += (unsigned long long)(p.RP[v + 1] - p.RP[v]);
+      }
+    }
+  }
+  if (ob && ln == 0) a.ovf = 1;
+}
+
+struct PullLds {
+  int32_t excl[64];  // exclusive prefix of the live items' slice lengths
+  int32_t k0[64];    // first in-list entry of each item's slice
+  int32_t v[64];
+  int32_t deg[64];   // in-degree of the item's vertex
+  uint64_t tgt[64];  // its unseen sources
+  int32_t po[64];    // parent edges of one pass: item, parent, source mask
+  int32_t pu[64];
+  uint64_t pm[64];
+};
+
 // Pull over (word, slice) items, grouped by word: the waves working on a word
 // share its frontier array and the count segments of its hubs in the caches.
-// A wavefront first tests 64 consecutive items at once
-// (one per lane), then expands the active ones (some source of the word
-// still unseen at v) in turn: it discovers those sources (next frontier
-// word) and writes their depths and counts.
+// A wavefront takes 64 consecutive items of one word; the in-list entries of
+// the live ones (some source of the word still unseen at v) are spread over
+// the lanes by a prefix sum of their lengths, 64 entries per pass, so short
+// in-lists (grids, road networks) share a pass.  Entry u of item v: u's
+// frontier word AND v's unseen sources marks the sources for which u -> v is
+// a shortest-path edge into depth d + 1; those parent edges, kept in item
+// order, are then taken 16 at a time with the lanes as sources: all 16
+// count gathers are issued before any is added, and an item's discoveries
+// are written when the next item starts.
 template <int W, typename CT>
 __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, const uint64_t* Fc,
+                                           uint64_t* Fn, const uint64_t* sd, bool pov,
+                                           int64_t gtid, int64_t gstride, PullLds& L, BAcc& a) {
+  const int ln = lane();
+  const int64_t nwaves = gstride >> 6;
+  // chunks of 64 items of one word.  With the grid a multiple of 8 the
+  // chunks follow the XCDs (workgroup b runs on XCD b % 8): XCD x takes the
+  // words w = x (mod 8) one after another, or shares word x % W with the
+  // other XCDs of that residue when W < 8, so each XCD's L2 holds the
+  // frontier array and hot count segments of its own word only.
+  const int64_t cpw = ((int64_t)p.nslices + 63) >> 6;
+  const bool xm = (gridDim.x & 7) == 0;
+  const int xcd = blockIdx.x & 7;
+  const int64_t wpb = blockDim.x >> 6;
+  const int nsh = W >= 8 ? 1 : 8 / W;
+  const int64_t nloc = !xm ? (int64_t)W * cpw
+                           : (W >= 8 ? (int64_t)(W / 8) * cpw
+                                     : (cpw + nsh - 1 - xcd / W) / nsh);
+  const int64_t l0 = xm ? (int64_t)(blockIdx.x >> 3) * wpb + (threadIdx.x >> 6) : (gtid >> 6);
+  const int64_t ls = xm ? (int64_t)(gridDim.x >> 3) * wpb : nwaves;
+  constexpr int G = 8;  // fewer in flight than the one-item pull: registers
+  for (int64_t li = l0; li < nloc; li += ls) {
+    int w;
+    int64_t c;
+    if (!xm) {
+      w = (int)(li / cpw);
+      c = li - (int64_t)w * cpw;
+    } else if (W >= 8) {
+      const int64_t q = li / cpw;
+      w = xcd + 8 * (int)q;
+      c = li - q * cpw;
+    } else {
+      w = xcd % W;
+      c = li * nsh + xcd / W;
+    }
+    const int64_t jj = c * 64 + ln;  // slice index within word w
+    int v = 0, k0 = 0, len = 0, deg = 0;
+    uint64_t tgt = 0;
+    if (jj < p.nslices) {
+      const uint64_t e = p.slices[jj];
+      v = (int)(uint32_t)e;
+      const int beg0 = p.CP[v], end0 = p.CP[v + 1];
+      k0 = beg0 + (int)(e >> 32) * kBatchSlice;
+      deg = end0 - beg0;
+      const int64_t iw = (int64_t)w * p.n + v;
+      tgt = ~(sd[iw] | Fc[iw]);
+      if (tgt) len = min(end0, k0 + kBatchSlice) - k0;
+    }
+    const uint32_t incl = wave_incl_scan((uint32_t)len);
+    const int total = (int)__shfl(incl, 63);
+    if (total == 0) continue;
+    wave_lds_sync();
+    L.excl[ln] = (int)incl - len;
+    L.k0[ln] = k0;
+    L.v[ln] = v;
+    L.deg[ln] = deg;
+    L.tgt[ln] = tgt;
+    wave_lds_sync();
+    const CT* sgw = (const CT*)p.sigma + (int64_t)w * p.n * 64 + ln;
+    const uint64_t* Fcw = Fc + (int64_t)w * p.n;
+    const uint64_t* ovw = p.ovfw + (int64_t)w * p.n;
+    int cur = -1;  // item being accumulated (uniform)
+    CT acc = 0;
+    bool hit = false, ovb = false;
+    for (int e0 = 0; e0 < total; e0 += 64) {
+      const int e = e0 + ln;
+      int o = 0, u = 0;
+      uint64_t mm = 0;
+      if (e < total) {
+        o = owner_of(L.excl, e);
+        u = p.IC[L.k0[o] + (e - L.excl[o])];
+        mm = Fcw[u] & L.tgt[o];
+      }
+      const uint64_t bal = __ballot(mm != 0);
+      if (bal == 0) continue;
+      const int cnt = __popcll(bal);
+      wave_lds_sync();
+      if (mm) {
+        const int pos = __popcll(bal & lanemask_lt());
+        L.po[pos] = o;
+        L.pu[pos] = u;
+        L.pm[pos] = mm;
+      }
+      wave_lds_sync();
+      for (int g0 = 0; g0 < cnt; g0 += G) {
+        CT x[G];
+        int og[G];
+        uint32_t hm = 0, pm = 0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          x[g] = 0;
+          og[g] = -1;
+          if (g0 + g < cnt) {
+            og[g] = __builtin_amdgcn_readfirstlane(L.po[g0 + g]);
+            const int ue = __builtin_amdgcn_readfirstlane(L.pu[g0 + g]);
+            const uint64_t me = rfl64(L.pm[g0 + g]);
+            if ((me >> ln) & 1ull) {
+              x[g] = sgw[(int64_t)ue * 64];
+              hm |= 1u << g;
+              if (pov && ((ovw[ue] >> ln) & 1ull)) pm |= 1u << g;
+            }
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (og[g] < 0) break;
+          if (og[g] != cur) {
+            if (cur >= 0)
+              pull_flush<W, CT>(p, d, Fn, w, __builtin_amdgcn_readfirstlane(L.v[cur]),
+                                __builtin_amdgcn_readfirstlane(L.deg[cur]), acc, hit, ovb, a);
+            cur = og[g];
+            acc = 0;
+            hit = false;
+            ovb = false;
+          }
+          const CT y = acc + x[g];
+          if (y < acc) ovb = true;
+          acc = y;
+          hit |= (hm >> g) & 1u;
+          ovb |= (pm >> g) & 1u;
+        }
+      }
+    }
+    if (cur >= 0)
+      pull_flush<W, CT>(p, d, Fn, w, __builtin_amdgcn_readfirstlane(L.v[cur]),
+                        __builtin_amdgcn_readfirstlane(L.deg[cur]), acc, hit, ovb, a);
+  }
+}
+
+// Pull, one item at a time (graphs with long in-lists): a wavefront tests 64
+// consecutive items of one word at once (one per lane), then expands the
+// live ones in turn, each with its own pass over its in-list slice: one lane
+// per entry for the parent test, then one lane per source for the counts,
+// 16 gathers in flight.  Chunks are scheduled as in batch_pull.
+template <int W, typename CT>
+__device__ __forceinline__ void batch_pull_items(const BatchParams& p, int32_t d, const uint64_t* Fc,
                                            uint64_t* Fn, const uint64_t* sd, bool pov,
                                            int64_t gtid, int64_t gstride, BAcc& a) {
   const int ln = lane();
@@ -347,50 +555,7 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
           }
         }
       }
-      const uint64_t nb = __ballot(hit);
-      if (nb == 0) continue;
-      uint64_t ob = __ballot(ovb);
-      const int64_t iw = (int64_t)wi * p.n + vi;
-      const int64_t ix = iw * 64 + ln;
-      if (dg <= kBatchSlice) {  // the only slice of v: plain stores
-        if (hit) {
-          p.S[ix] = d + 1;
-          ((CT*)p.sigma)[ix] = acc;
-        }
-        if (ln == 0) {
-          Fn[iw] = nb;
-          if (ob) p.ovfw[iw] |= ob;
-        }
-        if (ln == 0) {
-          a.q += 1;
-          a.bits += (unsigned long long)__popcll(nb);
-          a.edges += (unsigned long long)__popcll(nb) * (unsigned long long)dg;
-          a.mout += (unsigned long long)(p.RP[vi + 1] - p.RP[vi]);
-        }
-      } else {  // one of several slices: partial sums into a zeroed row
-        bool cy = false;
-        if (hit) {
-          const CT old = atomic_add_ct((CT*)p.sigma + ix, acc);
-          cy = (CT)(old + acc) < old;
-          p.S[ix] = d + 1;
-        }
-        ob |= __ballot(cy);
-        if (ln == 0) {
-          if (ob) atomicOr((unsigned long long*)&p.ovfw[iw], (unsigned long long)ob);
-          {
-            const unsigned long long old =
-                atomicOr((unsigned long long*)&Fn[iw], (unsigned long long)nb);
-            const uint64_t newly = nb & ~(uint64_t)old;
-            a.bits += (unsigned long long)__popcll(newly);
-            a.edges += (unsigned long long)__popcll(newly) * (unsigned long long)dg;
-            if (old == 0) {
-              a.q += 1;
-              a.mout += (unsigned long long)(p.RP[vi + 1] - p.RP[vi]);
-            }
-          }
-        }
-      }
-      if (ob && ln == 0) a.ovf = 1;
+      pull_flush<W, CT>(p, d, Fn, wi, vi, dg, acc, hit, ovb, a);
     }
   }
 }
@@ -524,10 +689,14 @@ __device__ __forceinline__ void batch_push(const BatchParams& p, const uint64_t*
   }
 }
 
-// Level d: expand the frontier of depth d into depth d + 1.
-template <int W, typename CT>
+// Level d: expand the frontier of depth d into depth d + 1.  LB selects the
+// pull that shares passes between items (short in-lists).
+template <int W, typename CT, bool LB>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int32_t d) {
-  __shared__ PushLds lds[kBatchBlock / 64];
+  __shared__ union {
+    PushLds push;
+    typename std::conditional<LB, PullLds, PushLds>::type pull;
+  } lds[kBatchBlock / 64];
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
 
@@ -627,9 +796,12 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int3
     batch_push<W, CT>(p, Fc, Fn, sd, d, st.ovf_any != 0, p.lst[d % 3],
                              st.list_cur ? (int64_t)q : -1, p.lst[(d + 1) % 3],
                              &p.cset[(d + 1) % 3].s[0].tail, gtid, gstride,
-                             lds[threadIdx.x >> 6], a);
+                             lds[threadIdx.x >> 6].push, a);
+  else if constexpr (LB)
+    batch_pull<W, CT>(p, d, Fc, Fn, sd, st.ovf_any != 0, gtid, gstride,
+                      lds[threadIdx.x >> 6].pull, a);
   else
-    batch_pull<W, CT>(p, d, Fc, Fn, sd, st.ovf_any != 0, gtid, gstride, a);
+    batch_pull_items<W, CT>(p, d, Fc, Fn, sd, st.ovf_any != 0, gtid, gstride, a);
   flush_acc(p.cset[(d + 1) % 3].s[blockIdx.x & (kStripes - 1)], a);
 }
 
@@ -841,6 +1013,8 @@ BatchParams make_bparams(const tbfs_batch* b) {
   p.bigo = g->bigo;
   p.nbigo = g->nbigo;
   p.alpha = b->o.batch_alpha > 0 ? b->o.batch_alpha : kBatchAlpha;
+  // short in-lists share a pass; long ones pay off one item at a time
+  p.pull_lb = (int64_t)g->nz < (int64_t)kBatchLbDegree * g->n ? 1 : 0;
   for (int i = 0; i < 3; ++i) p.F[i] = b->F[i];
   for (int i = 0; i < 3; ++i) p.lst[i] = b->lst[i];
   for (int i = 0; i < 2; ++i) p.seen[i] = b->seen[i];
@@ -884,7 +1058,10 @@ int enqueue_w(tbfs_batch* b, int d0, int levels) {
   const int narrow = std::max(1, b->cus);
   for (int d = d0; d < d0 + levels; ++d) {
     const int gd = d < kBatchWideLevels ? b->grid : std::min(b->grid, narrow);
-    hipLaunchKernelGGL((k_batch_level<W, CT>), dim3(gd), dim3(kBatchBlock), 0, s, p, d);
+    if (p.pull_lb)
+      hipLaunchKernelGGL((k_batch_level<W, CT, true>), dim3(gd), dim3(kBatchBlock), 0, s, p, d);
+    else
+      hipLaunchKernelGGL((k_batch_level<W, CT, false>), dim3(gd), dim3(kBatchBlock), 0, s, p, d);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(b->stats_h, b->stats, sizeof(RunStats), hipMemcpyDeviceToHost, s));
