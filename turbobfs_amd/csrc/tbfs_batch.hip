@@ -73,7 +73,8 @@ using namespace dev;
 constexpr int kBatchSlice = 256;  // in-list entries per pull slice
 constexpr int kBatchBlock = 256;  // threads per workgroup (4 wavefronts)
 constexpr int kBatchMaxW = 16;    // words per vertex: up to 1024 sources
-constexpr float kBatchAlpha = 32.0f;
+constexpr float kBatchAlpha = 32.0f;       // default push / pull switch
+constexpr float kBatchAlphaShort = 4.0f;   // the same for short in-lists
 constexpr int kBatchWideLevels = 16;  // levels on the full-occupancy grid
 constexpr int kBatchLbDegree = 12;    // mean in-degree below which pulls share passes  // default push / pull switch
 
@@ -1012,9 +1013,11 @@ BatchParams make_bparams(const tbfs_batch* b) {
   p.nslices = g->nbsl;
   p.bigo = g->bigo;
   p.nbigo = g->nbigo;
-  p.alpha = b->o.batch_alpha > 0 ? b->o.batch_alpha : kBatchAlpha;
   // short in-lists share a pass; long ones pay off one item at a time
   p.pull_lb = (int64_t)g->nz < (int64_t)kBatchLbDegree * g->n ? 1 : 0;
+  // a pull costs more per entry on short in-lists: push longer there
+  p.alpha = b->o.batch_alpha > 0 ? b->o.batch_alpha
+                                 : (p.pull_lb ? kBatchAlphaShort : kBatchAlpha);
   for (int i = 0; i < 3; ++i) p.F[i] = b->F[i];
   for (int i = 0; i < 3; ++i) p.lst[i] = b->lst[i];
   for (int i = 0; i < 2; ++i) p.seen[i] = b->seen[i];

@@ -125,9 +125,10 @@ int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots
 // asynchronous single-source runs (query server)
 int ctx_async_begin(tbfs_ctx* c, int count);
 int ctx_async_enqueue(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs_opts& o,
-                      int idx, int32_t* S, uint64_t* sigma, uint8_t* ovf);
+                      int idx, int levels, int32_t* S, uint64_t* sigma, uint8_t* ovf);
 int ctx_async_finish(tbfs_ctx* c, int count);
-int ctx_async_poll(tbfs_ctx* c, int count, bool* ready, std::vector<int>* unfinished);
+int ctx_async_poll(tbfs_ctx* c, int count, bool* ready, std::vector<int>* unfinished,
+                   int* maxlev);
 
 // batched sources (tbfs_batch.hip)
 int batch_words_for(int32_t count);

@@ -963,18 +963,19 @@ int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots
 
 // Asynchronous single-source BFS for the query server: each source is a
 // chained BFS (level budget chain_k) followed by the copy of its results to
-// host buffers, all on the context's stream; ctx_async_finish queues the
-// stats copy and an event, ctx_async_poll reports when that event has passed
-// and which sources ran out of levels (they are enqueued again by the caller
-// after the budget grew).
+// host buffers, all on the context's stream (`levels` level kernels per
+// source); ctx_async_finish queues the stats copy and an event,
+// ctx_async_poll reports when that event has passed, which sources ran out
+// of levels (the caller enqueues them again with a larger budget) and the
+// deepest level count seen.
 int ctx_async_begin(tbfs_ctx* c, int count) { return ctx_reserve_stats(c, count); }
 
 int ctx_async_enqueue(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs_opts& o,
-                      int idx, int32_t* S, uint64_t* sigma, uint8_t* ovf) {
-  if (idx < 0 || idx >= c->stats_cap) return TBFS_ERR_INVALID;
+                      int idx, int levels, int32_t* S, uint64_t* sigma, uint8_t* ovf) {
+  if (idx < 0 || idx >= c->stats_cap || levels < 1) return TBFS_ERR_INVALID;
   int rc = check_run(c, g, root, o);
   if (rc) return rc;
-  rc = enqueue_chained(c, g, root, o, c->stats + idx, c->chain_k);
+  rc = enqueue_chained(c, g, root, o, c->stats + idx, levels);
   if (rc) return rc;
   hipStream_t s = c->stream;
   const size_t n = (size_t)g->n;
@@ -993,22 +994,19 @@ int ctx_async_finish(tbfs_ctx* c, int count) {
   return TBFS_OK;
 }
 
-int ctx_async_poll(tbfs_ctx* c, int count, bool* ready, std::vector<int>* unfinished) {
+int ctx_async_poll(tbfs_ctx* c, int count, bool* ready, std::vector<int>* unfinished,
+                   int* maxlev) {
   *ready = false;
   const hipError_t e = hipEventQuery(c->ev1);
   if (e == hipErrorNotReady) return TBFS_OK;
   if (e != hipSuccess) return hip_fail(e, "hipEventQuery", __FILE__, __LINE__);
   *ready = true;
   unfinished->clear();
-  int maxlev = 0;
+  *maxlev = 0;
   for (int i = 0; i < count; ++i) {
     if (c->stats_h[i].levels < 0) unfinished->push_back(i);
-    maxlev = std::max(maxlev, c->stats_h[i].levels);
+    *maxlev = std::max(*maxlev, c->stats_h[i].levels);
   }
-  // more levels per source from now on: the deepest seen plus one spare, or
-  // twice the old budget when a source did not finish
-  const int want = unfinished->empty() ? maxlev + 2 : 2 * c->chain_k;
-  c->chain_k = std::max(c->chain_k, std::min(1 << 16, want));
   return TBFS_OK;
 }
 

@@ -78,6 +78,7 @@ struct tbfs_server {
   // single-source query, and one batch of 1, 2, 4, 8, 16 words (0: unknown)
   std::vector<double> t_single;
   std::vector<std::array<double, 5>> t_batch;
+  std::vector<int> budget;  // level kernels per single-source run, per graph
   int32_t nmax = 0;
   std::vector<std::deque<Query>> pending;
   std::deque<tbfs_query_done> completed;
@@ -171,12 +172,16 @@ bool step_slot(tbfs_server* s, Slot& sl) {
   }
   if (sl.state == 3) {
     bool ready = false;
-    int rc = ctx_async_poll(sl.c, (int)sl.q.size(), &ready, &sl.redo);
+    int maxlev = 0;
+    int rc = ctx_async_poll(sl.c, (int)sl.q.size(), &ready, &sl.redo, &maxlev);
     if (rc) {
       complete(s, sl, rc, s->now_us());
       return true;
     }
     if (!ready) return false;
+    int& bud = s->budget[sl.graph];
+    // the deepest BFS seen plus one spare; twice the budget if one ran out
+    bud = std::max(bud, std::min(1 << 16, sl.redo.empty() ? maxlev + 2 : 2 * bud));
     if (sl.redo.empty()) {
       complete(s, sl, TBFS_OK, s->now_us());
       return true;
@@ -184,7 +189,8 @@ bool step_slot(tbfs_server* s, Slot& sl) {
     // sources that ran out of levels: again, with the larger budget
     for (int i : sl.redo) {
       const Query& q = sl.q[i];
-      rc = ctx_async_enqueue(sl.c, s->graphs[sl.graph], q.root, s->bopts, i, q.S, q.sigma, q.ovf);
+      rc = ctx_async_enqueue(sl.c, s->graphs[sl.graph], q.root, s->bopts, i, bud, q.S, q.sigma,
+                             q.ovf);
       if (rc) break;
     }
     if (!rc) rc = ctx_async_finish(sl.c, (int)sl.q.size());
@@ -242,7 +248,8 @@ bool admit(tbfs_server* s, Slot& sl) {
     if (!rc) rc = ctx_async_begin(sl.c, (int)sl.q.size());
     for (size_t i = 0; i < sl.q.size() && !rc; ++i) {
       const Query& q = sl.q[i];
-      rc = ctx_async_enqueue(sl.c, s->graphs[g], q.root, s->bopts, (int)i, q.S, q.sigma, q.ovf);
+      rc = ctx_async_enqueue(sl.c, s->graphs[g], q.root, s->bopts, (int)i, s->budget[g], q.S,
+                             q.sigma, q.ovf);
     }
     if (!rc) rc = ctx_async_finish(sl.c, (int)sl.q.size());
     if (rc) {
@@ -343,6 +350,7 @@ int tbfs_server_create(const tbfs_graph* const* graphs, int32_t ngraphs,
   s->pending.resize(ngraphs);
   s->t_single.assign(ngraphs, 0.0);
   s->t_batch.assign(ngraphs, std::array<double, 5>{});
+  s->budget.assign(ngraphs, 8);
   s->nmax = nmax;
   for (int i = 0; i < ngraphs; ++i) {
     int w = 1;
