@@ -14,7 +14,8 @@ CPU baseline (the reference's own sequential bfs_seq_td_csc, compiled from
 its sources into oracle/_ref, else this project's C restatement), and
 (unless --no-batch) the batched-sources throughput against the per-source
 loop (R-MAT scale 18, 1024 sources per batch), a 512-graph block-diagonal
-multi-root BFS against a per-graph loop, and (unless --no-serve) the query
+multi-root BFS against a per-graph loop, the per-level cost of each level
+loop on a high-diameter 2048x2048 grid, and (unless --no-serve) the query
 server's saturated throughput and p50 / p99 latency under Poisson arrivals.
 """
 from __future__ import annotations
@@ -100,7 +101,7 @@ def cpu_baseline(g, sources, budget_s: float):
     }
 
 
-def bench_batched(tb, graphs, reps: int = 3):
+def bench_batched(tb, graphs, reps: int = 3, tune: bool = True, loop: bool = True):
     """Batched sources vs the per-source loop on the same GPU: R-MAT scale 18
     (ef16, a=.57 b=.19 c=.19, symmetrized, relabelled, seed 4), 1024 sources
     of nonzero degree (seed 4) in one batch, 64 per frontier word."""
@@ -110,31 +111,41 @@ def bench_batched(tb, graphs, reps: int = 3):
     b = tb.Batch(g.n, len(src))
     b.run(gr, src)  # warm-up
     runs = [b.run(gr, src) for _ in range(reps)]
+    ms_default = float(np.median([r["ms"] for r in runs]))
+    # autotuned launch grid and push / pull threshold for this graph
+    best = b.autotune(gr, src)[0] if tune else tb.default_opts()
+    runs = [b.run(gr, src, best) for _ in range(reps)]
     ms = float(np.median([r["ms"] for r in runs]))
     st = runs[0]
-    ctx = tb.Context(g.n)
-    ctx.run_many(gr, src[:64])  # warm-up
-    ctx.mark(0)
-    per = ctx.run_many(gr, src)
-    ctx.mark(1)
-    loop_ms = ctx.elapsed(0, 1)
     batch_sps = len(src) / (ms * 1e-3)
-    loop_sps = len(src) / (loop_ms * 1e-3)
     b.close()
-    ctx.close()
+    if loop:
+        ctx = tb.Context(g.n)
+        ctx.run_many(gr, src[:64])  # warm-up
+        ctx.mark(0)
+        per = ctx.run_many(gr, src)
+        ctx.mark(1)
+        loop_ms = ctx.elapsed(0, 1)
+        loop_sps = len(src) / (loop_ms * 1e-3)
+        loop_gteps = sum(x["edges"] for x in per) / 2 / (loop_ms * 1e-3) / 1e9
+        ctx.close()
+    else:
+        loop_sps = loop_gteps = float("nan")
     return {
         "workload": "R-MAT scale 18 ef16 symmetrized relabelled seed 4, 1024 sources (seed 4) "
                     "in one batch; every source's S and sigma exact",
         "n": g.n, "nz": g.nz, "sources": len(src),
         "sources_per_s": batch_sps,
         "batch_ms": ms,
+        "batch_ms_default_opts": ms_default,
+        "autotuned": {"grid": int(best.grid), "batch_alpha": float(best.batch_alpha)},
         "gteps": st["edges"] / 2 / (ms * 1e-3) / 1e9,
         "levels": st["levels"], "push_levels": st["push_levels"],
         "pull_levels": st["pull_levels"], "count_bits": st["count_bits"],
         "per_source_loop_sources_per_s": loop_sps,
         "per_source_loop": "tbfs_bfs_many, chained direction-optimizing loop, same GPU",
         "speedup_vs_per_source_loop": batch_sps / loop_sps,
-        "per_source_loop_gteps": sum(x["edges"] for x in per) / 2 / (loop_ms * 1e-3) / 1e9,
+        "per_source_loop_gteps": loop_gteps,
     }
 
 
@@ -173,6 +184,47 @@ def bench_multigraph(tb, graphs, reps: int = 3):
         "per_graph_loop": "tbfs_bfs per graph, resident graphs, host wall time",
         "speedup_vs_per_graph_loop": loop_ms / ms,
     }
+
+
+def bench_grid(tb, graphs, nsrc: int = 16, names=("chained", "graph", "persistent_1024x64", "host")):
+    """High-diameter single-source BFS: 2048x2048 4-neighbour grid with 10% of
+    its edges removed (seed 3), vertex ids permuted, sources in the largest
+    component (seed 3).  Per-level cost of each level loop."""
+    from turbobfs_amd import _lib
+    g = graphs.grid(2048, 2048, 0.1, seed=3, relabel=True)
+    lc = graphs.largest_component_vertices(g)
+    src = graphs.sample_sources(lc, nsrc, seed=3)
+    gr = tb.Graph(g.CP, g.IC, symmetric=True)
+    ctx = tb.Context(g.n)
+    loops = [
+        ("chained", dict(loop=_lib.LOOP_CHAINED)),
+        ("graph", dict(loop=_lib.LOOP_GRAPH)),
+        ("persistent_1024x64", dict(loop=_lib.LOOP_PERSISTENT, block=1024, grid=64)),
+        ("host", dict(loop=_lib.LOOP_HOST)),
+    ]
+    out = {"workload": "2048x2048 grid, 10% of edges removed (seed 3), relabelled; "
+                       f"{nsrc} sources in the largest component (seed 3)",
+           "n": g.n, "nz": g.nz, "loops": {}}
+    for name, kw in loops:
+        if name not in names:
+            continue
+        o = tb.default_opts(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, **kw)
+        k = nsrc if name != "host" else 4
+        ctx.run_many(gr, src[:2], o)   # warm-up (level budget, graph capture)
+        st = ctx.run_many(gr, src[:k], o)
+        ms = sum(x["ms"] for x in st)
+        lv = sum(x["levels"] for x in st)
+        out["loops"][name] = {"sources": k, "ms_per_bfs": ms / k, "levels_per_bfs": lv / k,
+                              "us_per_level": 1e3 * ms / lv,
+                              "gteps": sum(x["edges"] for x in st) / 2 / (ms * 1e-3) / 1e9}
+    b = tb.Batch(g.n, nsrc)
+    b.run(gr, src)
+    st = b.run(gr, src)
+    out["batch_of_sources"] = {"sources": nsrc, "ms": st["ms"], "levels": st["levels"],
+                               "sources_per_s": nsrc / (st["ms"] * 1e-3)}
+    ctx.close()
+    b.close()
+    return out
 
 
 def bench_serving(tb, graphs, seconds: float = 1.5, rates=(0.25, 0.5, 0.8)):
@@ -285,6 +337,13 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-sources and multi-graph measurements")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--only", choices=["batch", "multigraph", "grid", "serve"], default=None,
+                    help="run only that measurement and print its JSON object (for profiling)")
+    ap.add_argument("--no-tune", action="store_true", help="batched run with default options only")
+    ap.add_argument("--grid-loops", default="chained,graph,persistent_1024x64,host",
+                    help="level loops timed by --only grid (comma-separated)")
+    ap.add_argument("--no-loop", action="store_true",
+                    help="batched run without the per-source loop it is compared with")
     ap.add_argument("--calib-mib", type=int, default=512,
                     help="size of the plain copy kernel run first (byte-counter reference)")
     args = ap.parse_args()
@@ -301,6 +360,14 @@ def main():
     if tb.device_count() < 1:
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        fn = {"batch": lambda: bench_batched(tb, graphs, tune=not args.no_tune,
+                                             loop=not args.no_loop),
+              "multigraph": lambda: bench_multigraph(tb, graphs),
+              "grid": lambda: bench_grid(tb, graphs, names=tuple(args.grid_loops.split(","))),
+              "serve": lambda: bench_serving(tb, graphs)}[args.only]
+        print(json.dumps({args.only: fn()}))
+        return
     g = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)
     sources = graphs.sample_sources(graphs.nonzero_degree_vertices(g), args.sources, seed=2)
     H, P = _lib.LOOP_HOST, _lib.LOOP_PERSISTENT
@@ -391,6 +458,7 @@ def main():
     if not args.no_batch:
         line["batched"] = bench_batched(tb, graphs)
         line["multi_graph"] = bench_multigraph(tb, graphs)
+        line["high_diameter"] = bench_grid(tb, graphs)
     if not args.no_serve:
         line["serving"] = bench_serving(tb, graphs)
     if not args.no_cpu:

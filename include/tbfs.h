@@ -218,6 +218,16 @@ void *tbfs_batch_stream(tbfs_batch *b);
 int tbfs_batch_run(tbfs_batch *b, const tbfs_graph *g, const int32_t *roots,
                    int32_t count, const tbfs_opts *opts, tbfs_stats *st);
 
+/* Autotune a batch for graph g and these roots: times the batch with launch
+ * grids {default, 8, 16, 32 workgroups per CU} x push thresholds
+ * batch_alpha {4, 32, 128} (best of two runs each; every candidate gives
+ * identical results) and returns the fastest in *best (the other fields from
+ * base, or the defaults when base is NULL) and its time in *best_ms.  Costs
+ * about 25 batch runs; the batch object then holds the last run's results. */
+int tbfs_batch_autotune(tbfs_batch *b, const tbfs_graph *g, const int32_t *roots,
+                        int32_t count, const tbfs_opts *base, tbfs_opts *best,
+                        double *best_ms);
+
 /* Results of sources first .. first+count-1 of the last batch, one row of n
  * entries per source (S[i*n + v] is the depth of v from roots[first+i]; any
  * pointer may be NULL; n must equal the graph's vertex count). */

@@ -198,6 +198,21 @@ def test_batch_direction_invariance(gpu, case, alpha):
         assert st["pull_levels"] >= st["levels"] - 1
 
 
+def test_batch_autotune_keeps_results(gpu):
+    g = graphs.rmat(13, 16, 0.57, 0.19, 0.19, seed=8, relabel=True)
+    roots = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 128, seed=8)
+    gr = tb.Graph(g.CP, g.IC, symmetric=True)
+    b = tb.Batch(g.n, 128)
+    best, ms = b.autotune(gr, roots)
+    assert ms > 0 and best.batch_alpha in (4.0, 32.0, 128.0)
+    S0, sig0, ovf0 = b.read()          # the last candidate's results
+    b.run(gr, roots, best)
+    S1, sig1, ovf1 = b.read()
+    assert np.array_equal(S0, S1) and np.array_equal(sig0, sig1) and np.array_equal(ovf0, ovf1)
+    eS, esig, eovf, _, _ = _expected(g, roots)
+    assert np.array_equal(S1, eS) and np.array_equal(sig1, esig)
+
+
 def test_batch_reuse_and_partial_read(gpu):
     g1 = graphs.erdos_renyi(4096, 8.0 / 4095.0, seed=1)
     g2 = graphs.grid(40, 30, 0.05, seed=1, relabel=True)

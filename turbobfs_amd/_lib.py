@@ -133,6 +133,8 @@ SIGNATURES = {
     "tbfs_batch_destroy": (C.c_int, [C.c_void_p]),
     "tbfs_batch_stream": (C.c_void_p, [C.c_void_p]),
     "tbfs_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, _i32p, C.c_int32, C.POINTER(Opts), C.POINTER(Stats)]),
+    "tbfs_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, _i32p, C.c_int32, C.POINTER(Opts),
+                                      C.POINTER(Opts), C.POINTER(C.c_double)]),
     "tbfs_batch_read": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _i32p, _u64p, _u8p]),
     "tbfs_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
     "tbfs_server_destroy": (C.c_int, [C.c_void_p]),

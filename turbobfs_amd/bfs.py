@@ -263,6 +263,21 @@ class Batch:
         out["count_bits"] = int(st.pad)
         return out
 
+    def autotune(self, g: Graph, roots, base: Opts | None = None):
+        """Time the launch-grid / push-threshold candidates on this graph and
+        these roots; returns (best opts, best ms)."""
+        roots = np.ascontiguousarray(roots, dtype=np.int32)
+        best = Opts()
+        ms = C.c_double()
+        check(_lib.load().tbfs_batch_autotune(self._h, g.handle, _ptr(roots, _i32p),
+                                              int(roots.shape[0]),
+                                              C.byref(base) if base is not None else None,
+                                              C.byref(best), C.byref(ms)), "tbfs_batch_autotune")
+        self._n = g.n
+        self._g = g
+        self._count = int(roots.shape[0])
+        return best, float(ms.value)
+
     def read(self, first: int = 0, count: int | None = None):
         """(S, sigma, ovf) of sources first .. first+count-1, shape (count, n)."""
         if count is None:

@@ -12,7 +12,8 @@
 //                         by push levels; a level whose frontier and previous
 //                         frontier are both listed touches only those words
 // Counts are uint32 first (CT = uint32_t: half the bytes of every gather);
-// a batch in which some count reaches 2^32 runs again with CT = uint64_t.
+// a batch in which some count reaches 2^32 runs again with CT = uint64_t,
+// and later batches on that graph start with uint64_t.
 //   S       int32[W][n][64] per-source depth } valid where the pair's seen
 //   sigma   CT[W][n][64]    per-source count } bit is set (never-reached
 //                                              pairs are not written; reads
@@ -1117,7 +1118,7 @@ int batch_start(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32
   b->o = o;
   b->count = count;
   b->W = W;
-  b->ct = o.batch_count_bits == 64 ? 8 : 4;
+  b->ct = (o.batch_count_bits == 64 || (o.batch_count_bits == 0 && g->wide_counts)) ? 8 : 4;
   b->grid = 0;
   b->done = false;
   b->seen_final = nullptr;
@@ -1156,6 +1157,7 @@ int batch_progress(tbfs_batch* b, bool* finished) {
     return rc;
   }
   if (b->ct == 4 && rs.ovf_any && b->o.batch_count_bits != 32) {  // widen and run again
+    const_cast<tbfs_graph*>(b->g)->wide_counts = true;  // start wide on this graph next time
     b->ct = 8;
     b->grid = 0;
     const int rc = enqueue(b, 0, b->chain_k);
@@ -1271,9 +1273,54 @@ int batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* 
   return TBFS_OK;
 }
 
+// Autotuner: the batch's launch grid and push / pull threshold, chosen by
+// timing candidates on the caller's graph and roots (every candidate gives
+// identical results; only the time differs).
+int batch_autotune(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t count,
+                   const tbfs_opts* base, tbfs_opts* best, double* best_ms) {
+  if (!b || !g || !roots || !best) return TBFS_ERR_INVALID;
+  tbfs_opts o;
+  if (base) o = *base; else tbfs_opts_default(&o);
+  const int cus = std::max(1, b->cus);
+  const int grids[] = {0, cus * 8, cus * 16, cus * 32};
+  const float alphas[] = {4.0f, 32.0f, 128.0f};
+  tbfs_stats st{};
+  o.grid = 0;
+  o.batch_alpha = 0;
+  int rc = batch_run(b, g, roots, count, &o, &st);  // warm-up (slices, first launches)
+  if (rc) return rc;
+  double bt = -1;
+  tbfs_opts bo = o;
+  for (int gsz : grids) {
+    for (float al : alphas) {
+      tbfs_opts t = o;
+      t.grid = gsz;
+      t.batch_alpha = al;
+      double ms = -1;
+      for (int rep = 0; rep < 2; ++rep) {
+        rc = batch_run(b, g, roots, count, &t, &st);
+        if (rc) return rc;
+        if (ms < 0 || st.ms < ms) ms = st.ms;
+      }
+      if (bt < 0 || ms < bt) {
+        bt = ms;
+        bo = t;
+      }
+    }
+  }
+  *best = bo;
+  if (best_ms) *best_ms = bt;
+  return TBFS_OK;
+}
+
 }  // namespace tbfs
 
 extern "C" {
+
+int tbfs_batch_autotune(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t count,
+                        const tbfs_opts* base, tbfs_opts* best, double* best_ms) {
+  return tbfs::batch_autotune(b, g, roots, count, base, best, best_ms);
+}
 
 int tbfs_batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch** out) {
   return tbfs::batch_create(n_cap, b_cap, out);

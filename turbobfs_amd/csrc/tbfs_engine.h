@@ -28,6 +28,7 @@ struct tbfs_graph {
   uint64_t* skip = nullptr;  // static: in-degree 0 or > kBigIn (one bit per vertex)
   uint64_t* bsl = nullptr;   // batched pull slices of the in-lists (v | slice << 32),
   int32_t nbsl = 0;          // built on first use
+  bool wide_counts = false;  // a batch on this graph needed 64-bit counts: start with them
 };
 
 // Per-BFS device state bound to one stream.
