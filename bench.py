@@ -101,6 +101,38 @@ def cpu_baseline(g, sources, budget_s: float):
     }
 
 
+def bench_variants(tb, g, gr, ctx, sources, nsrc: int = 16):
+    """Every reference variant's kernel mapping on the same resident graph:
+    GTEPS and algorithmic GB/s (B_alg / time) per variant, sources in turn."""
+    from turbobfs_amd import _lib
+    H = _lib.LOOP_HOST
+    variants = [
+        ("td_csc_sc", dict(mode=_lib.MODE_PULL_SC, loop=H)),
+        ("td_csc_wa", dict(mode=_lib.MODE_PULL_WA, loop=H)),
+        ("bu_csc_sc_bbreak", dict(mode=_lib.MODE_PULL_SC, loop=H, sigma=0, bbreak=1)),
+        ("bu_csc_wa_bbreak", dict(mode=_lib.MODE_PULL_WA, loop=H, sigma=0, bbreak=1)),
+        ("td_cooc", dict(mode=_lib.MODE_COOC, loop=H)),
+        ("tdbu_push_pull_host", dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, loop=H)),
+        ("tdbu_push_pull_chained", dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB,
+                                        loop=_lib.LOOP_CHAINED)),
+    ]
+    roots = np.asarray(sources[:nsrc], np.int32)
+    n = g.n
+    out = {}
+    for name, kw in variants:
+        o = tb.default_opts(**kw)
+        ctx.run_many(gr, roots[:1], o)
+        ctx.mark(0)
+        st = ctx.run_many(gr, roots, o)
+        ctx.mark(1)
+        ms = ctx.elapsed(0, 1)
+        edges = sum(x["edges"] for x in st)
+        b_alg = len(roots) * (4 * (n + 1) + 12 * n) + 4 * edges
+        out[name] = {"gteps": edges / 2 / (ms * 1e-3) / 1e9, "ms_per_bfs": ms / len(roots),
+                     "alg_gbs": b_alg / (ms * 1e-3) / 1e9, "sources": len(roots)}
+    return out
+
+
 def bench_batched(tb, graphs, reps: int = 3, tune: bool = True, loop: bool = True):
     """Batched sources vs the per-source loop on the same GPU: R-MAT scale 18
     (ef16, a=.57 b=.19 c=.19, symmetrized, relabelled, seed 4), 1024 sources
@@ -337,6 +369,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-sources and multi-graph measurements")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the per-variant (TD / BU / TDBU / COOC) GTEPS table")
     ap.add_argument("--only", choices=["batch", "multigraph", "grid", "serve"], default=None,
                     help="run only that measurement and print its JSON object (for profiling)")
     ap.add_argument("--no-tune", action="store_true", help="batched run with default options only")
@@ -455,6 +489,8 @@ def main():
                            args.mode, "whole BFS (all level kernels of one source)"),
         },
     }
+    if not args.no_variants:
+        line["variants"] = bench_variants(tb, g, gr, ctx, sources)
     if not args.no_batch:
         line["batched"] = bench_batched(tb, graphs)
         line["multi_graph"] = bench_multigraph(tb, graphs)

@@ -34,8 +34,9 @@ namespace tbfs {
 
 constexpr int kWave = 64;
 constexpr int kPushChunk = 64;    // out-list entries per queue entry
-constexpr int kBigIn = 1024;      // in-degree above which a column is split
-constexpr int kPullChunk = 1024;  // in-list entries per static pull slice
+constexpr int kBigIn = 256;      // in-degree above which a column is split
+constexpr int kPullChunk = 256;  // list entries per static slice (one pass of a wavefront)
+static_assert(kPullChunk % kWave == 0, "slices are whole passes of a wavefront");
 
 // Counters of the vertices discovered in one level.  Every field is updated
 // with one device-scope atomic per wavefront.

@@ -33,6 +33,7 @@ namespace {
 using namespace dev;
 
 constexpr int kUnroll = 4;  // independent edges in flight per lane
+constexpr int kSliceRegs = kPullChunk / kWave;  // slice entries per lane
 
 struct WaveLds {
   int32_t excl[64];  // exclusive prefix of per-lane work
@@ -185,34 +186,45 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
                                               int32_t nbigo, int64_t wave,
                                               int64_t nwaves, WaveLds* L,
                                               LevelAcc& a) {
-  // this wave's share: entries wave, wave + nwaves, ...; 64 examined at once
+  // this wave's share: entries wave, wave + nwaves, ...; 64 examined at once.
+  // Each lane loads the bounds and count of its own entry; the wavefront then
+  // takes the active slices in turn, kSliceRegs out-list entries per lane, all
+  // loads of a slice in flight together.
   const int64_t nbo = nbigo > wave ? (nbigo - wave + nwaves - 1) / nwaves : 0;
   for (int64_t b = 0; b < nbo; b += 64) {
     const int64_t i = wave + (b + lane()) * nwaves;
     bool act = false;
-    uint64_t ent = 0;
+    int cb = 0, ce = 0;
+    uint64_t x = 0;
+    int ou = 0;
     if (b + lane() < nbo) {
-      ent = bigo[i];
-      act = fbit(st.fcur, (int)(uint32_t)ent);
+      const uint64_t ent = bigo[i];
+      const int u = (int)(uint32_t)ent;
+      act = fbit(st.fcur, u);
+      if (act) {
+        cb = g.RP[u] + (int)(ent >> 32) * kPullChunk;
+        ce = min(g.RP[u + 1], cb + kPullChunk);
+        x = st.sigma[u];
+        if (ovf_any) ou = st.ovf[u];
+      }
     }
     uint64_t todo = __ballot(act);
     while (todo) {
       const int j = __ffsll((unsigned long long)todo) - 1;
       todo &= todo - 1;
-      const uint64_t e = __shfl(ent, j);
-      const int u = (int)(uint32_t)e;
-      const int cb = g.RP[u] + (int)(e >> 32) * kPullChunk;
-      const int ce = min(g.RP[u + 1], cb + kPullChunk);
-      const uint64_t x = st.sigma[u];
-      const bool ou = ovf_any && st.ovf[u];
-      for (int k0 = cb + lane(); k0 < ce; k0 += 64 * kUnroll) {
-        int v[kUnroll];
+      const int scb = __shfl(cb, j);
+      const int sce = __shfl(ce, j);
+      const uint64_t sx = __shfl(x, j);
+      const bool sou = __shfl(ou, j) != 0;
+      int v[kSliceRegs];
 #pragma unroll
-        for (int t = 0; t < kUnroll; ++t) v[t] = (k0 + 64 * t < ce) ? g.JR[k0 + 64 * t] : -1;
-#pragma unroll
-        for (int t = 0; t < kUnroll; ++t)
-          if (v[t] >= 0) push_edge(g, st, dn, v[t], x, ou, a);
+      for (int t = 0; t < kSliceRegs; ++t) {
+        const int k = scb + lane() + 64 * t;
+        v[t] = k < sce ? g.JR[k] : -1;
       }
+#pragma unroll
+      for (int t = 0; t < kSliceRegs; ++t)
+        if (v[t] >= 0) push_edge(g, st, dn, v[t], sx, sou, a);
     }
   }
   const int64_t nwords = ((int64_t)g.n + 63) >> 6;
@@ -297,57 +309,89 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
                                               int64_t nwaves, WaveLds* L,
                                               LevelAcc& a,
                                               const uint64_t* fpar) {
+  // Slices: each lane loads the bounds of its own entry; the wavefront takes
+  // the active slices in turn (kSliceRegs in-list entries per lane, loads in
+  // flight together) and the lane owning a slice keeps its sum; the atomics
+  // of up to 64 slices then go out together.
   const int64_t nbc = nbigc > wave ? (nbigc - wave + nwaves - 1) / nwaves : 0;
   for (int64_t b = 0; b < nbc; b += 64) {
     const int64_t i = wave + (b + lane()) * nwaves;
     bool act = false;
-    uint64_t ent = 0;
+    int v = 0, cb = 0, ce = 0;
     if (b + lane() < nbc) {
-      ent = bigc[i];
-      const int sv = st.S[(int)(uint32_t)ent];
+      const uint64_t ent = bigc[i];
+      v = (int)(uint32_t)ent;
+      const int sv = st.S[v];
       act = (sv == -1 || sv == dn);  // not reached before this level
+      if (act) {
+        cb = g.CP[v] + (int)(ent >> 32) * kPullChunk;
+        ce = min(g.CP[v + 1], cb + kPullChunk);
+      }
     }
     uint64_t todo = __ballot(act);
+    bool mine_f = false;
+    uint64_t mine_sum = 0;
+    uint32_t mine_ov = 0;
     while (todo) {
       const int j = __ffsll((unsigned long long)todo) - 1;
       todo &= todo - 1;
-      const uint64_t e = __shfl(ent, j);
-      const int v = (int)(uint32_t)e;
-      const int cb = g.CP[v] + (int)(e >> 32) * kPullChunk;
-      const int ce = min(g.CP[v + 1], cb + kPullChunk);
+      const int scb = __shfl(cb, j);
+      const int sce = __shfl(ce, j);
+      int u[kSliceRegs];
+#pragma unroll
+      for (int t = 0; t < kSliceRegs; ++t) {
+        const int k = scb + lane() + 64 * t;
+        u[t] = k < sce ? g.IC[k] : -1;
+      }
+      bool par[kSliceRegs];
+#pragma unroll
+      for (int t = 0; t < kSliceRegs; ++t) par[t] = u[t] >= 0 && fbit(fpar, u[t]);
       bool f = false;
       uint64_t sum = 0;
       uint32_t ov = 0;
-      for (int k = cb + lane(); k < ce; k += 64) {
-        const int u = g.IC[k];
-        if (fbit(fpar, u)) {
-          f = true;
-          if (SIGMA) {
-            add_carry(sum, ov, st.sigma[u]);
-            if (ovf_any) ov |= st.ovf[u];
-          }
+      if (SIGMA) {
+        uint64_t xs[kSliceRegs];
+        uint32_t us[kSliceRegs];
+#pragma unroll
+        for (int t = 0; t < kSliceRegs; ++t) {
+          xs[t] = par[t] ? st.sigma[u[t]] : 0ull;
+          us[t] = (par[t] && ovf_any) ? (uint32_t)st.ovf[u[t]] : 0u;
         }
+#pragma unroll
+        for (int t = 0; t < kSliceRegs; ++t) {
+          f |= par[t];
+          add_carry(sum, ov, xs[t]);
+          ov |= us[t];
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < kSliceRegs; ++t) f |= par[t];
       }
       if (__ballot(f)) {
         if (SIGMA) wave_sum_carry(sum, ov);
-        if (lane() == 0) {
-          if (SIGMA) {
-            const unsigned long long old =
-                atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)sum);
-            if (old + sum < old) ov = 1;
-            if (ov) {
-              st.ovf[v] = 1;
-              a.ovf = 1;
-            }
-          }
-          const unsigned long long bit = 1ull << (v & 63);
-          const unsigned long long old =
-              atomicOr((unsigned long long*)&st.fnext[v >> 6], bit);
-          if ((old & bit) == 0) {  // first slice of v to find a parent
-            st.S[v] = dn;
-            a.found(g, v);
-          }
+        if (lane() == j) {
+          mine_f = true;
+          mine_sum = sum;
+          mine_ov = ov;
         }
+      }
+    }
+    if (mine_f) {
+      if (SIGMA) {
+        const unsigned long long old =
+            atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)mine_sum);
+        if (old + mine_sum < old) mine_ov = 1;
+        if (mine_ov) {
+          st.ovf[v] = 1;
+          a.ovf = 1;
+        }
+      }
+      const unsigned long long bit = 1ull << (v & 63);
+      const unsigned long long old =
+          atomicOr((unsigned long long*)&st.fnext[v >> 6], bit);
+      if ((old & bit) == 0) {  // first slice of v to find a parent
+        st.S[v] = dn;
+        a.found(g, v);
       }
     }
   }
