@@ -122,10 +122,12 @@ __device__ __forceinline__ LevelCtr read_set(const CtrSet* set, bool agent) {
 
 // One edge u -> v of a push level, u at depth d with count x (overflow ou).
 // "Reached at depth <= d" is read from the reached bitmap or the current
-// frontier (both small and cached); otherwise v is reached at depth d+1, so
-// the count is added at once and, independently, v's bit is claimed in the
-// next frontier: the lane that sets the bit records the depth.  The two
-// device-scope atomics do not wait on each other.
+// frontier (both small and cached); otherwise v is reached at depth d+1 and
+// the count is added at once.  A vertex not reached before this level holds
+// count 0, so only an adder that finds 0 can be the first: those (normally
+// one per vertex; more only when a count is 0 mod 2^64) claim v's bit in the
+// next frontier, and the one that sets it records the depth.  Device-scope
+// atomics go out to memory, so the second one is skipped where it can be.
 __device__ __forceinline__ void push_edge(const GraphView& g, const StateView& st,
                                           int dn, int v, uint64_t x, bool ou,
                                           LevelAcc& a) {
@@ -133,15 +135,17 @@ __device__ __forceinline__ void push_edge(const GraphView& g, const StateView& s
   if ((st.vis[v >> 6] | st.fcur[v >> 6]) & bit) return;
   const unsigned long long old =
       atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)x);
-  const unsigned long long oldb =
-      atomicOr((unsigned long long*)&st.fnext[v >> 6], (unsigned long long)bit);
   if (old + x < old || ou) {
     st.ovf[v] = 1;
     a.ovf = 1;
   }
-  if ((oldb & bit) == 0) {
-    st.S[v] = dn;
-    a.found(g, v);
+  if (old == 0) {
+    const unsigned long long oldb =
+        atomicOr((unsigned long long*)&st.fnext[v >> 6], (unsigned long long)bit);
+    if ((oldb & bit) == 0) {
+      st.S[v] = dn;
+      a.found(g, v);
+    }
   }
 }
 
@@ -623,6 +627,7 @@ __device__ __forceinline__ void write_stats(const PersistParams& p,
 // out of the L2); it must hold ceil(n/64) words.
 __device__ __forceinline__ void run_level(const PersistParams& p, int d,
                                           bool pushing, uint32_t ovf_any,
+                                          unsigned long long mu,
                                           int64_t gtid, int64_t gstride,
                                           WaveLds* L, LevelAcc& acc,
                                           uint64_t* fb_lds) {
@@ -650,7 +655,9 @@ __device__ __forceinline__ void run_level(const PersistParams& p, int d,
     push_lb_level(g, st, d + 1, ovf_any, p.bigo, p.nbigo, wave, nwaves, L, acc);
   } else {
     const uint64_t* fpar = st.fcur;
-    if (fb_lds) {
+    // staging costs every workgroup a read of the whole bitmap: worth it
+    // only when the unexplored in-lists (parent tests) outnumber the vertices
+    if (fb_lds && mu > (unsigned long long)g.n) {
       const uint4* src = reinterpret_cast<const uint4*>(st.fcur);
       uint4* dst = reinterpret_cast<uint4*>(fb_lds);
       const int64_t n16 = nwords >> 1;
@@ -683,7 +690,7 @@ __global__ __launch_bounds__(BLOCK) void k_persist(PersistParams p) {
   for (;;) {
     const LevelCtr c = read_set(&p.cset[s.d % 3], true);
     if (!step_state(p, &c, s)) break;
-    run_level(p, s.d, s.pushing != 0, s.ovf_any, gtid, gstride, L, acc, nullptr);
+    run_level(p, s.d, s.pushing != 0, s.ovf_any, s.mu, gtid, gstride, L, acc, nullptr);
     ++s.d;
     grid_barrier(p.bar, ++bk);
   }
@@ -748,7 +755,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t 
   nx.d = d + 1;
   if (gtid == 0) p.loop[(step + 1) & 1] = nx;
   LevelAcc acc;
-  run_level(p, d, s.pushing != 0, s.ovf_any, gtid, gstride, &lds[threadIdx.x >> 6], acc,
+  run_level(p, d, s.pushing != 0, s.ovf_any, s.mu, gtid, gstride, &lds[threadIdx.x >> 6], acc,
             FBL ? fb_dyn : nullptr);
 }
 
