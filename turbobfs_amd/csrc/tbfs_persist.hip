@@ -33,12 +33,16 @@ namespace {
 using namespace dev;
 
 constexpr int kUnroll = 4;  // independent edges in flight per lane
+constexpr int kPullRegs = 8;  // in-list entries per lane per light-column pass
 constexpr int kSliceRegs = kPullChunk / kWave;  // slice entries per lane
+constexpr int kSlicesPerPass = 2;  // pull slices taken together by a wavefront
 
 struct WaveLds {
   int32_t excl[64];  // exclusive prefix of per-lane work
   int32_t beg[64];   // first edge index of each lane's slice
-  uint64_t x[64];    // push: count of each lane's vertex; pull: per-column sums
+  uint64_t x[64];    // push: count of each lane's vertex; pull: per-column sum
+                     // of the parents' counts' low 32 bits
+  uint64_t xh[64];   // pull: per-column sum of their high 32 bits
 };
 
 // Per-lane tallies of the vertices this lane discovered in the current level.
@@ -47,9 +51,14 @@ struct LevelAcc {
   uint32_t ovf = 0;
 
   __device__ __forceinline__ void found(const GraphView& g, int v) {
+    found_in(g, v, indeg(g, v));
+  }
+  // v's in-degree already known: on a symmetric graph (out-lists alias the
+  // in-lists) nothing more is loaded
+  __device__ __forceinline__ void found_in(const GraphView& g, int v, uint32_t di) {
     q += 1;
-    mout += outdeg(g, v);
-    min += indeg(g, v);
+    min += di;
+    mout += (g.RP == g.CP) ? di : outdeg(g, v);
   }
 };
 
@@ -298,6 +307,57 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
   }
 }
 
+// One pass over in-list entries e0, e0 + 64, ... (R per lane) of the
+// columns laid out in L (exclusive prefix excl, first entry beg), every load
+// of the pass issued before any is used: owners, then in-list entries, then
+// parent bits, then parents' counts.  Lanes past the end read entry 0 and
+// are masked out; a count is read for every entry (non-parents read vertex
+// 0's, one shared line), so no load sits behind a branch.  Column sums go
+// into LDS as 32-bit halves, each added into a 64-bit word: exact for any
+// in-degree below 2^32, so no atomic's result is needed to find a carry.
+template <int R, bool SIGMA>
+__device__ __forceinline__ void pull_light_pass(const GraphView& g, const StateView& st,
+                                                WaveLds* L, const uint64_t* fpar,
+                                                uint32_t ovf_any, int e0, int total,
+                                                uint64_t& fmask, uint64_t& omask) {
+  int o[R], u[R];
+  bool par[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int e = e0 + 64 * t;
+    const bool ok = e < total;
+    o[t] = owner_of(L->excl, ok ? e : 0);
+    const int k = ok ? L->beg[o[t]] + (e - L->excl[o[t]]) : 0;
+    u[t] = g.IC[k];
+    par[t] = ok;
+  }
+#pragma unroll
+  for (int t = 0; t < R; ++t) par[t] = par[t] && fbit(fpar, u[t]);
+  if (SIGMA) {
+    uint64_t xs[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) xs[t] = st.sigma[par[t] ? u[t] : 0];
+    if (ovf_any) {
+#pragma unroll
+      for (int t = 0; t < R; ++t)
+        if (par[t] && st.ovf[u[t]]) omask |= 1ull << o[t];
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      if (par[t]) {
+        fmask |= 1ull << o[t];
+        atomicAdd((unsigned long long*)&L->x[o[t]], (unsigned long long)(xs[t] & 0xffffffffull));
+        const unsigned long long h = xs[t] >> 32;
+        if (h) atomicAdd((unsigned long long*)&L->xh[o[t]], h);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+      if (par[t]) fmask |= 1ull << o[t];
+  }
+}
+
 // Pull level: (a) static slices of the high in-degree columns, reduced with
 // global atomics; (b) the other unreached columns: words are dealt to
 // wavefronts round-robin; a wavefront loads 64 of its words of the
@@ -336,47 +396,68 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
     bool mine_f = false;
     uint64_t mine_sum = 0;
     uint32_t mine_ov = 0;
+    // kSlicesPerPass active slices per pass (kSliceRegs entries per lane of
+    // each); a slot without a slice repeats the first one's start, empty
     while (todo) {
-      const int j = __ffsll((unsigned long long)todo) - 1;
-      todo &= todo - 1;
-      const int scb = __shfl(cb, j);
-      const int sce = __shfl(ce, j);
-      int u[kSliceRegs];
+      int js[kSlicesPerPass], scb[kSlicesPerPass], sce[kSlicesPerPass];
 #pragma unroll
-      for (int t = 0; t < kSliceRegs; ++t) {
-        const int k = scb + lane() + 64 * t;
-        u[t] = k < sce ? g.IC[k] : -1;
+      for (int h = 0; h < kSlicesPerPass; ++h) {
+        js[h] = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
+        todo &= todo - 1;
+        const int jj = js[h] >= 0 ? js[h] : js[0];
+        scb[h] = __shfl(cb, jj);
+        sce[h] = js[h] >= 0 ? __shfl(ce, jj) : scb[h];
       }
-      bool par[kSliceRegs];
+      constexpr int R = kSlicesPerPass * kSliceRegs;
+      int u[R];
+      bool par[R];
 #pragma unroll
-      for (int t = 0; t < kSliceRegs; ++t) par[t] = u[t] >= 0 && fbit(fpar, u[t]);
-      bool f = false;
-      uint64_t sum = 0;
-      uint32_t ov = 0;
-      if (SIGMA) {
-        uint64_t xs[kSliceRegs];
-        uint32_t us[kSliceRegs];
+      for (int h = 0; h < kSlicesPerPass; ++h) {
 #pragma unroll
         for (int t = 0; t < kSliceRegs; ++t) {
-          xs[t] = par[t] ? st.sigma[u[t]] : 0ull;
-          us[t] = (par[t] && ovf_any) ? (uint32_t)st.ovf[u[t]] : 0u;
+          const int k = scb[h] + lane() + 64 * t;
+          u[h * kSliceRegs + t] = g.IC[k < sce[h] ? k : scb[0]];
+          par[h * kSliceRegs + t] = k < sce[h];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < R; ++t) par[t] = par[t] && fbit(fpar, u[t]);
+      bool f[kSlicesPerPass];
+      uint64_t sum[kSlicesPerPass];
+      uint32_t ov[kSlicesPerPass];
+#pragma unroll
+      for (int h = 0; h < kSlicesPerPass; ++h) {
+        f[h] = false;
+        sum[h] = 0;
+        ov[h] = 0;
+      }
+      if (SIGMA) {
+        uint64_t xs[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+          const uint64_t xv = st.sigma[par[t] ? u[t] : 0];  // no load behind a branch
+          xs[t] = par[t] ? xv : 0ull;
         }
 #pragma unroll
-        for (int t = 0; t < kSliceRegs; ++t) {
-          f |= par[t];
-          add_carry(sum, ov, xs[t]);
-          ov |= us[t];
+        for (int t = 0; t < R; ++t) {
+          const int h = t / kSliceRegs;
+          f[h] |= par[t];
+          add_carry(sum[h], ov[h], xs[t]);
+          if (ovf_any && par[t]) ov[h] |= (uint32_t)st.ovf[u[t]];
         }
       } else {
 #pragma unroll
-        for (int t = 0; t < kSliceRegs; ++t) f |= par[t];
+        for (int t = 0; t < R; ++t) f[t / kSliceRegs] |= par[t];
       }
-      if (__ballot(f)) {
-        if (SIGMA) wave_sum_carry(sum, ov);
-        if (lane() == j) {
-          mine_f = true;
-          mine_sum = sum;
-          mine_ov = ov;
+#pragma unroll
+      for (int h = 0; h < kSlicesPerPass; ++h) {
+        if (__ballot(f[h])) {
+          if (SIGMA) wave_sum_carry(sum[h], ov[h]);
+          if (lane() == js[h]) {
+            mine_f = true;
+            mine_sum = sum[h];
+            mine_ov = ov[h];
+          }
         }
       }
     }
@@ -424,45 +505,18 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
       wave_lds_sync();
       L->excl[lane()] = (int)incl - deg;
       L->beg[lane()] = beg;
-      if (SIGMA) L->x[lane()] = 0;
+      if (SIGMA) {
+        L->x[lane()] = 0;
+        L->xh[lane()] = 0;
+      }
       wave_lds_sync();
       uint64_t fmask = 0, omask = 0;
-      for (int e0 = lane(); e0 < total; e0 += 64 * kUnroll) {
-        int u[kUnroll], o[kUnroll];
-#pragma unroll
-        for (int t = 0; t < kUnroll; ++t) {
-          const int e = e0 + 64 * t;
-          o[t] = -1;
-          if (e < total) {
-            o[t] = owner_of(L->excl, e);
-            u[t] = g.IC[L->beg[o[t]] + (e - L->excl[o[t]])];
-          }
-        }
-        bool par[kUnroll];
-#pragma unroll
-        for (int t = 0; t < kUnroll; ++t) par[t] = o[t] >= 0 && fbit(fpar, u[t]);
-        if (SIGMA) {
-          uint64_t xs[kUnroll];
-          uint32_t us[kUnroll];
-#pragma unroll
-          for (int t = 0; t < kUnroll; ++t) {
-            xs[t] = par[t] ? st.sigma[u[t]] : 0ull;
-            us[t] = (par[t] && ovf_any) ? (uint32_t)st.ovf[u[t]] : 0u;
-          }
-#pragma unroll
-          for (int t = 0; t < kUnroll; ++t) {
-            if (par[t]) {
-              fmask |= 1ull << o[t];
-              const unsigned long long old =
-                  atomicAdd((unsigned long long*)&L->x[o[t]], (unsigned long long)xs[t]);
-              if (old + xs[t] < old || us[t]) omask |= 1ull << o[t];
-            }
-          }
-        } else {
-#pragma unroll
-          for (int t = 0; t < kUnroll; ++t)
-            if (par[t]) fmask |= 1ull << o[t];
-        }
+      // one pass of 4 entries per lane for short words, else passes of 8
+      if (total <= 64 * 4) {
+        pull_light_pass<4, SIGMA>(g, st, L, fpar, ovf_any, lane(), total, fmask, omask);
+      } else {
+        for (int e0 = lane(); e0 < total; e0 += 64 * kPullRegs)
+          pull_light_pass<kPullRegs, SIGMA>(g, st, L, fpar, ovf_any, e0, total, fmask, omask);
       }
       fmask = wave_or_u64(fmask);
       if (fmask == 0) continue;
@@ -471,13 +525,18 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
       if ((fmask >> lane()) & 1ull) {
         st.S[v] = dn;
         if (SIGMA) {
-          st.sigma[v] = L->x[lane()];
-          if ((omask >> lane()) & 1ull) {
+          // count = xh * 2^32 + x; at least 2^64 iff xh >= 2^32 or the sum
+          // of the low 64 bits carries
+          const uint64_t xh = L->xh[lane()];
+          const uint64_t hs = xh << 32;
+          const uint64_t sum = hs + L->x[lane()];
+          st.sigma[v] = sum;
+          if ((xh >> 32) != 0 || sum < hs || ((omask >> lane()) & 1ull)) {
             st.ovf[v] = 1;
             a.ovf = 1;
           }
         }
-        a.found(g, v);
+        a.found_in(g, v, (uint32_t)deg);
       }
       if (lane() == 0) atomicOr((unsigned long long*)&st.fnext[w], fmask);
     }
