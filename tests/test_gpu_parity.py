@@ -156,8 +156,8 @@ def test_perturbed_grid_modes(gpu):
 
 
 def test_hub_columns_split_across_slices(gpu):
-    """Star + clique mix: columns with in-degree > 1024 take the static
-    pull-slice path and out-lists > 64 the sliced push path."""
+    """Star + clique mix: columns with in-degree > 256 take the static
+    pull-slice path and out-lists > 256 the sliced push path."""
     hub_leaves = 5000
     n = hub_leaves + 1 + 40
     src, dst = [], []
@@ -183,6 +183,53 @@ def test_hub_columns_split_across_slices(gpu):
     g = graphs.CSC(CP.astype(np.int32), IC)
     _check_modes(g, [0, 3, base + 5], symmetric=True)
     _check_modes(g, [base], symmetric=False)
+
+
+def _sym_csc(n, src, dst):
+    src = np.asarray(src, np.int32)
+    dst = np.asarray(dst, np.int32)
+    s2 = np.concatenate([src, dst])
+    d2 = np.concatenate([dst, src])
+    key = np.unique(d2.astype(np.int64) * n + s2)
+    dst_u, src_u = (key // n).astype(np.int32), (key % n).astype(np.int32)
+    CP = np.zeros(n + 1, np.int64)
+    CP[1:] = np.cumsum(np.bincount(dst_u, minlength=n))
+    return graphs.CSC(CP.astype(np.int32), src_u)
+
+
+def test_slice_boundaries(gpu):
+    """Lists of 255..1025 entries around the 256-entry slice size: the root
+    reaches 3 vertices, each joined to all of layer A (1100 vertices, count
+    3); hub i joins the first d_i - 5 vertices of A (count 3 (d_i - 5)) and
+    5 leaves of its own, so its list holds d_i entries.  Pull levels take the
+    hubs through 1..5 slices, push levels take the lists in slices."""
+    degs = [255, 256, 257, 300, 511, 512, 513, 768, 1025]
+    na = 1100
+    n_pre, a0 = 3, 4
+    h0 = a0 + na
+    l0 = h0 + len(degs)
+    nleaf = 5
+    n = l0 + len(degs) * nleaf
+    src, dst = [], []
+    for p in range(1, 1 + n_pre):
+        src.append(0)
+        dst.append(p)
+        src += [p] * na
+        dst += list(range(a0, a0 + na))
+    for i, d in enumerate(degs):
+        src += [h0 + i] * (d - nleaf)
+        dst += list(range(a0, a0 + d - nleaf))
+        for k in range(nleaf):
+            src.append(h0 + i)
+            dst.append(l0 + i * nleaf + k)
+    g = _sym_csc(n, src, dst)
+    assert list(np.diff(g.CP)[h0:l0]) == degs
+    pull_modes = MODES + [
+        ("chain_pull", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED, alpha=1e9)),
+        ("chain_pull_no_lds", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED,
+                                   alpha=1e9, fb_lds=-1)),
+    ]
+    _check_modes(g, [0, a0 + 3, h0 + 8, l0 + 2], symmetric=True, modes=pull_modes)
 
 
 @pytest.mark.parametrize("loop", [L.LOOP_PERSISTENT, L.LOOP_CHAINED, L.LOOP_HOST, L.LOOP_GRAPH])
