@@ -265,13 +265,13 @@ def bench_serving(tb, graphs, seconds: float = 1.5, rates=(0.25, 0.5, 0.8)):
     edges removed, seed 3), (graph, source) uniform (seed 5).  First the
     saturated rate (a burst of queries submitted at once), then Poisson
     arrivals at fractions of it; latency = submit -> S, sigma, overflow flags
-    in the query's pinned host buffers."""
+    in the query's pinned host buffers.  The same streams then go to a
+    per-source loop: the same server with one slot and one query per group
+    (each query a single-source BFS, in arrival order)."""
     cs = [graphs.rmat(sc, 16, 0.57, 0.19, 0.19, seed=sc, relabel=True) for sc in range(16, 23)]
     cs.append(graphs.grid(1024, 1024, 0.1, seed=3, relabel=True))
     names = [f"rmat{sc}" for sc in range(16, 23)] + ["grid1024"]
     gs = [tb.Graph(g.CP, g.IC, symmetric=True) for g in cs]
-    srv = tb.Server(gs)
-    caps = [srv.info(i)["batch_cap"] for i in range(len(gs))]
     # result buffers: a ring of pinned rows per graph, reused round-robin
     rings = []
     for g in cs:
@@ -280,7 +280,7 @@ def bench_serving(tb, graphs, seconds: float = 1.5, rates=(0.25, 0.5, 0.8)):
                       tb.host_array(k * g.n, np.uint8)))
     use = [0] * len(cs)
 
-    def submit(gi, r):
+    def submit(srv, gi, r):
         k, S, sig, ovf = rings[gi]
         j = use[gi] % k
         use[gi] += 1
@@ -288,64 +288,79 @@ def bench_serving(tb, graphs, seconds: float = 1.5, rates=(0.25, 0.5, 0.8)):
         return srv.submit(gi, r, S[j * n:(j + 1) * n], sig[j * n:(j + 1) * n],
                           ovf[j * n:(j + 1) * n])
 
-    rng = np.random.default_rng(5)
-
-    def draw(count):
+    def draw(rng, count):
         gi = rng.integers(0, len(cs), size=count)
         rt = [int(rng.integers(0, cs[int(x)].n)) for x in gi]
         return [(int(a), b) for a, b in zip(gi, rt)]
 
-    # warm-up: every graph once (first-use slice tables, kernel loading)
-    for gi in range(len(cs)):
-        submit(gi, 0)
-    srv.drain()
-    srv.poll(4096)
-
-    def collect(expect):
+    def collect(srv, expect):
         out = []
         while len(out) < expect:
             out += srv.poll(4096, wait_us=100000)
         return out
 
-    # saturated: everything at once
-    burst = draw(1536)
-    for gi, r in burst:
-        submit(gi, r)
-    done = collect(len(burst))
-    t_first = min(d["submit_us"] for d in done)
-    t_last = max(d["done_us"] for d in done)
-    sat_qps = len(done) / ((t_last - t_first) * 1e-6)
-    lat = np.array([d["done_us"] - d["submit_us"] for d in done]) * 1e-3
+    def run(srv, burst_n, rates_qps):
+        rng = np.random.default_rng(5)
+        # warm-up: every graph once (first-use slice tables, kernel loading)
+        for gi in range(len(cs)):
+            submit(srv, gi, 0)
+        srv.drain()
+        srv.poll(4096)
+        # saturated: everything at once
+        burst = draw(rng, burst_n)
+        for gi, r in burst:
+            submit(srv, gi, r)
+        done = collect(srv, len(burst))
+        t_first = min(d["submit_us"] for d in done)
+        t_last = max(d["done_us"] for d in done)
+        sat_qps = len(done) / ((t_last - t_first) * 1e-6)
+        lat = np.array([d["done_us"] - d["submit_us"] for d in done]) * 1e-3
+        out = {"saturated": {"queries": len(done), "queries_per_s": sat_qps,
+                             "p50_ms": float(np.percentile(lat, 50)),
+                             "p99_ms": float(np.percentile(lat, 99)),
+                             "mean_batch": float(np.mean([d["batch"] for d in done]))},
+               "poisson": []}
+        for rate in (rates_qps(sat_qps) if callable(rates_qps) else rates_qps):
+            count = max(64, int(rate * seconds))
+            arr = np.cumsum(rng.exponential(1.0 / rate, size=count))
+            qs = draw(rng, count)
+            t0 = time.perf_counter()
+            for (gi, r), ta in zip(qs, arr):
+                while time.perf_counter() - t0 < ta:
+                    pass
+                submit(srv, gi, r)
+            done = collect(srv, count)
+            lat = np.array([d["done_us"] - d["submit_us"] for d in done]) * 1e-3
+            span = (max(d["done_us"] for d in done) - min(d["submit_us"] for d in done)) * 1e-6
+            out["poisson"].append({
+                "offered_qps": rate, "queries": count, "achieved_qps": count / span,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_batch": float(np.mean([d["batch"] for d in done])),
+            })
+        return out
+
+    srv = tb.Server(gs)
+    caps = [srv.info(i)["batch_cap"] for i in range(len(gs))]
+    batched = run(srv, 1536, lambda sat: [f * sat for f in rates])
+    srv.close()
+    for p, f in zip(batched["poisson"], rates):
+        p["offered_fraction"] = f
+    # the per-source loop at the lowest offered rate (and saturated)
+    loop_srv = tb.Server(gs, slots=1, max_batch=1)
+    loop = run(loop_srv, 256, [batched["poisson"][0]["offered_qps"]])
+    loop_srv.close()
     result = {
         "workload": "8 resident graphs: R-MAT scales 16-22 (ef16) and a 1024x1024 grid with "
                     "10% of edges removed; (graph, source) uniform, seed 5; results copied to "
                     "pinned host buffers",
         "graphs": {nm: {"n": g.n, "nz": g.nz, "batch_cap": c} for nm, g, c in zip(names, cs, caps)},
-        "saturated": {"queries": len(done), "queries_per_s": sat_qps,
-                      "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                      "mean_batch": float(np.mean([d["batch"] for d in done]))},
-        "poisson": [],
+        "saturated": batched["saturated"],
+        "poisson": batched["poisson"],
+        "per_source_loop": dict(loop, server="one slot, one query per group: a single-source "
+                                             "BFS per query in arrival order"),
+        "saturated_speedup_vs_per_source_loop":
+            batched["saturated"]["queries_per_s"] / loop["saturated"]["queries_per_s"],
     }
-    for frac in rates:
-        rate = frac * sat_qps
-        count = max(64, int(rate * seconds))
-        arr = np.cumsum(rng.exponential(1.0 / rate, size=count))
-        qs = draw(count)
-        t0 = time.perf_counter()
-        for (gi, r), ta in zip(qs, arr):
-            while time.perf_counter() - t0 < ta:
-                pass
-            submit(gi, r)
-        done = collect(count)
-        lat = np.array([d["done_us"] - d["submit_us"] for d in done]) * 1e-3
-        span = (max(d["done_us"] for d in done) - min(d["submit_us"] for d in done)) * 1e-6
-        result["poisson"].append({
-            "offered_fraction": frac, "offered_qps": rate, "queries": count,
-            "achieved_qps": count / span,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_batch": float(np.mean([d["batch"] for d in done])),
-        })
-    srv.close()
     return result
 
 
