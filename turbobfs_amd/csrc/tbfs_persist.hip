@@ -489,16 +489,35 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
     const uint64_t done_l =
         (b + lane() < nmine) ? (st.vis[wl] | st.skip[wl] | st.fcur[wl]) : ~0ull;
     uint64_t todo = __ballot(done_l != ~0ull);
+    // column bounds of the next word with work are loaded one word ahead:
+    // their latency hides behind the current word's passes
+    int nbeg = 0, nend = 0;
+    if (todo) {
+      const int j = __ffsll((unsigned long long)todo) - 1;
+      const int64_t w = wave + (b + j) * nwaves;
+      if (!((__shfl(done_l, j) >> lane()) & 1ull)) {
+        const int v = (int)(w * 64) + lane();
+        nbeg = g.CP[v];
+        nend = g.CP[v + 1];
+      }
+    }
     while (todo) {
       const int j = __ffsll((unsigned long long)todo) - 1;
       todo &= todo - 1;
       const int64_t w = wave + (b + j) * nwaves;
-      const uint64_t done = __shfl(done_l, j);
       const int v = (int)(w * 64) + lane();
-      int beg = 0, deg = 0;
-      if (!((done >> lane()) & 1ull)) {
-        beg = g.CP[v];
-        deg = g.CP[v + 1] - beg;
+      const int beg = nbeg;
+      const int deg = nend - nbeg;
+      nbeg = 0;
+      nend = 0;
+      if (todo) {
+        const int jn = __ffsll((unsigned long long)todo) - 1;
+        const int64_t wn = wave + (b + jn) * nwaves;
+        if (!((__shfl(done_l, jn) >> lane()) & 1ull)) {
+          const int vn = (int)(wn * 64) + lane();
+          nbeg = g.CP[vn];
+          nend = g.CP[vn + 1];
+        }
       }
       const uint32_t incl = wave_incl_scan((uint32_t)deg);
       const int total = (int)__shfl(incl, 63);
