@@ -131,6 +131,36 @@ int tbfs_graph_destroy(tbfs_graph *g);
 int tbfs_graph_info(const tbfs_graph *g, int32_t *n, int32_t *nz,
                     uint32_t *flags);
 
+/* Degree statistics of a resident graph, computed on the device: what the
+ * reference's drivers print before a run (TurboBFS_CSC_TD/degree.c:38-141:
+ * degree_max, degree_sta, degree_dist / degree_dist_ac, called from
+ * bfstdcsc_main.c:173-230).  `which`: TBFS_DEGREE_IN (column lengths, the
+ * in-neighbours; on a symmetric graph this is the reference's undirected
+ * `degree` vector) or TBFS_DEGREE_OUT (out-list lengths).  max / argmax: the
+ * largest degree and the first vertex that has it; mean over all n
+ * vertices; sd: sample standard deviation (divides by n - 1; 0 when n < 2);
+ * bins: bins[0] counts degree 0, bins[k] (k >= 1) degrees in
+ * [2^(k-1), 2^k), so bins[1] is degree 1 and bins[2] degrees 2..3, as the
+ * reference's accumulated distribution groups them; every vertex is counted
+ * (the reference's loop leaves out degree == max when max is a power of two).
+ * nbins: highest nonempty bin + 1.  The statistics also key the batch
+ * autotuner's table (tbfs_batch_autotune). */
+#define TBFS_DEGREE_IN 0
+#define TBFS_DEGREE_OUT 1
+typedef struct tbfs_degree_stats {
+  int32_t max;
+  int32_t argmax;
+  double mean;
+  double sd;
+  int64_t sum;
+  int64_t bins[33];
+  int32_t nbins;
+  int32_t reserved[3];
+} tbfs_degree_stats;
+
+int tbfs_graph_degree_stats(const tbfs_graph *g, int32_t which,
+                            tbfs_degree_stats *out);
+
 /* A BFS context: one HIP stream plus per-BFS state for graphs of up to
  * n_cap vertices.  One context runs one BFS at a time. */
 int tbfs_ctx_create(int32_t n_cap, tbfs_ctx **out);
@@ -223,10 +253,23 @@ int tbfs_batch_run(tbfs_batch *b, const tbfs_graph *g, const int32_t *roots,
  * batch_alpha {4, 32, 128} (best of two runs each; every candidate gives
  * identical results) and returns the fastest in *best (the other fields from
  * base, or the defaults when base is NULL) and its time in *best_ms.  Costs
- * about 25 batch runs; the batch object then holds the last run's results. */
+ * about 25 batch runs; the batch object then holds the last run's results.
+ * The winner is also recorded in a process-wide table keyed by the graph's
+ * degree statistics (log2 of n, of the maximum and of the mean in-degree)
+ * and the batch's number of 64-source words: a later tbfs_batch_run with
+ * opts->grid == 0 and opts->batch_alpha == 0 (the defaults), on this graph
+ * or any graph with the same key, uses the recorded grid and threshold. */
 int tbfs_batch_autotune(tbfs_batch *b, const tbfs_graph *g, const int32_t *roots,
                         int32_t count, const tbfs_opts *base, tbfs_opts *best,
                         double *best_ms);
+
+/* The tuned grid / batch_alpha the table holds for a batch of `count`
+ * sources on g, written into *o (other fields untouched).  Returns TBFS_OK
+ * when an entry applies, TBFS_ERR_UNSUPPORTED when none does. */
+int tbfs_batch_tuned(const tbfs_graph *g, int32_t count, tbfs_opts *o);
+
+/* Empty the autotuner's table. */
+int tbfs_batch_tune_clear(void);
 
 /* Results of sources first .. first+count-1 of the last batch, one row of n
  * entries per source (S[i*n + v] is the depth of v from roots[first+i]; any

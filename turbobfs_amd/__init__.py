@@ -20,12 +20,15 @@ from .bfs import (  # noqa: F401
     bfs_gpu_td_csc_wa,
     bfs_gpu_tdbu_csc_sc,
     bfs_gpu_tdbu_csc_wa,
+    batch_tune_clear,
+    batch_tuned,
     default_opts,
     host_array,
 )
 
 __all__ = [
     "Batch", "Context", "Graph", "Server", "host_array", "Result", "default_opts", "graphs", "load", "LIB_PATH",
+    "batch_tuned", "batch_tune_clear",
     "device_count", "device_info",
     "TbfsError",
     "bfs_gpu_td_csc_sc", "bfs_gpu_td_csc_wa", "bfs_gpu_bu_csc_sc",

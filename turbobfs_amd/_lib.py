@@ -98,6 +98,23 @@ class QueryDone(C.Structure):
     ]
 
 
+DEGREE_IN = 0
+DEGREE_OUT = 1
+
+
+class DegreeStats(C.Structure):
+    _fields_ = [
+        ("max", C.c_int32),
+        ("argmax", C.c_int32),
+        ("mean", C.c_double),
+        ("sd", C.c_double),
+        ("sum", C.c_int64),
+        ("bins", C.c_int64 * 33),
+        ("nbins", C.c_int32),
+        ("reserved", C.c_int32 * 3),
+    ]
+
+
 # name -> (restype, argtypes); every symbol the public headers declare
 SIGNATURES = {
     # drop-in surface (bfstdcsc.h, bfsbucsc.h, bfstdbucsc.h, bfstdcooc.h)
@@ -115,6 +132,9 @@ SIGNATURES = {
     "tbfs_graph_create_cooc": (C.c_int, [_i32p, _i32p, C.c_int32, C.c_int32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "tbfs_graph_destroy": (C.c_int, [C.c_void_p]),
     "tbfs_graph_info": (C.c_int, [C.c_void_p, _i32p, _i32p, C.POINTER(C.c_uint32)]),
+    "tbfs_graph_degree_stats": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(DegreeStats)]),
+    "tbfs_batch_tuned": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Opts)]),
+    "tbfs_batch_tune_clear": (C.c_int, []),
     "tbfs_ctx_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
     "tbfs_ctx_destroy": (C.c_int, [C.c_void_p]),
     "tbfs_ctx_stream": (C.c_void_p, [C.c_void_p]),

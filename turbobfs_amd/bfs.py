@@ -130,6 +130,19 @@ class Graph:
     def handle(self):
         return self._h
 
+    def degree_stats(self, which: str = "in") -> dict:
+        """Degree statistics computed on the device (max, first vertex with
+        it, mean, sample sd, degree sum, power-of-two bins: bins[0] degree 0,
+        bins[k] degrees in [2^(k-1), 2^k)), of the in-lists or out-lists."""
+        if which not in ("in", "out"):
+            raise ValueError(f"which must be 'in' or 'out', not {which!r}")
+        ds = _lib.DegreeStats()
+        w = _lib.DEGREE_IN if which == "in" else _lib.DEGREE_OUT
+        check(_lib.load().tbfs_graph_degree_stats(self._h, w, C.byref(ds)),
+              "tbfs_graph_degree_stats")
+        return {"max": ds.max, "argmax": ds.argmax, "mean": ds.mean, "sd": ds.sd,
+                "sum": ds.sum, "bins": [int(ds.bins[k]) for k in range(ds.nbins)]}
+
     def close(self):
         if self._h:
             _lib.load().tbfs_graph_destroy(self._h)
@@ -301,6 +314,21 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def batch_tuned(g: Graph, count: int) -> Opts | None:
+    """The autotuner table's grid / batch_alpha for a batch of `count`
+    sources on g (None when no entry applies)."""
+    o = default_opts()
+    rc = _lib.load().tbfs_batch_tuned(g.handle, int(count), C.byref(o))
+    if rc == _lib.TBFS_ERR_UNSUPPORTED:
+        return None
+    check(rc, "tbfs_batch_tuned")
+    return o
+
+
+def batch_tune_clear() -> None:
+    check(_lib.load().tbfs_batch_tune_clear(), "tbfs_batch_tune_clear")
 
 
 def host_array(count: int, dtype) -> np.ndarray:

@@ -29,6 +29,8 @@ struct tbfs_graph {
   uint64_t* bsl = nullptr;   // batched pull slices of the in-lists (v | slice << 32),
   int32_t nbsl = 0;          // built on first use
   bool wide_counts = false;  // a batch on this graph needed 64-bit counts: start with them
+  bool deg_in_valid = false; // deg_in computed (first use)
+  tbfs_degree_stats deg_in{};
 };
 
 // Per-BFS device state bound to one stream.
@@ -109,6 +111,9 @@ inline int grid_for(int64_t items, int block, int cap) {
   return (int)g;
 }
 
+// Degree statistics of g's in-lists (which == TBFS_DEGREE_IN) or out-lists,
+// computed on stream s (waits for it); the in-list statistics are kept in g.
+int graph_degree_stats(tbfs_graph* g, int32_t which, hipStream_t s, tbfs_degree_stats* out);
 int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
                      int32_t nz, uint32_t flags, tbfs_graph** out);
 int graph_create_cooc(const int32_t* I, const int32_t* J, int32_t n,

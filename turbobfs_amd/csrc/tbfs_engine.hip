@@ -4,6 +4,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -322,6 +323,51 @@ int graph_build_bigc(tbfs_graph* g) {
     return TBFS_OK;
   }
   return build_slices(g->RP, g->n, &g->bigo, &g->nbigo);
+}
+
+int graph_degree_stats(tbfs_graph* g, int32_t which, hipStream_t s, tbfs_degree_stats* out) {
+  if (!g || !out || (which != TBFS_DEGREE_IN && which != TBFS_DEGREE_OUT)) return TBFS_ERR_INVALID;
+  if (which == TBFS_DEGREE_IN && g->deg_in_valid) {
+    *out = g->deg_in;
+    return TBFS_OK;
+  }
+  const int32_t* ptr = which == TBFS_DEGREE_IN ? g->CP : g->RP;
+  if (!ptr) return TBFS_ERR_UNSUPPORTED;
+  DegAcc* acc = nullptr;
+  int rc = dev_alloc(&acc, 1);
+  if (rc) return rc;
+  DegAcc h{};
+  hipError_t e = hipMemsetAsync(acc, 0, sizeof(DegAcc), s);
+  if (e == hipSuccess) e = kern::degree_stats(s, ptr, g->n, acc);
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, acc, sizeof(DegAcc), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dev_free(acc);
+  if (e != hipSuccess) return hip_fail(e, "degree_stats", __FILE__, __LINE__);
+  tbfs_degree_stats r{};
+  const int64_t n = g->n;
+  r.max = n > 0 ? (int32_t)(h.maxkey >> 32) : 0;
+  r.argmax = n > 0 ? (int32_t)~(uint32_t)h.maxkey : 0;
+  r.sum = (int64_t)h.sum;
+  r.mean = n > 0 ? (double)h.sum / (double)n : 0.0;
+  // sample variance from exact integer sums: (n sum(d^2) - sum(d)^2) / (n (n - 1))
+  if (n > 1) {
+    const unsigned __int128 sq = ((unsigned __int128)h.sq_hi << 32) + h.sq_lo;
+    const unsigned __int128 num = (unsigned __int128)n * sq -
+                                  (unsigned __int128)h.sum * (unsigned __int128)h.sum;
+    const long double var = (long double)num / ((long double)n * (long double)(n - 1));
+    r.sd = (double)std::sqrt(var);
+  }
+  r.nbins = 0;
+  for (int k = 0; k < 33; ++k) {
+    r.bins[k] = (int64_t)h.bins[k];
+    if (h.bins[k]) r.nbins = k + 1;
+  }
+  if (which == TBFS_DEGREE_IN) {
+    g->deg_in = r;
+    g->deg_in_valid = true;
+  }
+  *out = r;
+  return TBFS_OK;
 }
 
 // ----------------------------------------------------------------------------
@@ -1078,6 +1124,15 @@ int tbfs_graph_info(const tbfs_graph* g, int32_t* n, int32_t* nz,
   if (nz) *nz = g->nz;
   if (flags) *flags = g->flags;
   return TBFS_OK;
+}
+
+int tbfs_graph_degree_stats(const tbfs_graph* g, int32_t which, tbfs_degree_stats* out) {
+  if (!g || !out) return TBFS_ERR_INVALID;
+  hipStream_t s = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  const int rc = graph_degree_stats(const_cast<tbfs_graph*>(g), which, s, out);
+  (void)hipStreamDestroy(s);
+  return rc;
 }
 
 int tbfs_ctx_create(int32_t n_cap, tbfs_ctx** out) {

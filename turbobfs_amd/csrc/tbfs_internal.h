@@ -144,6 +144,17 @@ struct PersistParams {
   LoopState* loop;  // [2] (chained loop only)
 };
 
+// Degree statistics of the lists delimited by ptr[0..n]: acc (zeroed by the
+// caller) receives the maximum key (degree << 32 | ~vertex), the degree sum,
+// the sum of squares as low / high 32-bit halves, and 33 power-of-two bins.
+struct DegAcc {
+  unsigned long long maxkey;
+  unsigned long long sum;
+  unsigned long long sq_lo;
+  unsigned long long sq_hi;
+  unsigned long long bins[33];
+};
+
 // Kernel launch wrappers.  Every launch is made in the translation unit that
 // defines the kernel.
 namespace kern {
@@ -175,6 +186,7 @@ hipError_t expand_cols(hipStream_t s, const int32_t* CP, int32_t n,
                        int32_t* col);
 hipError_t histogram(hipStream_t s, const int32_t* keys, int32_t nz,
                      int32_t* cnt);
+hipError_t degree_stats(hipStream_t s, const int32_t* ptr, int32_t n, DegAcc* acc);
 hipError_t check_csc(hipStream_t s, const int32_t* CP, const int32_t* IC,
                      int32_t n, int32_t nz, int32_t* bad);
 

@@ -305,6 +305,59 @@ __global__ void k_histogram(const int32_t* keys, int32_t nz, int32_t* cnt) {
     atomicAdd(&cnt[keys[k]], 1);
 }
 
+// Degree statistics (see DegAcc): per workgroup, LDS bins and wavefront
+// reductions, then one device atomic per workgroup and field.  Bin of degree
+// d: 0 for d == 0, else floor(log2 d) + 1.
+__global__ __launch_bounds__(256) void k_degree_stats(const int32_t* ptr, int32_t n,
+                                                      DegAcc* acc) {
+  __shared__ unsigned long long bins[33];
+  __shared__ unsigned long long red[4][4];
+  if (threadIdx.x < 33) bins[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long mk = 0, sum = 0, lo = 0, hi = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    const uint32_t d = (uint32_t)(ptr[v + 1] - ptr[v]);
+    const unsigned long long key = ((unsigned long long)d << 32) | (uint32_t)~(uint32_t)v;
+    mk = key > mk ? key : mk;
+    sum += d;
+    const unsigned long long sq = (unsigned long long)d * d;
+    lo += sq & 0xffffffffull;
+    hi += sq >> 32;
+    atomicAdd(&bins[d ? 32 - __clz((int)d) : 0], 1ull);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(mk, off);
+    mk = o > mk ? o : mk;
+    sum += __shfl_xor(sum, off);
+    lo += __shfl_xor(lo, off);
+    hi += __shfl_xor(hi, off);
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wv][0] = mk;
+    red[wv][1] = sum;
+    red[wv][2] = lo;
+    red[wv][3] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = 0, t1 = 0, t2 = 0, t3 = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+      m = red[i][0] > m ? red[i][0] : m;
+      t1 += red[i][1];
+      t2 += red[i][2];
+      t3 += red[i][3];
+    }
+    if (m) atomicMax(&acc->maxkey, m);
+    if (t1) atomicAdd(&acc->sum, t1);
+    if (t2) atomicAdd(&acc->sq_lo, t2);
+    if (t3) atomicAdd(&acc->sq_hi, t3);
+  }
+  if (threadIdx.x < 33 && bins[threadIdx.x]) atomicAdd(&acc->bins[threadIdx.x], bins[threadIdx.x]);
+}
+
 // bad[0] |= 1: CP not monotone / CP[0] != 0 / CP[n] != nz; 2: IC out of range
 __global__ void k_check_csc(const int32_t* CP, const int32_t* IC, int32_t n,
                             int32_t nz, int32_t* bad) {
@@ -410,6 +463,14 @@ hipError_t histogram(hipStream_t s, const int32_t* keys, int32_t nz,
   if (grid > 65536) grid = 65536;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(k_histogram, dim3(grid), dim3(256), 0, s, keys, nz, cnt);
+  return hipGetLastError();
+}
+
+hipError_t degree_stats(hipStream_t s, const int32_t* ptr, int32_t n, DegAcc* acc) {
+  int grid = (int)(((int64_t)n + 255) / 256);
+  if (grid > 4096) grid = 4096;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_degree_stats, dim3(grid), dim3(256), 0, s, ptr, n, acc);
   return hipGetLastError();
 }
 
