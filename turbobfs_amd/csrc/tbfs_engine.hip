@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -325,11 +326,18 @@ int graph_build_bigc(tbfs_graph* g) {
   return build_slices(g->RP, g->n, &g->bigo, &g->nbigo);
 }
 
+// Guards the in-degree statistics kept in each graph (a server's dispatcher
+// and the caller's threads may ask for them at the same time).
+static std::mutex g_deg_mu;
+
 int graph_degree_stats(tbfs_graph* g, int32_t which, hipStream_t s, tbfs_degree_stats* out) {
   if (!g || !out || (which != TBFS_DEGREE_IN && which != TBFS_DEGREE_OUT)) return TBFS_ERR_INVALID;
-  if (which == TBFS_DEGREE_IN && g->deg_in_valid) {
-    *out = g->deg_in;
-    return TBFS_OK;
+  if (which == TBFS_DEGREE_IN) {
+    std::lock_guard<std::mutex> lk(g_deg_mu);
+    if (g->deg_in_valid) {
+      *out = g->deg_in;
+      return TBFS_OK;
+    }
   }
   const int32_t* ptr = which == TBFS_DEGREE_IN ? g->CP : g->RP;
   if (!ptr) return TBFS_ERR_UNSUPPORTED;
@@ -363,6 +371,7 @@ int graph_degree_stats(tbfs_graph* g, int32_t which, hipStream_t s, tbfs_degree_
     if (h.bins[k]) r.nbins = k + 1;
   }
   if (which == TBFS_DEGREE_IN) {
+    std::lock_guard<std::mutex> lk(g_deg_mu);
     g->deg_in = r;
     g->deg_in_valid = true;
   }
