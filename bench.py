@@ -101,9 +101,10 @@ def cpu_baseline(g, sources, budget_s: float):
     }
 
 
-def bench_variants(tb, g, gr, ctx, sources, nsrc: int = 16):
+def bench_variants(tb, g, gr, ctx, sources, nsrc: int = 16, names=None):
     """Every reference variant's kernel mapping on the same resident graph:
-    GTEPS and algorithmic GB/s (B_alg / time) per variant, sources in turn."""
+    GTEPS and algorithmic GB/s (B_alg / time) per variant, sources in turn
+    (`names`: only these variants)."""
     from turbobfs_amd import _lib
     H = _lib.LOOP_HOST
     variants = [
@@ -120,6 +121,8 @@ def bench_variants(tb, g, gr, ctx, sources, nsrc: int = 16):
     n = g.n
     out = {}
     for name, kw in variants:
+        if names and name not in names:
+            continue
         o = tb.default_opts(**kw)
         ctx.run_many(gr, roots[:1], o)
         ctx.mark(0)
@@ -386,7 +389,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the per-variant (TD / BU / TDBU / COOC) GTEPS table")
-    ap.add_argument("--only", choices=["batch", "multigraph", "grid", "serve"], default=None,
+    ap.add_argument("--variants", default="",
+                    help="comma-separated subset of the per-variant table (default: all)")
+    ap.add_argument("--only", choices=["batch", "multigraph", "grid", "serve", "variants"],
+                    default=None,
                     help="run only that measurement and print its JSON object (for profiling)")
     ap.add_argument("--no-tune", action="store_true", help="batched run with default options only")
     ap.add_argument("--grid-loops", default="chained,graph,persistent_1024x64,host",
@@ -409,6 +415,16 @@ def main():
     if tb.device_count() < 1:
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
+    if args.only == "variants":
+        g = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)
+        sources = graphs.sample_sources(graphs.nonzero_degree_vertices(g), args.sources, seed=2)
+        gr = tb.Graph(g.CP, g.IC, symmetric=True)
+        ctx = tb.Context(g.n)
+        if args.calib_mib > 0:  # reference dispatch for byte counters
+            ctx.calibrate_copy(args.calib_mib << 20, 1)
+        names = [x for x in args.variants.split(",") if x]
+        print(json.dumps({"variants": bench_variants(tb, g, gr, ctx, sources, names=names)}))
+        return
     if args.only:
         fn = {"batch": lambda: bench_batched(tb, graphs, tune=not args.no_tune,
                                              loop=not args.no_loop),
@@ -506,7 +522,8 @@ def main():
         },
     }
     if not args.no_variants:
-        line["variants"] = bench_variants(tb, g, gr, ctx, sources)
+        line["variants"] = bench_variants(tb, g, gr, ctx, sources,
+                                          names=[x for x in args.variants.split(",") if x])
     if not args.no_batch:
         line["batched"] = bench_batched(tb, graphs)
         line["multi_graph"] = bench_multigraph(tb, graphs)
