@@ -78,6 +78,11 @@ struct tbfs_server {
   // single-source query, and one batch of 1, 2, 4, 8, 16 words (0: unknown)
   std::vector<double> t_single;
   std::vector<std::array<double, 5>> t_batch;
+  // completed runs of each kind: the first of each pays one-time costs (the
+  // graph's batch slice table, a re-run with wider counts, the single-source
+  // level budget found by doubling) and is not taken as a cost sample
+  std::vector<int> n_single;
+  std::vector<std::array<int, 5>> n_batch;
   std::vector<int> budget;  // level kernels per single-source run, per graph
   int32_t nmax = 0;
   std::vector<std::deque<Query>> pending;
@@ -111,10 +116,11 @@ int word_index(int words) {
 void complete(tbfs_server* s, Slot& sl, int status, double t_done) {
   if (status == TBFS_OK && sl.graph >= 0 && !sl.q.empty()) {
     const double ms = (t_done - sl.t_start) * 1e-3;
-    double& t = sl.words ? s->t_batch[sl.graph][word_index(sl.words)]
-                         : s->t_single[sl.graph];
+    const int wi = word_index(sl.words);
+    double& t = sl.words ? s->t_batch[sl.graph][wi] : s->t_single[sl.graph];
+    int& runs = sl.words ? s->n_batch[sl.graph][wi] : s->n_single[sl.graph];
     const double x = sl.words ? ms : ms / (double)sl.q.size();
-    t = t > 0 ? 0.7 * t + 0.3 * x : x;
+    if (runs++ > 0) t = t > 0 ? 0.7 * t + 0.3 * x : x;
   }
   std::lock_guard<std::mutex> lk(s->mu);
   for (const Query& q : sl.q) {
@@ -202,7 +208,11 @@ bool step_slot(tbfs_server* s, Slot& sl) {
 
 // Single-source runs or one batch: single runs while k queries cost less
 // that way than one batch of their word count, by the costs measured so far
-// (a graph's batch cost is measured first, then one single query).
+// (a graph's batch cost is measured first, then one single query).  A slot
+// takes one single run at a time: more waiting queries of the graph go to
+// the other idle slots, where their (latency-bound) level loops overlap.
+constexpr size_t kSingleMax = 1;
+
 bool use_single(tbfs_server* s, int g, size_t* k) {
   const int wi = word_index(batch_words_for((int32_t)*k));
   const double tb = s->t_batch[g][wi];
@@ -212,7 +222,9 @@ bool use_single(tbfs_server* s, int g, size_t* k) {
     *k = 1;
     return true;
   }
-  return (double)*k * t1 < tb;
+  if ((double)*k * t1 >= tb) return false;
+  *k = std::min(*k, kSingleMax);
+  return true;
 }
 
 // Form a batch for an idle slot; true if one was started.
@@ -350,6 +362,8 @@ int tbfs_server_create(const tbfs_graph* const* graphs, int32_t ngraphs,
   s->pending.resize(ngraphs);
   s->t_single.assign(ngraphs, 0.0);
   s->t_batch.assign(ngraphs, std::array<double, 5>{});
+  s->n_single.assign(ngraphs, 0);
+  s->n_batch.assign(ngraphs, std::array<int, 5>{});
   s->budget.assign(ngraphs, 8);
   s->nmax = nmax;
   for (int i = 0; i < ngraphs; ++i) {
