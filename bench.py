@@ -507,6 +507,11 @@ def main():
             "push_levels": float(np.mean([st["push_levels"] for st in stats])),
             "pull_levels": float(np.mean([st["pull_levels"] for st in stats])),
             "bfs_ms_median": float(np.median([st["ms"] for st in stats])),
+            # per-source GTEPS from each BFS's own event pair
+            "gteps_median": float(np.median([st["edges"] / 2 / (st["ms"] * 1e-3) / 1e9
+                                             for st in stats])),
+            "gteps_harmonic_mean": float(len(stats) / sum((st["ms"] * 1e-3) * 1e9 /
+                                                          (st["edges"] / 2) for st in stats)),
             "in_degree": {k: v for k, v in gr.degree_stats("in").items() if k != "bins"},
         },
         "copy_kernel_gbs": (2 * calib_bytes / (copy_ms * 1e-3) / 1e9) if copy_ms else None,
