@@ -91,7 +91,11 @@ typedef struct tbfs_opts {
                                ones if some count reaches 2^32; 64 = 64-bit
                                counts only; 32 = 32-bit only (a count that
                                reaches 2^32 is then flagged as overflowed) */
-  int32_t reserved[4];
+  int32_t batch_order; /* batches: 0 = run on a copy of the graph with the
+                          vertices numbered by decreasing in-degree when the
+                          degrees are skewed (sd > 2 x mean), 1 = always,
+                          -1 = never; results are in the caller's numbering */
+  int32_t reserved[3];
 } tbfs_opts;
 
 typedef struct tbfs_stats {

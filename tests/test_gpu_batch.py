@@ -44,12 +44,14 @@ def _expected(g, roots):
     return S, sig, ovf, edges, levels
 
 
-def _check_batch(g, roots, batch=None, symmetric=True, first=0, count=None, alpha=0.0):
+def _check_batch(g, roots, batch=None, symmetric=True, first=0, count=None, alpha=0.0,
+                 order=0):
     roots = np.asarray(roots, np.int32)
     gr = tb.Graph(g.CP, g.IC, symmetric=symmetric)
     b = batch if batch is not None else tb.Batch(g.n, max(64, len(roots)))
-    print(f"[batch] n={g.n} nz={g.nz} sources={len(roots)} alpha={alpha}", flush=True)
-    st = b.run(gr, roots, tb.default_opts(batch_alpha=alpha))
+    print(f"[batch] n={g.n} nz={g.nz} sources={len(roots)} alpha={alpha} order={order}",
+          flush=True)
+    st = b.run(gr, roots, tb.default_opts(batch_alpha=alpha, batch_order=order))
     S, sig, ovf = b.read(first, count)
     k = S.shape[0]
     eS, esig, eovf, edges, levels = _expected(g, roots[first:first + k])
@@ -196,6 +198,40 @@ def test_batch_direction_invariance(gpu, case, alpha):
         assert st["pull_levels"] == 0 and st["push_levels"] == st["levels"]
     if alpha > 1e6:  # a frontier without out-edges (if any) still goes top-down
         assert st["pull_levels"] >= st["levels"] - 1
+
+
+@pytest.mark.parametrize("order", [-1, 1])
+@pytest.mark.parametrize("case", ["rmat13", "grid_overflow", "directed_tina", "star"])
+def test_batch_degree_order(gpu, case, order):
+    """Batches on the degree-ordered copy of the graph (order 1; by default
+    only for skewed degrees) and on the graph itself (order -1) give every
+    source exactly the oracle's result in the caller's numbering."""
+    sym = True
+    if case == "rmat13":
+        g = graphs.rmat(13, 16, 0.57, 0.19, 0.19, seed=6, relabel=True)
+        roots = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 130, seed=6)
+    elif case == "grid_overflow":
+        g = graphs.grid(48, 48, 0.0, seed=0, relabel=False)
+        roots = np.array([0, 47, 48 * 48 - 1, 1000, 5], np.int32)
+    elif case == "directed_tina":
+        g = mtx.read_csc(golden("Tina_AskCal.mtx"))
+        roots = np.arange(g.n, dtype=np.int32)
+        sym = False
+    else:
+        n = 3000
+        g = _csc_from_edges(n, np.zeros(n - 1, np.int64), np.arange(1, n))
+        roots = np.array([0, 1, 2999, 17], np.int32)
+    _check_batch(g, roots, symmetric=sym, order=order)
+
+
+def test_batch_degree_order_default_and_invalid(gpu):
+    g = graphs.rmat(12, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)
+    roots = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 64, seed=1)
+    _check_batch(g, roots, order=0)  # skewed: runs on the copy
+    gr = tb.Graph(g.CP, g.IC, symmetric=True)
+    b = tb.Batch(g.n, 64)
+    with pytest.raises(tb.TbfsError):
+        b.run(gr, roots, tb.default_opts(batch_order=2))
 
 
 def test_batch_autotune_keeps_results(gpu):

@@ -59,7 +59,8 @@ class Opts(C.Structure):
         ("fb_lds", C.c_int32),
         ("batch_alpha", C.c_float),
         ("batch_count_bits", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("batch_order", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 

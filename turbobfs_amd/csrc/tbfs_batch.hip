@@ -137,6 +137,7 @@ struct BatchParams {
   BLoop* loop;    // [2]
   RunStats* stats;
   const int32_t* roots;
+  const int32_t* perm;  // caller's vertex id -> id in the graph run (null: same)
 };
 
 __device__ __forceinline__ uint64_t rl64(uint64_t x, int l) {
@@ -204,7 +205,7 @@ template <int W, typename CT>
 __global__ __launch_bounds__(256) void k_batch_seed(BatchParams p) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= p.nsrc) return;
-  const int r = p.roots[j];
+  const int r = p.perm ? p.perm[p.roots[j]] : p.roots[j];
   const uint64_t bit = 1ull << (j & 63);
   const unsigned long long old =
       atomicOr((unsigned long long*)&p.F[0][(int64_t)(j >> 6) * p.n + r], (unsigned long long)bit);
@@ -812,13 +813,14 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int3
 // Per-source rows out of the [n][B] layout: sources first .. first+count-1
 // (count <= 64) into So[count][n], sgo[count][n], ovo[count][n].  Pairs whose
 // bit is clear in `seen` (never reached) read S = -1, sigma = 0: the level
-// kernels never write them.
+// kernels never write them.  perm (may be null): the caller's vertex v is
+// vertex perm[v] of the graph the batch ran on.
 template <typename CT>
 __global__ __launch_bounds__(256) void k_batch_extract(
     const int32_t* __restrict__ S, const CT* __restrict__ sigma,
     const uint64_t* __restrict__ ovfw, const uint64_t* __restrict__ seen, int32_t n,
     int32_t first, int32_t count, int32_t* So, uint64_t* sgo,
-    uint8_t* ovo) {
+    uint8_t* ovo, const int32_t* __restrict__ perm) {
   __shared__ int32_t ts[64][65];
   __shared__ uint64_t tg[64][65];
   __shared__ uint8_t to[64][65];
@@ -828,7 +830,7 @@ __global__ __launch_bounds__(256) void k_batch_extract(
     const int64_t v = v0 + r;
     if (v < n && tx < count) {
       const int j = first + tx;
-      const int64_t iw = (int64_t)(j >> 6) * n + v;
+      const int64_t iw = (int64_t)(j >> 6) * n + (perm ? perm[v] : v);
       const bool reached = (seen[iw] >> (j & 63)) & 1ull;
       ts[r][tx] = reached ? S[iw * 64 + (j & 63)] : -1;
       tg[r][tx] = reached ? (uint64_t)sigma[iw * 64 + (j & 63)] : 0ull;
@@ -873,6 +875,9 @@ struct tbfs_batch {
   uint64_t* xsig = nullptr;
   uint8_t* xovf = nullptr;
   int64_t xcap = 0;           // entries of the staging buffers
+  int32_t* xperm = nullptr;   // the last batch's id map (a copy: the graph may go)
+  int64_t xperm_cap = 0;
+  bool use_perm = false;
   int cus = 0;
   int chain_k = 8;
   // the batch in flight (or the last one)
@@ -915,6 +920,7 @@ void batch_free(tbfs_batch* b) {
   dev_free(b->xS);
   dev_free(b->xsig);
   dev_free(b->xovf);
+  dev_free(b->xperm);
   if (b->stats_h) (void)hipHostFree(b->stats_h);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev_step) (void)hipEventDestroy(b->ev_step);
@@ -1031,6 +1037,7 @@ BatchParams make_bparams(const tbfs_batch* b) {
   p.loop = b->loop;
   p.stats = b->stats;
   p.roots = b->roots;
+  p.perm = b->use_perm ? b->xperm : nullptr;
   return p;
 }
 
@@ -1169,8 +1176,8 @@ int batch_start(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32
   if (o_in) o = *o_in; else tbfs_opts_default(&o);
   if (o.batch_count_bits != 0 && o.batch_count_bits != 32 && o.batch_count_bits != 64)
     return TBFS_ERR_INVALID;
-  int rc = graph_ensure_batch_slices(g);
-  if (rc) return rc;
+  if (o.batch_order < -1 || o.batch_order > 1) return TBFS_ERR_INVALID;
+  int rc = TBFS_OK;
   if (o.grid == 0 && o.batch_alpha == 0 && !tune_empty()) {  // defaults: the tuned values
     TuneKey k;
     TuneEntry e;
@@ -1181,6 +1188,26 @@ int batch_start(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32
       o.batch_alpha = e.alpha;
     }
   }
+  // the graph the batch runs on: g, or its degree-ordered copy (roots and
+  // result rows are mapped through a copy of its id map)
+  const int32_t* pm = nullptr;
+  {
+    tbfs_graph* gv = nullptr;
+    rc = graph_batch_view(g, o.batch_order, b->stream, &gv, &pm);
+    if (rc) return rc;
+    b->use_perm = pm != nullptr;
+    if (pm && b->xperm_cap < g->n) {
+      HIPCHK(hipStreamSynchronize(b->stream));
+      dev_free(b->xperm);
+      b->xperm_cap = 0;
+      rc = dev_alloc(&b->xperm, (size_t)g->n);
+      if (rc) return rc;
+      b->xperm_cap = g->n;
+    }
+    g = gv;
+  }
+  rc = graph_ensure_batch_slices(g);
+  if (rc) return rc;
   b->g = g;
   b->n = g->n;
   b->o = o;
@@ -1193,6 +1220,9 @@ int batch_start(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32
   HIPCHK(hipMemcpyAsync(b->roots, roots, (size_t)count * sizeof(int32_t),
                         hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipEventRecord(b->ev0, b->stream));
+  if (pm)  // inside the timed span: it is part of running on the copy
+    HIPCHK(hipMemcpyAsync(b->xperm, pm, (size_t)g->n * sizeof(int32_t),
+                          hipMemcpyDeviceToDevice, b->stream));
   rc = enqueue(b, 0, b->chain_k);
   if (rc) return rc;
   b->active = true;
@@ -1299,11 +1329,11 @@ int batch_enqueue_rows(tbfs_batch* b, int32_t first, int32_t count, int32_t* con
     if (b->ct == 4)
       hipLaunchKernelGGL(k_batch_extract<uint32_t>, dim3((n + 63) / 64), dim3(256), 0, s, b->S,
                          (const uint32_t*)b->sigma, b->ovfw, b->seen_final, n, first + f, c,
-                         b->xS, b->xsig, b->xovf);
+                         b->xS, b->xsig, b->xovf, b->use_perm ? b->xperm : nullptr);
     else
       hipLaunchKernelGGL(k_batch_extract<uint64_t>, dim3((n + 63) / 64), dim3(256), 0, s, b->S,
                          (const uint64_t*)b->sigma, b->ovfw, b->seen_final, n, first + f, c,
-                         b->xS, b->xsig, b->xovf);
+                         b->xS, b->xsig, b->xovf, b->use_perm ? b->xperm : nullptr);
     HIPCHK(hipGetLastError());
     for (int32_t r = 0; r < c; ++r) {
       const size_t off = (size_t)r * n;

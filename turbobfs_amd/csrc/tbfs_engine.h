@@ -31,6 +31,13 @@ struct tbfs_graph {
   bool wide_counts = false;  // a batch on this graph needed 64-bit counts: start with them
   bool deg_in_valid = false; // deg_in computed (first use)
   tbfs_degree_stats deg_in{};
+  // batches: a copy numbered by decreasing in-degree (hubs' state rows
+  // together), built on first use; bperm maps an id of this graph to its id
+  // there; bview_default: -1 not decided, 0 / 1 whether batches use it by
+  // default (skewed degrees)
+  tbfs_graph* bview = nullptr;
+  int32_t* bperm = nullptr;
+  int bview_default = -1;
 };
 
 // Per-BFS device state bound to one stream.
@@ -111,6 +118,11 @@ inline int grid_for(int64_t items, int block, int cap) {
   return (int)g;
 }
 
+// The graph a batch runs on: g itself, or its degree-ordered copy (order:
+// the batch_order option) with *perm mapping g's ids to the copy's (else
+// null); built on first use, on stream s (waits for it).
+int graph_batch_view(tbfs_graph* g, int32_t order, hipStream_t s, tbfs_graph** view,
+                     const int32_t** perm);
 // Degree statistics of g's in-lists (which == TBFS_DEGREE_IN) or out-lists,
 // computed on stream s (waits for it); the in-list statistics are kept in g.
 int graph_degree_stats(tbfs_graph* g, int32_t which, hipStream_t s, tbfs_degree_stats* out);

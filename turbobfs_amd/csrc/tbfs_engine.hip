@@ -52,6 +52,8 @@ namespace tbfs {
 
 void graph_free(tbfs_graph* g) {
   if (!g) return;
+  graph_free(g->bview);
+  dev_free(g->bperm);
   if (g->RP != g->CP) dev_free(g->RP);
   if (g->JR != g->IC) dev_free(g->JR);
   if (g->J_owned) dev_free(g->J);
@@ -376,6 +378,89 @@ int graph_degree_stats(tbfs_graph* g, int32_t which, hipStream_t s, tbfs_degree_
     g->deg_in_valid = true;
   }
   *out = r;
+  return TBFS_OK;
+}
+
+// Degree-ordered copy of g: vertices sorted by decreasing in-degree (stable,
+// so ties keep their order), every list renamed and placed in its column's
+// new slot; the copy is then set up like any device-resident CSC graph.
+static int graph_build_view(tbfs_graph* g, hipStream_t s) {
+  const int32_t n = g->n, nz = g->nz;
+  uint32_t *keys = nullptr, *keys_out = nullptr;
+  int32_t *ids = nullptr, *order = nullptr, *deg2 = nullptr, *CP2 = nullptr, *col = nullptr,
+          *IC2 = nullptr, *perm = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_sort = 0, tmp_scan = 0;
+  tbfs_graph* v = nullptr;
+  int rc = dev_alloc(&keys, (size_t)n);
+  if (!rc) rc = dev_alloc(&keys_out, (size_t)n);
+  if (!rc) rc = dev_alloc(&ids, (size_t)n);
+  if (!rc) rc = dev_alloc(&order, (size_t)n);
+  if (!rc) rc = dev_alloc(&deg2, (size_t)n + 1);
+  if (!rc) rc = dev_alloc(&CP2, (size_t)n + 1);
+  if (!rc) rc = dev_alloc(&col, (size_t)std::max(nz, 1));
+  if (!rc) rc = dev_alloc(&IC2, (size_t)std::max(nz, 1));
+  if (!rc) rc = dev_alloc(&perm, (size_t)n);
+  if (rc) goto out;
+  {
+    hipError_t e = kern::order_keys(s, g->CP, n, keys, ids);
+    if (e == hipSuccess)
+      e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_sort, keys, keys_out, ids, order, n, 0,
+                                             32, s);
+    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_scan, deg2, CP2, n + 1, s);
+    if (e != hipSuccess) { rc = hip_fail(e, "view sizes", __FILE__, __LINE__); goto out; }
+    rc = dev_alloc((char**)&tmp, std::max(tmp_sort, tmp_scan));
+    if (rc) goto out;
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_sort, keys, keys_out, ids, order, n, 0, 32, s);
+    if (e == hipSuccess) e = kern::order_perm(s, order, g->CP, n, perm, deg2);
+    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_scan, deg2, CP2, n + 1, s);
+    if (e == hipSuccess && nz > 0) e = kern::expand_cols(s, g->CP, n, col);
+    if (e == hipSuccess && nz > 0) e = kern::order_entries(s, g->IC, col, g->CP, CP2, perm, nz, IC2);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { rc = hip_fail(e, "view build", __FILE__, __LINE__); goto out; }
+  }
+  rc = graph_create_csc(CP2, IC2, n, nz, (g->flags & TBFS_GRAPH_SYMMETRIC) | TBFS_GRAPH_DEVICE, &v);
+  if (rc) goto out;
+  g->bview = v;
+  g->bperm = perm;
+  perm = nullptr;
+out:
+  if (tmp) (void)hipFree(tmp);
+  dev_free(keys);
+  dev_free(keys_out);
+  dev_free(ids);
+  dev_free(order);
+  dev_free(deg2);
+  dev_free(CP2);
+  dev_free(col);
+  dev_free(IC2);
+  dev_free(perm);
+  return rc;
+}
+
+static std::mutex g_view_mu;
+
+int graph_batch_view(tbfs_graph* g, int32_t order, hipStream_t s, tbfs_graph** view,
+                     const int32_t** perm) {
+  *view = g;
+  *perm = nullptr;
+  if (order < 0 || !g->CP || g->n < 2) return TBFS_OK;
+  std::lock_guard<std::mutex> lk(g_view_mu);
+  if (order == 0) {
+    if (g->bview_default < 0) {
+      tbfs_degree_stats ds;
+      const int rc = graph_degree_stats(g, TBFS_DEGREE_IN, s, &ds);
+      if (rc) return rc;
+      g->bview_default = (ds.mean > 0 && ds.sd > 2.0 * ds.mean) ? 1 : 0;
+    }
+    if (!g->bview_default) return TBFS_OK;
+  }
+  if (!g->bview) {
+    const int rc = graph_build_view(g, s);
+    if (rc) return rc;
+  }
+  *view = g->bview;
+  *perm = g->bperm;
   return TBFS_OK;
 }
 

@@ -187,6 +187,12 @@ hipError_t expand_cols(hipStream_t s, const int32_t* CP, int32_t n,
 hipError_t histogram(hipStream_t s, const int32_t* keys, int32_t nz,
                      int32_t* cnt);
 hipError_t degree_stats(hipStream_t s, const int32_t* ptr, int32_t n, DegAcc* acc);
+// Degree-ordered numbering of a CSC graph (see graph_batch_view).
+hipError_t order_keys(hipStream_t s, const int32_t* CP, int32_t n, uint32_t* keys, int32_t* ids);
+hipError_t order_perm(hipStream_t s, const int32_t* order, const int32_t* CP, int32_t n,
+                      int32_t* perm, int32_t* deg2);
+hipError_t order_entries(hipStream_t s, const int32_t* IC, const int32_t* col, const int32_t* CP,
+                         const int32_t* CP2, const int32_t* perm, int32_t nz, int32_t* IC2);
 hipError_t check_csc(hipStream_t s, const int32_t* CP, const int32_t* IC,
                      int32_t n, int32_t nz, int32_t* bad);
 

@@ -358,6 +358,41 @@ __global__ __launch_bounds__(256) void k_degree_stats(const int32_t* ptr, int32_
   if (threadIdx.x < 33 && bins[threadIdx.x]) atomicAdd(&acc->bins[threadIdx.x], bins[threadIdx.x]);
 }
 
+// Degree-ordered numbering (batches): sort keys (larger in-degree first,
+// ties by id through the stable sort), then the inverse permutation and the
+// new column lengths, then every entry to its new place.
+__global__ void k_order_keys(const int32_t* CP, int32_t n, uint32_t* keys, int32_t* ids) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    keys[v] = 0x7fffffffu - (uint32_t)(CP[v + 1] - CP[v]);
+    ids[v] = (int32_t)v;
+  }
+}
+
+__global__ void k_order_perm(const int32_t* order, const int32_t* CP, int32_t n,
+                             int32_t* perm, int32_t* deg2) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c <= n; c += stride) {
+    if (c == n) {
+      deg2[n] = 0;
+      continue;
+    }
+    const int32_t o = order[c];
+    perm[o] = (int32_t)c;
+    deg2[c] = CP[o + 1] - CP[o];
+  }
+}
+
+__global__ void k_order_entries(const int32_t* IC, const int32_t* col, const int32_t* CP,
+                                const int32_t* CP2, const int32_t* perm, int32_t nz,
+                                int32_t* IC2) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nz; k += stride) {
+    const int32_t o = col[k];
+    IC2[CP2[perm[o]] + (k - CP[o])] = perm[IC[k]];
+  }
+}
+
 // bad[0] |= 1: CP not monotone / CP[0] != 0 / CP[n] != nz; 2: IC out of range
 __global__ void k_check_csc(const int32_t* CP, const int32_t* IC, int32_t n,
                             int32_t nz, int32_t* bad) {
@@ -471,6 +506,32 @@ hipError_t degree_stats(hipStream_t s, const int32_t* ptr, int32_t n, DegAcc* ac
   if (grid > 4096) grid = 4096;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(k_degree_stats, dim3(grid), dim3(256), 0, s, ptr, n, acc);
+  return hipGetLastError();
+}
+
+static int fill_grid(int64_t m) {
+  int64_t g = (m + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+hipError_t order_keys(hipStream_t s, const int32_t* CP, int32_t n, uint32_t* keys, int32_t* ids) {
+  hipLaunchKernelGGL(k_order_keys, dim3(fill_grid(n)), dim3(256), 0, s, CP, n, keys, ids);
+  return hipGetLastError();
+}
+
+hipError_t order_perm(hipStream_t s, const int32_t* order, const int32_t* CP, int32_t n,
+                      int32_t* perm, int32_t* deg2) {
+  hipLaunchKernelGGL(k_order_perm, dim3(fill_grid((int64_t)n + 1)), dim3(256), 0, s, order, CP,
+                     n, perm, deg2);
+  return hipGetLastError();
+}
+
+hipError_t order_entries(hipStream_t s, const int32_t* IC, const int32_t* col, const int32_t* CP,
+                         const int32_t* CP2, const int32_t* perm, int32_t nz, int32_t* IC2) {
+  hipLaunchKernelGGL(k_order_entries, dim3(fill_grid(nz)), dim3(256), 0, s, IC, col, CP, CP2, perm,
+                     nz, IC2);
   return hipGetLastError();
 }
 
