@@ -107,6 +107,10 @@ typedef struct tbfs_stats {
   int64_t edges;       /* m_s: in-list entries of the reached vertices */
   int64_t ovf64;       /* -1 when some reached count is >= 2^64, else 0 */
   double ms;           /* device time of the BFS, first to last kernel */
+  int32_t launches;    /* chained and graph loops: level kernels the BFS used
+                          (the one that found the frontier empty included);
+                          0 for the other loops and for batches */
+  int32_t pad2;
 } tbfs_stats;
 
 /* Fill *o with the defaults: AUTO, load-balanced kernels, chained loop,

@@ -283,6 +283,33 @@ def test_invalid_inputs_rejected(gpu):
     assert tb.bfs_gpu_bu_csc_sc(badic, g.CP, S, 0, g.nz, g.n, 1, 1) == 1
 
 
+def test_launch_budget_follows_the_graph(gpu):
+    """Chained loop: a BFS uses one level launch per depth plus the one that
+    finds the frontier empty; the next call's budget is the most launches
+    the previous call needed plus one.  A deep graph (a 300 x 4 grid, 303
+    levels) after a shallow one (R-MAT 12) runs past the shrunken budget and
+    is finished by the host-checked re-run; a shallow graph after the deep
+    one runs with the deep budget's spare launches.  Every result stays
+    exact."""
+    deep = graphs.grid(300, 4, 0.0, seed=0, relabel=False)
+    shallow = graphs.rmat(12, 16, seed=7)
+    gd = tb.Graph(deep.CP, deep.IC, symmetric=True)
+    gs = tb.Graph(shallow.CP, shallow.IC, symmetric=True)
+    ctx = tb.Context(max(deep.n, shallow.n))
+    for g, gr, roots in ((shallow, gs, [0, 1, 2]), (deep, gd, [0, 5]), (shallow, gs, [3, 4]),
+                         (deep, gd, [7, 0])):
+        st = ctx.run_many(gr, roots)
+        for r, x in zip(roots, st):
+            S, sig, ovf, lv = oracle.textbook(g.CP, g.IC, int(r))
+            assert x["levels"] == lv and x["launches"] == lv + 1, (r, x)
+        S, sig, ovf, lv = oracle.textbook(g.CP, g.IC, int(roots[-1]))
+        got = ctx.read(g.n)  # the last source's results stay in the context
+        np.testing.assert_array_equal(got[0], S)
+        np.testing.assert_array_equal(got[1], sig)
+        np.testing.assert_array_equal(got[2], ovf)
+    ctx.close()
+
+
 def test_repetition_same_result(gpu):
     g = _er4096()
     S1 = np.zeros(g.n, np.int32)

@@ -74,6 +74,8 @@ class Stats(C.Structure):
         ("edges", C.c_int64),
         ("ovf64", C.c_int64),
         ("ms", C.c_double),
+        ("launches", C.c_int32),
+        ("pad2", C.c_int32),
     ]
 
 

@@ -180,7 +180,7 @@ class Context:
         o = opts if opts is not None else default_opts()
         check(_lib.load().tbfs_bfs(self._h, g.handle, int(root), C.byref(o), C.byref(st)),
               "tbfs_bfs")
-        return {f: getattr(st, f) for f, _ in Stats._fields_ if f != "pad"}
+        return {f: getattr(st, f) for f, _ in Stats._fields_ if f not in ("pad", "pad2")}
 
     def run_many(self, g: Graph, roots, opts: Opts | None = None) -> list[dict]:
         """BFS from every root in turn, back to back on the device; returns
@@ -191,7 +191,7 @@ class Context:
         o = opts if opts is not None else default_opts()
         check(_lib.load().tbfs_bfs_many(self._h, g.handle, _ptr(roots, _i32p), k, C.byref(o),
                                         st), "tbfs_bfs_many")
-        return [{f: getattr(x, f) for f, _ in Stats._fields_ if f != "pad"} for x in st]
+        return [{f: getattr(x, f) for f, _ in Stats._fields_ if f not in ("pad", "pad2")} for x in st]
 
     def run_multi(self, g: Graph, roots, opts: Opts | None = None) -> dict:
         """One BFS from all roots at once (depth 0 = every root); results stay
@@ -202,7 +202,7 @@ class Context:
         check(_lib.load().tbfs_bfs_multi(self._h, g.handle, _ptr(roots, _i32p),
                                          int(roots.shape[0]), C.byref(o), C.byref(st)),
               "tbfs_bfs_multi")
-        return {f: getattr(st, f) for f, _ in Stats._fields_ if f != "pad"}
+        return {f: getattr(st, f) for f, _ in Stats._fields_ if f not in ("pad", "pad2")}
 
     def mark(self, slot: int) -> None:
         """Record timing marker `slot` on this context's stream."""
@@ -272,7 +272,7 @@ class Batch:
         self._n = g.n
         self._g = g
         self._count = int(roots.shape[0])
-        out = {f: getattr(st, f) for f, _ in Stats._fields_ if f != "pad"}
+        out = {f: getattr(st, f) for f, _ in Stats._fields_ if f not in ("pad", "pad2")}
         out["count_bits"] = int(st.pad)
         return out
 

@@ -1283,6 +1283,8 @@ void batch_fill_stats(const tbfs_batch* b, tbfs_stats* st) {
   st->edges = (int64_t)rs.edges;
   st->ovf64 = rs.ovf_any ? -1 : 0;
   st->ms = (double)b->ms;
+  st->launches = 0;
+  st->pad2 = 0;
 }
 
 int batch_run(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t count,

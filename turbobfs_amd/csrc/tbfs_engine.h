@@ -146,7 +146,7 @@ int ctx_async_enqueue(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs
                       int idx, int levels, int32_t* S, uint64_t* sigma, uint8_t* ovf);
 int ctx_async_finish(tbfs_ctx* c, int count);
 int ctx_async_poll(tbfs_ctx* c, int count, bool* ready, std::vector<int>* unfinished,
-                   int* maxlev);
+                   int* maxl);
 
 // batched sources (tbfs_batch.hip)
 int batch_words_for(int32_t count);

@@ -753,6 +753,8 @@ static void fill_stats(tbfs_stats* st, const RunStats& rs, float ms) {
   st->push_levels = rs.push_levels;
   st->pull_levels = rs.pull_levels;
   st->pad = 0;
+  st->launches = rs.launches;
+  st->pad2 = 0;
   st->reached = (int64_t)rs.reached;
   st->edges = (int64_t)rs.edges;
   st->ovf64 = rs.ovf_any ? -1 : 0;  // count known after tbfs_ctx_read
@@ -1025,7 +1027,7 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
     HIPCHK(hipEventElapsedTime(&ms[i], c->evs[i], c->evs[i + 1]));
   if (loop == TBFS_LOOP_CHAINED || loop == TBFS_LOOP_GRAPH) {
     bool redo_last = false;
-    int maxlev = 0;
+    int maxl = 0;
     for (int i = 0; i < count; ++i) {
       if (c->stats_h[i].levels < 0) {
         HIPCHK(hipEventRecord(c->ev0, s));
@@ -1037,15 +1039,16 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
         HIPCHK(hipMemcpy(&c->stats_h[i], c->stats + i, sizeof(RunStats), hipMemcpyDeviceToHost));
         redo_last = i != count - 1;
       }
-      maxlev = std::max(maxlev, c->stats_h[i].levels);
+      maxl = std::max(maxl, c->stats_h[i].launches);
     }
     if (redo_last) {  // leave the last source's results in the context
       rc = run_chained(c, g, roots[count - 1], o, c->stats + count - 1);
       if (rc) return rc;
     }
-    // one spare level: a source one level deeper than any seen so far still
+    // the next call's budget: the launches these sources needed plus one
+    // spare, so a source needing one launch more than any seen here still
     // finishes without a re-run (a spare launch costs ~4 us, a re-run a BFS)
-    c->chain_k = std::max(c->chain_k, std::min(4096, maxlev + 2));
+    c->chain_k = std::min(4096, std::max(2, maxl + 1));
   }
   c->last_n = g->n;
   if (st) {
@@ -1107,7 +1110,7 @@ int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots
 // source); ctx_async_finish queues the stats copy and an event,
 // ctx_async_poll reports when that event has passed, which sources ran out
 // of levels (the caller enqueues them again with a larger budget) and the
-// deepest level count seen.
+// most level launches a finished source used.
 int ctx_async_begin(tbfs_ctx* c, int count) { return ctx_reserve_stats(c, count); }
 
 int ctx_async_enqueue(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs_opts& o,
@@ -1135,17 +1138,17 @@ int ctx_async_finish(tbfs_ctx* c, int count) {
 }
 
 int ctx_async_poll(tbfs_ctx* c, int count, bool* ready, std::vector<int>* unfinished,
-                   int* maxlev) {
+                   int* maxl) {
   *ready = false;
   const hipError_t e = hipEventQuery(c->ev1);
   if (e == hipErrorNotReady) return TBFS_OK;
   if (e != hipSuccess) return hip_fail(e, "hipEventQuery", __FILE__, __LINE__);
   *ready = true;
   unfinished->clear();
-  *maxlev = 0;
+  *maxl = 0;
   for (int i = 0; i < count; ++i) {
     if (c->stats_h[i].levels < 0) unfinished->push_back(i);
-    *maxlev = std::max(*maxlev, c->stats_h[i].levels);
+    else *maxl = std::max(*maxl, c->stats_h[i].launches);
   }
   return TBFS_OK;
 }

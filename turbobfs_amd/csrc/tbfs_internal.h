@@ -76,7 +76,9 @@ struct alignas(64) RunStats {
   uint32_t ovf_any;
   unsigned long long reached;
   unsigned long long edges;
-  unsigned long long pad[4];
+  int32_t launches;  // chained loop: level kernels used, the last one included
+  int32_t pad0;
+  unsigned long long pad[3];
 };
 
 struct GraphView {

@@ -690,8 +690,9 @@ __device__ __forceinline__ bool step_state(const PersistParams& p,
 }
 
 __device__ __forceinline__ void write_stats(const PersistParams& p,
-                                            const LoopState& s) {
+                                            const LoopState& s, int32_t step) {
   p.stats->levels = s.d;
+  p.stats->launches = step + 1;
   p.stats->push_levels = s.push_levels;
   p.stats->pull_levels = s.pull_levels;
   p.stats->ovf_any = s.ovf_any;
@@ -772,7 +773,7 @@ __global__ __launch_bounds__(BLOCK) void k_persist(PersistParams p) {
     ++s.d;
     grid_barrier(p.bar, ++bk);
   }
-  if (gtid == 0) write_stats(p, s);
+  if (gtid == 0) write_stats(p, s, -1);
 }
 
 // Chained loop: state of depth 0 and the first loop-state slot.
@@ -825,7 +826,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t 
   if (!step_state(p, &c, s)) {
     if (gtid == 0) {
       p.loop[(step + 1) & 1] = s;
-      write_stats(p, s);
+      write_stats(p, s, step);
     }
     return;
   }

@@ -178,16 +178,17 @@ bool step_slot(tbfs_server* s, Slot& sl) {
   }
   if (sl.state == 3) {
     bool ready = false;
-    int maxlev = 0;
-    int rc = ctx_async_poll(sl.c, (int)sl.q.size(), &ready, &sl.redo, &maxlev);
+    int maxl = 0;
+    int rc = ctx_async_poll(sl.c, (int)sl.q.size(), &ready, &sl.redo, &maxl);
     if (rc) {
       complete(s, sl, rc, s->now_us());
       return true;
     }
     if (!ready) return false;
     int& bud = s->budget[sl.graph];
-    // the deepest BFS seen plus one spare; twice the budget if one ran out
-    bud = std::max(bud, std::min(1 << 16, sl.redo.empty() ? maxlev + 2 : 2 * bud));
+    // the most launches a BFS needed plus one spare; twice the budget if
+    // one ran out
+    bud = std::max(bud, std::min(1 << 16, sl.redo.empty() ? maxl + 1 : 2 * bud));
     if (sl.redo.empty()) {
       complete(s, sl, TBFS_OK, s->now_us());
       return true;
