@@ -1,14 +1,17 @@
 // tbfs_persist.hip -- load-balanced BFS levels and the device-resident level
-// loop (one cooperative launch per BFS, grid barrier between levels).
+// loops: one cooperative launch per BFS with a grid barrier between levels
+// (persistent), or one launch per level enqueued back to back (chained).
 //
 // Direction-optimizing BFS with exact path counts (the reference's TD/BU
 // switch, TurboBFS_CSC_TDBU/bfsgputdbucsc_sc.cu:108-153, redone for wave64):
 //   push level  expands the frontier, read from its bitmap.  A wavefront takes
-//               the set bits of one frontier word, scans their out-degrees and
-//               walks the concatenated out-lists 64 edges at a time; out-lists
-//               longer than kBigIn are cut into static slices (graph property)
-//               spread over all wavefronts.  Discovery is a CAS on S; counts are
-//               64-bit atomics whose old value exposes the carry.
+//               the set bits of 64 frontier words, numbers them across the
+//               words, scans their out-degrees and walks the concatenated
+//               out-lists 64 edges at a time; out-lists longer than kBigIn are
+//               cut into static slices (graph property) spread over all
+//               wavefronts.  Counts are 64-bit atomics whose old value exposes
+//               the carry; the adder that finds a count at 0 claims the
+//               vertex's bit in the next frontier and records its depth.
 //   pull level  sweeps the unvisited columns.  A wavefront takes the 64
 //               columns of one frontier word, scans their in-degrees and walks
 //               the concatenated in-lists; parents' counts are summed per
