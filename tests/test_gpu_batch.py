@@ -369,6 +369,38 @@ def test_multi_root_equals_super_source(gpu, case):
     assert np.array_equal(ovf, eovf[:g.n])
 
 
+def test_multi_root_after_graph_loop_then_close(gpu):
+    """Regression: a multi-root run with more roots than the context's root
+    buffer holds replaces that buffer, which a captured level loop reads its
+    root from.  The capture must be dropped (and captured again by the next
+    graph-loop BFS), nothing else freed twice, and closing the context must
+    leave no error behind for the next context."""
+    g = graphs.rmat(10, 16, 0.57, 0.19, 0.19, seed=3, relabel=True)
+    gr = tb.Graph(g.CP, g.IC, symmetric=True)
+    graph_loop = tb.default_opts(loop=tb._lib.LOOP_GRAPH)
+
+    def same(res, r):
+        S, sig, ovf, _ = oracle.textbook(g.CP, g.IC, r)
+        assert np.array_equal(res.S, S) and np.array_equal(res.sigma, sig)
+        assert np.array_equal(res.ovf, ovf)
+
+    ctx = tb.Context(g.n)
+    same(ctx.bfs(gr, 5, graph_loop), 5)
+    roots = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 40, seed=2)
+    ctx.run_multi(gr, roots)
+    S, sig, ovf = ctx.read(g.n)
+    h = _with_super_source(g, roots)
+    eS, esig, _, _ = oracle.textbook(h.CP, h.IC, g.n)
+    assert np.array_equal(S, np.where(eS[:g.n] >= 0, eS[:g.n] - 1, -1))
+    assert np.array_equal(sig, esig[:g.n])
+    same(ctx.bfs(gr, 7, graph_loop), 7)
+    ctx.close()
+    ctx2 = tb.Context(g.n)
+    same(ctx2.bfs(gr, 9), 9)
+    same(ctx2.bfs(gr, 11, graph_loop), 11)
+    ctx2.close()
+
+
 def test_multi_root_invalid(gpu):
     g = graphs.erdos_renyi(256, 0.05, seed=3)
     gr = tb.Graph(g.CP, g.IC, symmetric=True)

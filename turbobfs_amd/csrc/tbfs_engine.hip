@@ -1076,10 +1076,10 @@ int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots
   if (rc) return rc;
   if (c->roots_cap < nroots) {
     HIPCHK(hipStreamSynchronize(c->stream));
+    // a captured level loop reads its root from roots_d: capture it again
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
     dev_free(c->roots_d);
-  dev_free(c->gstats);
-  if (c->groots_h) (void)hipHostFree(c->groots_h);
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     c->roots_cap = 0;
     rc = dev_alloc(&c->roots_d, (size_t)nroots);
     if (rc) return rc;
