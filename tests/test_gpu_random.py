@@ -7,7 +7,9 @@ vertices.  Each goes through the chained loop (default options), the
 host-driven loop, the persistent loop, the graph-captured loop, a batch of
 random roots (repeats included) and a multi-root BFS whose roots sit in
 different components; every S, sigma (mod 2^64) and overflow flag must equal
-the oracle's, and components without a root stay unreached.
+the oracle's, and components without a root stay unreached.  Every third
+graph also goes through the seven reference entry points, and twelve of them
+are served together to a burst of random queries.
 """
 import numpy as np
 import pytest
@@ -99,3 +101,43 @@ def test_random_graph_every_path(gpu, case):
         rest = ~np.isin(comp, ids)
         assert (mS[rest] == -1).all() and (msig[rest] == 0).all() and (movf[rest] == 0).all()
     ctx.close()
+
+
+@pytest.mark.parametrize("case", range(0, 60, 3))
+def test_random_graph_dropin(gpu, case):
+    """The seven reference entry points on every third graph, from a random
+    root: the reference's output types, exact where representable and the
+    inexact status where not."""
+    from tests.test_gpu_parity import _check_dropin
+    name, g, symmetric = _draw(case)
+    r = int(np.random.default_rng(case + 1000).integers(0, g.n))
+    print(f"[random drop-in] {name} n={g.n} nz={g.nz} root={r}", flush=True)
+    _check_dropin(g, r)
+
+
+def test_random_graphs_served(gpu):
+    """Twelve of the graphs resident in one server, 150 queries on random
+    (graph, root) pairs submitted at once: every result equals the oracle's."""
+    drawn = [_draw(i) for i in range(12)]
+    cs = [g for _, g, _ in drawn]
+    srv = tb.Server([tb.Graph(g.CP, g.IC, symmetric=sym) for _, g, sym in drawn])
+    rng = np.random.default_rng(77)
+    store = {}
+    for _ in range(150):
+        gi = int(rng.integers(0, len(cs)))
+        r = int(rng.integers(0, cs[gi].n))
+        n = cs[gi].n
+        bufs = (tb.host_array(n, np.int32), tb.host_array(n, np.uint64), tb.host_array(n, np.uint8))
+        store[srv.submit(gi, r, *bufs)] = (gi, r) + bufs
+    done = []
+    while len(done) < len(store):
+        done += srv.poll(4096, wait_us=100000)
+    for d in done:
+        assert d["status"] == 0, d
+        gi, r, S, sig, ovf = store[d["ticket"]]
+        eS, esig, eovf, _ = oracle.textbook(cs[gi].CP, cs[gi].IC, r)
+        np.testing.assert_array_equal(S, eS, err_msg=f"graph {gi} root {r}")
+        np.testing.assert_array_equal(sig, esig, err_msg=f"graph {gi} root {r}")
+        np.testing.assert_array_equal(ovf, eovf, err_msg=f"graph {gi} root {r}")
+    srv.close()
+
