@@ -102,34 +102,44 @@ __device__ __forceinline__ void flush_level(CtrSet* set, LevelAcc& a) {
   a = LevelAcc();
 }
 
-// Totals of a striped counter set, on every lane of the calling wavefront
-// (lanes 0..kStripes-1 load one stripe each).  `agent`: the loads bypass
-// caches that may hold values from before the last grid barrier.
-__device__ __forceinline__ LevelCtr read_set(const CtrSet* set, bool agent) {
+// This lane's stripe of a striped counter set (lanes 0..kStripes-1 load one
+// stripe each).  `agent`: the loads bypass caches that may hold values from
+// before the last grid barrier.
+struct StripeVals {
   uint64_t qm = 0, mi = 0;
   uint32_t ov = 0;
+};
+__device__ __forceinline__ StripeVals load_set(const CtrSet* set, bool agent) {
+  StripeVals r;
   const int l = lane();
   if (l < kStripes) {
     const CtrStripe* st = &set->s[l];
     if (agent) {
-      qm = __hip_atomic_load(&st->qm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      mi = __hip_atomic_load(&st->min, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      ov = __hip_atomic_load(&st->ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      r.qm = __hip_atomic_load(&st->qm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      r.mi = __hip_atomic_load(&st->min, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      r.ov = __hip_atomic_load(&st->ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      qm = st->qm;
-      mi = st->min;
-      ov = st->ovf;
+      r.qm = st->qm;
+      r.mi = st->min;
+      r.ov = st->ovf;
     }
   }
-  const uint64_t q = wave_sum_u64(qm >> 32);
-  const uint64_t mo = wave_sum_u64(qm & 0xffffffffull);
+  return r;
+}
+
+// Totals of the stripes, on every lane of the calling wavefront.
+__device__ __forceinline__ LevelCtr sum_set(const StripeVals& r) {
   LevelCtr c;
-  c.q = q;
+  c.q = wave_sum_u64(r.qm >> 32);
   c.qe = 0;
-  c.mout = mo;
-  c.min = wave_sum_u64(mi);
-  c.ovf = __ballot(ov != 0) ? 1u : 0u;
+  c.mout = wave_sum_u64(r.qm & 0xffffffffull);
+  c.min = wave_sum_u64(r.mi);
+  c.ovf = __ballot(r.ov != 0) ? 1u : 0u;
   return c;
+}
+
+__device__ __forceinline__ LevelCtr read_set(const CtrSet* set, bool agent) {
+  return sum_set(load_set(set, agent));
 }
 
 // One edge u -> v of a push level, u at depth d with count x (overflow ou).
@@ -819,13 +829,16 @@ __global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t 
   extern __shared__ uint64_t fb_dyn[];
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  // launch `step` runs depth `step` (one level per launch): its counters are
+  // loaded together with the loop state instead of after it
+  const StripeVals early = load_set(&p.cset[step % 3], false);
   LoopState s = p.loop[step & 1];
   if (s.done) {
     if (gtid == 0) p.loop[(step + 1) & 1] = s;
     return;
   }
   const int32_t d = s.d;
-  const LevelCtr c = read_set(&p.cset[d % 3], false);
+  const LevelCtr c = sum_set(d == step ? early : load_set(&p.cset[d % 3], false));
   if (!step_state(p, &c, s)) {
     if (gtid == 0) {
       p.loop[(step + 1) & 1] = s;
