@@ -705,7 +705,11 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int3
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
 
-  // the level's counters are loaded together with the loop state
+  BLoop st = p.loop[d & 1];
+  if (st.done) {
+    if (gtid == 0) p.loop[(d + 1) & 1] = st;
+    return;
+  }
   unsigned long long q = 0, bits = 0, edges = 0, mout = 0;
   unsigned int ovf = 0;
   {
@@ -718,11 +722,6 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int3
       mout += cs.s[k].mout;
       ovf |= cs.s[k].ovf;
     }
-  }
-  BLoop st = p.loop[d & 1];
-  if (st.done) {
-    if (gtid == 0) p.loop[(d + 1) & 1] = st;
-    return;
   }
   st.reached += bits;
   st.edges += edges;
