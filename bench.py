@@ -97,8 +97,19 @@ def cpu_baseline(g, sources, budget_s: float):
         "value": gteps, "unit": "GTEPS", "cores": 1, "kind": kind,
         "sample": f"{done} of {len(sources)} sources, bfs_seq_td_csc "
                   f"(O(levels*(n+nz)) sweep), single thread, "
-                  f"{t_total:.1f} s on {os.cpu_count()} host cores",
+                  f"{t_total:.1f} s; 1 thread of {os.cpu_count()} host cores ({_cpu_model()})",
     }
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "cpu model unknown"
 
 
 def bench_variants(tb, g, gr, ctx, sources, nsrc: int = 16, names=None):
