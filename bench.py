@@ -232,10 +232,13 @@ def bench_multigraph(tb, graphs, reps: int = 3):
     }
 
 
-def bench_grid(tb, graphs, nsrc: int = 16, names=("chained", "graph", "persistent_1024x64", "host")):
+def bench_grid(tb, graphs, nsrc: int = 16, names=("chained", "graph", "persistent_1024x64", "host"),
+               cpu_passes: int = 0):
     """High-diameter single-source BFS: 2048x2048 4-neighbour grid with 10% of
     its edges removed (seed 3), vertex ids permuted, sources in the largest
-    component (seed 3).  Per-level cost of each level loop."""
+    component (seed 3).  Per-level cost of each level loop; with cpu_passes,
+    the restated sequential BFS timed over its first cpu_passes passes and
+    its whole run extrapolated (one thread)."""
     from turbobfs_amd import _lib
     g = graphs.grid(2048, 2048, 0.1, seed=3, relabel=True)
     lc = graphs.largest_component_vertices(g)
@@ -268,6 +271,18 @@ def bench_grid(tb, graphs, nsrc: int = 16, names=("chained", "graph", "persisten
     st = b.run(gr, src)
     out["batch_of_sources"] = {"sources": nsrc, "ms": st["ms"], "levels": st["levels"],
                                "sources_per_s": nsrc / (st["ms"] * 1e-3)}
+    if cpu_passes > 0:  # the restated sequential BFS: a timed sample of its passes
+        import oracle
+        levels0 = ctx.run_many(gr, src[:1])[0]["levels"]
+        t0 = time.perf_counter()
+        made = oracle.seq_td_csc_passes(g.CP, g.IC, int(src[0]), cpu_passes)[3]
+        per_pass = (time.perf_counter() - t0) / made
+        out["cpu_baseline"] = {
+            "kind": "port", "cores": 1, "us_per_level": per_pass * 1e6,
+            "ms_per_bfs_extrapolated": per_pass * (levels0 + 1) * 1e3,
+            "sample": f"first {made} of the {levels0 + 1} passes of the restated bfs_seq_td_csc "
+                      f"(each pass sweeps every column) from the first source, single thread, "
+                      f"{per_pass * made:.1f} s; the whole BFS = passes x time per pass"}
     ctx.close()
     b.close()
     return out
@@ -440,7 +455,8 @@ def main():
         fn = {"batch": lambda: bench_batched(tb, graphs, tune=not args.no_tune,
                                              loop=not args.no_loop),
               "multigraph": lambda: bench_multigraph(tb, graphs),
-              "grid": lambda: bench_grid(tb, graphs, names=tuple(args.grid_loops.split(","))),
+              "grid": lambda: bench_grid(tb, graphs, names=tuple(args.grid_loops.split(",")),
+                                         cpu_passes=0 if args.no_cpu else 24),
               "serve": lambda: bench_serving(tb, graphs)}[args.only]
         print(json.dumps({args.only: fn()}))
         return
@@ -543,7 +559,7 @@ def main():
     if not args.no_batch:
         line["batched"] = bench_batched(tb, graphs)
         line["multi_graph"] = bench_multigraph(tb, graphs)
-        line["high_diameter"] = bench_grid(tb, graphs)
+        line["high_diameter"] = bench_grid(tb, graphs, cpu_passes=0 if args.no_cpu else 24)
     if not args.no_serve:
         line["serving"] = bench_serving(tb, graphs)
     if not args.no_cpu:

@@ -40,6 +40,9 @@ def load():
         if not os.path.exists(LIB):
             build()
         lib = C.CDLL(LIB)
+        lib.orc_bfs_seq_td_csc_passes.restype = C.c_int
+        lib.orc_bfs_seq_td_csc_passes.argtypes = [_i32p, _i32p, C.c_int32, C.c_int32, _i32p,
+                                                  _u64p, _u8p, C.c_int32]
         lib.orc_bfs_seq_td_csc.restype = C.c_int
         lib.orc_bfs_seq_td_csc.argtypes = [_i32p, _i32p, C.c_int32, C.c_int32, C.c_int32,
                                            _i32p, _u64p, _u8p]
@@ -73,6 +76,22 @@ def seq_td_csc(CP, IC, r, s_unreached=-1):
                                   _p(S, _i32p), _p(sig, _u64p), _p(ovf, _u8p))
     if d < 0:
         raise ValueError("orc_bfs_seq_td_csc: bad arguments")
+    return S, sig, ovf, d
+
+
+def seq_td_csc_passes(CP, IC, r, max_passes):
+    """The restated bfs_seq_td_csc loop stopped after max_passes passes (a
+    timing sample of a BFS too long for one CPU thread).  Returns
+    (S, sigma, ovf, passes) of the partial run; S uses -1 for unreached."""
+    CP, IC = _csc(CP, IC)
+    n = CP.shape[0] - 1
+    S = np.empty(n, np.int32)
+    sig = np.empty(n, np.uint64)
+    ovf = np.empty(n, np.uint8)
+    d = load().orc_bfs_seq_td_csc_passes(_p(IC, _i32p), _p(CP, _i32p), n, int(r), _p(S, _i32p),
+                                         _p(sig, _u64p), _p(ovf, _u8p), int(max_passes))
+    if d < 0:
+        raise ValueError("orc_bfs_seq_td_csc_passes: bad arguments")
     return S, sig, ovf, d
 
 

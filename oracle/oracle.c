@@ -59,9 +59,28 @@ static inline void add64(uint64_t *a, uint8_t *ao, uint64_t b, uint8_t bo) {
  * Returns the number of passes of the level loop (the reference's final d),
  * or -1 on bad arguments / allocation failure.
  */
+static int seq_td_csc(const int32_t *IC, const int32_t *CP, int32_t n,
+                      int32_t r, int32_t s_unreached, int32_t *S,
+                      uint64_t *sigma, uint8_t *ovf, int32_t max_passes);
+
 int orc_bfs_seq_td_csc(const int32_t *IC, const int32_t *CP, int32_t n,
                        int32_t r, int32_t s_unreached, int32_t *S,
                        uint64_t *sigma, uint8_t *ovf) {
+  return seq_td_csc(IC, CP, n, r, s_unreached, S, sigma, ovf, 0);
+}
+
+/* The same loop stopped after max_passes passes (a timing sample of a BFS
+ * too long to run whole on one CPU thread); returns the passes made. */
+int orc_bfs_seq_td_csc_passes(const int32_t *IC, const int32_t *CP, int32_t n,
+                              int32_t r, int32_t *S, uint64_t *sigma, uint8_t *ovf,
+                              int32_t max_passes) {
+  if (max_passes <= 0) return -1;
+  return seq_td_csc(IC, CP, n, r, -1, S, sigma, ovf, max_passes);
+}
+
+static int seq_td_csc(const int32_t *IC, const int32_t *CP, int32_t n,
+                      int32_t r, int32_t s_unreached, int32_t *S,
+                      uint64_t *sigma, uint8_t *ovf, int32_t max_passes) {
   if (n <= 0 || r < 0 || r >= n) return -1;
   uint64_t *f = (uint64_t *)calloc((size_t)n, sizeof(uint64_t));
   uint64_t *ft = (uint64_t *)calloc((size_t)n, sizeof(uint64_t));
@@ -79,7 +98,7 @@ int orc_bfs_seq_td_csc(const int32_t *IC, const int32_t *CP, int32_t n,
   S[r] = 0;          /* 0 under both conventions */
   f[r] = 1;          /* bfstdcsc_seq.c:35 */
   int d = 0, c = 1;
-  while (c > 0) {    /* bfstdcsc_seq.c:48 */
+  while (c > 0 && (max_passes <= 0 || d < max_passes)) {    /* bfstdcsc_seq.c:48 */
     d++;
     /* sum_vs: sigma += f where f > 0 (bfstdcsc_seq_utils.c:21-31) */
     for (int32_t k = 0; k < n; k++)

@@ -176,6 +176,21 @@ def test_brute_force_random_digraphs(seed):
         assert not ovf.any()
 
 
+def test_pass_capped_run_is_a_prefix_of_the_full_run():
+    """orc_bfs_seq_td_csc_passes (the timing sample of long BFS runs) is the
+    same loop stopped after k passes: depths up to k, and the counts of the
+    depths below k, equal the full run's; nothing else is reached."""
+    g = graphs.grid(40, 40, 0.1, seed=3, relabel=True)
+    S, sig, ovf, passes = oracle.seq_td_csc(g.CP, g.IC, 5)
+    for k in (1, 5, 17, passes + 3):
+        Sp, sp, op, d = oracle.seq_td_csc_passes(g.CP, g.IC, 5, k)
+        assert d == min(k, passes)
+        np.testing.assert_array_equal(Sp, np.where(S <= k, S, -1))
+        np.testing.assert_array_equal(sp, np.where((S >= 0) & (S < k), sig, 0))
+    with pytest.raises(ValueError):
+        oracle.seq_td_csc_passes(g.CP, g.IC, 5, 0)
+
+
 def test_td_convention_unreached_zero():
     g = csc_from_edges(5, [(0, 1), (1, 2)])
     S0, _, _, passes = oracle.seq_td_csc(g.CP, g.IC, 0, s_unreached=0)
