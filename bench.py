@@ -112,6 +112,35 @@ def _cpu_model() -> str:
     return "cpu model unknown"
 
 
+def bench_dropin(tb, g, sources, nsrc: int = 4):
+    """The reference's entry point bfs_gpu_tdbu_csc_sc as a caller of the
+    reference uses it: host arrays in, host arrays out, the graph uploaded
+    and checked on every call.  Host wall time per call (PCIe included), one
+    warm-up call first; with repetition = 8 one upload serves 8 BFS."""
+    n, nz = g.n, g.nz
+    S = np.zeros(n, np.int32)
+    sig = np.zeros(n, np.int32)
+    tb.bfs_gpu_tdbu_csc_sc(g.IC, g.CP, S, sig, int(sources[0]), nz, n, 1, 1)
+    out = {"entry_point": "bfs_gpu_tdbu_csc_sc (int32 sigma; returns 5 where a count "
+                          "leaves int32)"}
+    deg = np.diff(g.CP).astype(np.int64)
+    for rep in (1, 8):
+        t = []
+        edges = 0
+        for r in sources[:nsrc]:
+            t0 = time.perf_counter()
+            rc = tb.bfs_gpu_tdbu_csc_sc(g.IC, g.CP, S, sig, int(r), nz, n, rep, 1)
+            t.append(time.perf_counter() - t0)
+            if rc not in (0, 5):
+                raise RuntimeError(f"bfs_gpu_tdbu_csc_sc returned {rc}")
+            edges += int(deg[S >= 0].sum())
+        ms = 1e3 * sum(t) / len(t)
+        out[f"repetition_{rep}"] = {
+            "ms_per_call": ms, "ms_per_bfs": ms / rep,
+            "gteps_pcie_inclusive": edges / 2 * rep / (sum(t)) / 1e9}
+    return out
+
+
 def bench_variants(tb, g, gr, ctx, sources, nsrc: int = 16, names=None):
     """Every reference variant's kernel mapping on the same resident graph:
     GTEPS and algorithmic GB/s (B_alg / time) per variant, sources in turn
@@ -556,6 +585,7 @@ def main():
     if not args.no_variants:
         line["variants"] = bench_variants(tb, g, gr, ctx, sources,
                                           names=[x for x in args.variants.split(",") if x])
+    line["dropin_pcie_inclusive"] = bench_dropin(tb, g, sources)
     if not args.no_batch:
         line["batched"] = bench_batched(tb, graphs)
         line["multi_graph"] = bench_multigraph(tb, graphs)

@@ -9,6 +9,7 @@
 // nonzero status codes instead of exit().
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -51,10 +52,13 @@ int run_dropin(const char* name, const int* A, const int* B, int nz, int n,
                  name, n, nz, r, repetition);
     return TBFS_ERR_INVALID;
   }
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
   tbfs_graph* g = nullptr;
   tbfs_ctx* c = nullptr;
   int rc = cooc ? graph_create_cooc(A, B, n, nz, 0u, &g)
                 : graph_create_csc(A, B, n, nz, 0u, &g);
+  const auto t1 = clk::now();
   if (rc) {
     std::fprintf(stderr, "turbobfs: %s: graph upload failed: %s\n", name,
                  tbfs_status_str(rc));
@@ -65,6 +69,7 @@ int run_dropin(const char* name, const int* A, const int* B, int nz, int n,
     graph_free(g);
     return rc;
   }
+  const auto t2 = clk::now();
   for (int i = 0; i < repetition && !rc; ++i) {
     rc = run_bfs(c, g, r, o, &res->st);
     res->total_ms += res->st.ms;
@@ -78,17 +83,25 @@ int run_dropin(const char* name, const int* A, const int* B, int nz, int n,
     rc = ctx_read(c, n, res->S.data(), want_sigma ? res->sigma.data() : nullptr,
                   want_sigma ? res->ovf.data() : nullptr);
   }
+  const auto t3 = clk::now();
   ctx_free(c);
   graph_free(g);
+  const auto t4 = clk::now();
+  auto ms = [](clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
   if (rc) {
     std::fprintf(stderr, "turbobfs: %s: BFS failed: %s\n", name,
                  tbfs_status_str(rc));
     return rc;
   }
-  if (verbose())
+  if (verbose()) {
     std::printf("\n%s::d = %d,r = %d,t_sum=%lfms \n%s::average time BFS = %lfms \n",
                 name, res->st.levels, r, res->total_ms, name,
                 res->total_ms / repetition);
+    std::printf("%s::host ms: graph upload + checks %.3f, context %.3f, BFS + read %.3f, "
+                "free %.3f\n", name, ms(t0, t1), ms(t1, t2), ms(t2, t3), ms(t3, t4));
+  }
   return TBFS_OK;
 }
 
