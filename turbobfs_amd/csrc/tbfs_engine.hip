@@ -703,8 +703,9 @@ static int enqueue_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   return TBFS_OK;
 }
 
-// Chained loop for one BFS: init, then chunks of chain_k level kernels; the
-// host reads the loop state once per chunk.  Ends with the stream drained.
+// Chained loop for one BFS: init, then chunks of level kernels (chain_k, then
+// growing); the host reads the loop state once per chunk.  Ends with the
+// stream drained.
 static int run_chained_p(tbfs_ctx* c, const tbfs_graph* g, const PersistParams& p,
                          const tbfs_opts& o) {
   const int grid = chain_grid(c, g, o);
@@ -712,12 +713,17 @@ static int run_chained_p(tbfs_ctx* c, const tbfs_graph* g, const PersistParams& 
   hipStream_t s = c->stream;
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
   if (p.nroots > 0) HIPCHK(kern::chain_seed(s, p));
+  // chunks of level kernels: chain_k, then growing by half up to 256, so a
+  // BFS far deeper than the budget costs few host checks and at most one
+  // chunk of spare launches
   int32_t d = 0;
+  int chunk = std::max(c->chain_k, 8);
   for (;;) {
-    for (int i = 0; i < c->chain_k; ++i)
+    for (int i = 0; i < chunk; ++i)
       HIPCHK(kern::chain_level(s, o.grid > 0 ? grid : level_grid(c, grid, fbl, d + i), p, d + i,
                                fbl));
-    d += c->chain_k;
+    d += chunk;
+    chunk = std::min(chunk + chunk / 2, 256);
     HIPCHK(hipMemcpyAsync(c->loop_h, c->loop + (d & 1), sizeof(LoopState),
                           hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
