@@ -126,7 +126,7 @@ out:
   return rc;
 }
 
-int graph_build_bigc(tbfs_graph* g);
+int graph_build_bigc(tbfs_graph* g, const int32_t* cp_host);
 
 int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
                      int32_t nz, uint32_t flags, tbfs_graph** out) {
@@ -178,7 +178,7 @@ int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
   }
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__); goto fail; }
-  rc = graph_build_bigc(g);
+  rc = graph_build_bigc(g, on_dev ? nullptr : CP);
   if (rc) goto fail;
   (void)hipStreamDestroy(s);
   *out = g;
@@ -237,7 +237,7 @@ int graph_create_cooc(const int32_t* I, const int32_t* J, int32_t n, int32_t nz,
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__); goto fail; }
   if (sorted) {
-    rc = graph_build_bigc(g);
+    rc = graph_build_bigc(g, nullptr);
     if (rc) goto fail;
   }
   (void)hipStreamDestroy(s);
@@ -262,10 +262,9 @@ int graph_ensure_cooc(tbfs_graph* g, hipStream_t s) {
 
 // Static slices (kPullChunk entries) of every list longer than kBigIn: in-lists
 // for the pull levels, out-lists for the push levels.
-static int build_slices(const int32_t* ptr_dev, int32_t n, uint64_t** out,
-                        int32_t* count) {
-  std::vector<int32_t> ptr((size_t)n + 1);
-  HIPCHK(hipMemcpy(ptr.data(), ptr_dev, ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+// Static slices of the lists longer than kBigIn, from the list offsets ptr
+// (host memory).
+static int build_slices(const int32_t* ptr, int32_t n, uint64_t** out, int32_t* count) {
   std::vector<uint64_t> big;
   for (int32_t v = 0; v < n; ++v) {
     const int64_t deg = (int64_t)ptr[v + 1] - ptr[v];
@@ -283,12 +282,20 @@ static int build_slices(const int32_t* ptr_dev, int32_t n, uint64_t** out,
   return TBFS_OK;
 }
 
-int graph_build_bigc(tbfs_graph* g) {
-  int rc = build_slices(g->CP, g->n, &g->bigc, &g->nbigc);
+// Slices of the long in-lists (and out-lists) and the bitmap of columns the
+// light-column pull skips.  cp_host: the in-list offsets in host memory when
+// the caller has them (else they are read back from the device).
+int graph_build_bigc(tbfs_graph* g, const int32_t* cp_host) {
+  std::vector<int32_t> cpv;
+  if (!cp_host) {
+    cpv.resize((size_t)g->n + 1);
+    HIPCHK(hipMemcpy(cpv.data(), g->CP, cpv.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    cp_host = cpv.data();
+  }
+  int rc = build_slices(cp_host, g->n, &g->bigc, &g->nbigc);
   if (rc) return rc;
   {
-    std::vector<int32_t> cp((size_t)g->n + 1);
-    HIPCHK(hipMemcpy(cp.data(), g->CP, cp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const int32_t* cp = cp_host;
     const size_t nw = ((size_t)g->n + 63) / 64;
     std::vector<uint64_t> skip(nw, 0ull);
     for (size_t v = 0; v < nw * 64; ++v) {
@@ -308,7 +315,9 @@ int graph_build_bigc(tbfs_graph* g) {
     g->nbigo = g->nbigc;
     return TBFS_OK;
   }
-  return build_slices(g->RP, g->n, &g->bigo, &g->nbigo);
+  std::vector<int32_t> rp((size_t)g->n + 1);
+  HIPCHK(hipMemcpy(rp.data(), g->RP, rp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return build_slices(rp.data(), g->n, &g->bigo, &g->nbigo);
 }
 
 // Guards the in-degree statistics kept in each graph (a server's dispatcher
