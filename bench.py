@@ -346,14 +346,17 @@ def bench_serving(tb, graphs, seconds: float = 1.5, rates=(0.25, 0.5, 0.8)):
         rings.append((k, tb.host_array(k * g.n, np.int32), tb.host_array(k * g.n, np.uint64),
                       tb.host_array(k * g.n, np.uint8)))
     use = [0] * len(cs)
+    graph_of = {}  # ticket -> graph
 
     def submit(srv, gi, r):
         k, S, sig, ovf = rings[gi]
         j = use[gi] % k
         use[gi] += 1
         n = cs[gi].n
-        return srv.submit(gi, r, S[j * n:(j + 1) * n], sig[j * n:(j + 1) * n],
-                          ovf[j * n:(j + 1) * n])
+        t = srv.submit(gi, r, S[j * n:(j + 1) * n], sig[j * n:(j + 1) * n],
+                       ovf[j * n:(j + 1) * n])
+        graph_of[t] = gi
+        return t
 
     def draw(rng, count):
         gi = rng.integers(0, len(cs), size=count)
@@ -398,11 +401,17 @@ def bench_serving(tb, graphs, seconds: float = 1.5, rates=(0.25, 0.5, 0.8)):
                 submit(srv, gi, r)
             done = collect(srv, count)
             lat = np.array([d["done_us"] - d["submit_us"] for d in done]) * 1e-3
+            gof = np.array([graph_of[d["ticket"]] for d in done])
+            per_graph = {names[gi]: {"queries": int((gof == gi).sum()),
+                                     "p50_ms": float(np.percentile(lat[gof == gi], 50)),
+                                     "p99_ms": float(np.percentile(lat[gof == gi], 99))}
+                         for gi in range(len(cs)) if (gof == gi).any()}
             span = (max(d["done_us"] for d in done) - min(d["submit_us"] for d in done)) * 1e-6
             out["poisson"].append({
                 "offered_qps": rate, "queries": count, "achieved_qps": count / span,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                 "mean_batch": float(np.mean([d["batch"] for d in done])),
+                "per_graph": per_graph,
             })
         return out
 

@@ -153,6 +153,13 @@ bool step_slot(tbfs_server* s, Slot& sl) {
       return true;
     }
     if (!fin) return false;
+    {  // a single-source run on this graph needs at most the batch's depth
+       // in launches, plus the one that finds the frontier empty
+      tbfs_stats bst{};
+      batch_fill_stats(sl.b, &bst);
+      int& bud = s->budget[sl.graph];
+      bud = std::max(bud, std::min(1 << 16, bst.levels + 2));
+    }
     const size_t k = sl.q.size();
     std::vector<int32_t*> sd(k);
     std::vector<uint64_t*> gd(k);
@@ -186,8 +193,9 @@ bool step_slot(tbfs_server* s, Slot& sl) {
     }
     if (!ready) return false;
     int& bud = s->budget[sl.graph];
-    // the most launches a BFS needed plus one spare; twice the budget if
-    // one ran out
+    // the most launches a BFS needed plus one spare (never lowered: sources
+    // of one graph differ in depth, and a run-out costs a whole BFS); twice
+    // the budget if one ran out
     bud = std::max(bud, std::min(1 << 16, sl.redo.empty() ? maxl + 1 : 2 * bud));
     if (sl.redo.empty()) {
       complete(s, sl, TBFS_OK, s->now_us());
