@@ -978,6 +978,7 @@ int batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch** out) {
 
 int64_t batch_cap_words(const tbfs_batch* b) { return b->cap_words; }
 hipEvent_t batch_step_event(const tbfs_batch* b) { return b->ev_step; }
+hipStream_t batch_stream(const tbfs_batch* b) { return b->stream; }
 
 namespace {
 

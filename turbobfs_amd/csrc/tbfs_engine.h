@@ -44,6 +44,7 @@ struct tbfs_graph {
 struct tbfs_ctx {
   int32_t n_cap = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;           // false: the stream belongs to the creator
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int32_t* S = nullptr;
   uint64_t* sigma = nullptr;
@@ -132,6 +133,8 @@ int graph_create_cooc(const int32_t* I, const int32_t* J, int32_t n,
                       int32_t nz, uint32_t flags, tbfs_graph** out);
 void graph_free(tbfs_graph* g);
 int ctx_create(int32_t n_cap, tbfs_ctx** out);
+// A context whose work goes to an existing stream (not destroyed with it).
+int ctx_create_on(int32_t n_cap, hipStream_t stream, tbfs_ctx** out);
 void ctx_free(tbfs_ctx* c);
 int run_bfs(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
             const tbfs_opts& o, tbfs_stats* st);
@@ -155,6 +158,7 @@ int batch_create_words(int64_t cap_words, tbfs_batch** out);
 void batch_free(tbfs_batch* b);
 int64_t batch_cap_words(const tbfs_batch* b);
 hipEvent_t batch_step_event(const tbfs_batch* b);
+hipStream_t batch_stream(const tbfs_batch* b);
 int batch_start(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t count,
                 const tbfs_opts* o);
 int batch_progress(tbfs_batch* b, bool* finished);

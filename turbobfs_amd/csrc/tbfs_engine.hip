@@ -485,11 +485,13 @@ void ctx_free(tbfs_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
-int ctx_create(int32_t n_cap, tbfs_ctx** out) {
+int ctx_create(int32_t n_cap, tbfs_ctx** out) { return ctx_create_on(n_cap, nullptr, out); }
+
+int ctx_create_on(int32_t n_cap, hipStream_t stream, tbfs_ctx** out) {
   if (!out || n_cap <= 0) return TBFS_ERR_INVALID;
   *out = nullptr;
   tbfs_ctx* c = new (std::nothrow) tbfs_ctx();
@@ -498,7 +500,14 @@ int ctx_create(int32_t n_cap, tbfs_ctx** out) {
   const size_t nw = ((size_t)n_cap + 63) / 64;
   int rc = TBFS_OK;
   hipError_t e = hipGetDevice(&c->device);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    if (stream) {
+      c->stream = stream;
+      c->own_stream = false;
+    } else {
+      e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    }
+  }
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e != hipSuccess) { rc = hip_fail(e, "ctx stream/events", __FILE__, __LINE__); goto fail; }

@@ -331,8 +331,8 @@ void server_free(tbfs_server* s) {
     s->th.join();
   }
   for (Slot& sl : s->slots) {
+    if (sl.c) tbfs_ctx_destroy(sl.c);  // first: it runs on the batch's stream
     batch_free(sl.b);
-    if (sl.c) tbfs_ctx_destroy(sl.c);
   }
   delete s;
 }
@@ -388,9 +388,14 @@ int tbfs_server_create(const tbfs_graph* const* graphs, int32_t ngraphs,
     server_free(s);
     return rc;
   }
+  // a slot's batches and single-source runs share the batch's stream: the
+  // slots' streams, created one after another, then sit on different
+  // hardware queues (a busy stream sharing a queue with another busy one
+  // waits for it)
   s->slots.resize(nslots);
   for (Slot& sl : s->slots) {
     rc = batch_create_words(cap_words, &sl.b);
+    if (!rc) rc = ctx_create_on(nmax, batch_stream(sl.b), &sl.c);
     if (rc) {
       server_free(s);
       return rc;
