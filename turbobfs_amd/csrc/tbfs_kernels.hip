@@ -502,8 +502,10 @@ hipError_t histogram(hipStream_t s, const int32_t* keys, int32_t nz,
 }
 
 hipError_t degree_stats(hipStream_t s, const int32_t* ptr, int32_t n, DegAcc* acc) {
+  // every workgroup ends with ~37 atomics on the same few words: few
+  // workgroups, each looping over many vertices
   int grid = (int)(((int64_t)n + 255) / 256);
-  if (grid > 4096) grid = 4096;
+  if (grid > 512) grid = 512;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(k_degree_stats, dim3(grid), dim3(256), 0, s, ptr, n, acc);
   return hipGetLastError();
