@@ -60,42 +60,36 @@ int graph_build_transpose(tbfs_graph* g, hipStream_t s) {
   if (rc) return rc;
   rc = dev_alloc(&g->JR, (size_t)nz);
   if (rc) return rc;
-  int32_t *col = nullptr, *keys_out = nullptr, *cnt = nullptr;
+  int32_t *col = nullptr, *keys_out = nullptr;
   void* tmp = nullptr;
-  size_t tmp_sort = 0, tmp_scan = 0;
+  size_t tmp_sort = 0;
   int bits = 1;
   while (bits < 31 && (1ll << bits) < (int64_t)n) ++bits;
   rc = dev_alloc(&col, (size_t)nz);
   if (!rc) rc = dev_alloc(&keys_out, (size_t)nz);
-  if (!rc) rc = dev_alloc(&cnt, (size_t)n + 1);
   if (rc) goto out;
   {
+    // sort (row, column) pairs by row: the columns land in JR as the
+    // out-lists, the sorted rows give RP
     hipError_t e;
     e = kern::expand_cols(s, g->CP, n, col);
     if (e != hipSuccess) { rc = hip_fail(e, "expand_cols", __FILE__, __LINE__); goto out; }
     e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_sort, g->IC, keys_out,
                                            col, g->JR, nz, 0, bits, s);
     if (e != hipSuccess) { rc = hip_fail(e, "SortPairs(size)", __FILE__, __LINE__); goto out; }
-    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_scan, cnt, g->RP, n + 1, s);
-    if (e != hipSuccess) { rc = hip_fail(e, "ExclusiveSum(size)", __FILE__, __LINE__); goto out; }
-    rc = dev_alloc((char**)&tmp, std::max(tmp_sort, tmp_scan));
+    rc = dev_alloc((char**)&tmp, std::max<size_t>(tmp_sort, 1));
     if (rc) goto out;
     e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_sort, g->IC, keys_out, col,
                                            g->JR, nz, 0, bits, s);
     if (e != hipSuccess) { rc = hip_fail(e, "SortPairs", __FILE__, __LINE__); goto out; }
-    e = hipMemsetAsync(cnt, 0, ((size_t)n + 1) * sizeof(int32_t), s);
-    if (e != hipSuccess) { rc = hip_fail(e, "hipMemsetAsync", __FILE__, __LINE__); goto out; }
-    e = kern::histogram(s, g->IC, nz, cnt);
-    if (e != hipSuccess) { rc = hip_fail(e, "histogram", __FILE__, __LINE__); goto out; }
-    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_scan, cnt, g->RP, n + 1, s);
-    if (e != hipSuccess) { rc = hip_fail(e, "ExclusiveSum", __FILE__, __LINE__); goto out; }
+    e = kern::offsets_sorted(s, keys_out, nz, n, g->RP);
+    if (e != hipSuccess) { rc = hip_fail(e, "offsets_sorted", __FILE__, __LINE__); goto out; }
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__); goto out; }
   }
 out:
   dev_free(col);
   dev_free(keys_out);
-  dev_free(cnt);
   if (tmp) (void)hipFree(tmp);
   return rc;
 }
@@ -103,28 +97,12 @@ out:
 // Derive CP from a non-decreasing column array J (COOC sorted by column).
 int graph_cp_from_sorted(tbfs_graph* g, const int32_t* Jdev, hipStream_t s) {
   const int32_t n = g->n, nz = g->nz;
-  int32_t* cnt = nullptr;
-  void* tmp = nullptr;
-  size_t tmp_scan = 0;
   int rc = dev_alloc(&g->CP, (size_t)n + 1);
-  if (!rc) rc = dev_alloc(&cnt, (size_t)n + 1);
-  if (rc) goto out;
-  {
-    hipError_t e = hipMemsetAsync(cnt, 0, ((size_t)n + 1) * sizeof(int32_t), s);
-    if (e == hipSuccess) e = kern::histogram(s, Jdev, nz, cnt);
-    if (e == hipSuccess)
-      e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_scan, cnt, g->CP, n + 1, s);
-    if (e != hipSuccess) { rc = hip_fail(e, "cp_from_sorted", __FILE__, __LINE__); goto out; }
-    rc = dev_alloc((char**)&tmp, tmp_scan);
-    if (rc) goto out;
-    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_scan, cnt, g->CP, n + 1, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) { rc = hip_fail(e, "cp_from_sorted", __FILE__, __LINE__); goto out; }
-  }
-out:
-  dev_free(cnt);
-  if (tmp) (void)hipFree(tmp);
-  return rc;
+  if (rc) return rc;
+  hipError_t e = kern::offsets_sorted(s, Jdev, nz, n, g->CP);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "cp_from_sorted", __FILE__, __LINE__);
+  return TBFS_OK;
 }
 
 // ----------------------------------------------------------------------------

@@ -186,8 +186,9 @@ hipError_t copy16(hipStream_t s, const void* src, void* dst, int64_t n16);
 // setup helpers
 hipError_t expand_cols(hipStream_t s, const int32_t* CP, int32_t n,
                        int32_t* col);
-hipError_t histogram(hipStream_t s, const int32_t* keys, int32_t nz,
-                     int32_t* cnt);
+// ptr[r] = first k with keys[k] >= r, r = 0 .. n (keys non-decreasing)
+hipError_t offsets_sorted(hipStream_t s, const int32_t* keys, int32_t nz, int32_t n,
+                          int32_t* ptr);
 hipError_t degree_stats(hipStream_t s, const int32_t* ptr, int32_t n, DegAcc* acc);
 // Degree-ordered numbering of a CSC graph (see graph_batch_view).
 hipError_t order_keys(hipStream_t s, const int32_t* CP, int32_t n, uint32_t* keys, int32_t* ids);

@@ -298,11 +298,20 @@ __global__ void k_expand_cols(const int32_t* CP, int32_t n, int32_t* col) {
   }
 }
 
-__global__ void k_histogram(const int32_t* keys, int32_t nz, int32_t* cnt) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nz;
-       k += stride)
-    atomicAdd(&cnt[keys[k]], 1);
+// Offsets of a non-decreasing key array with keys in [0, n): ptr[r] = the
+// first k with keys[k] >= r, for r = 0 .. n (ptr[n] = nz).  One binary search
+// per offset and no atomics: counting the keys with atomics serialised on the
+// keys of hub lists (1.4 ms for R-MAT scale 20's 31.4M row indices).
+__global__ void k_offsets_sorted(const int32_t* keys, int32_t nz, int32_t n, int32_t* ptr) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > n) return;
+  int32_t lo = 0, hi = nz;
+  while (lo < hi) {
+    const int32_t mid = lo + ((hi - lo) >> 1);
+    if ((int64_t)keys[mid] < r) lo = mid + 1;
+    else hi = mid;
+  }
+  ptr[r] = lo;
 }
 
 // Degree statistics (see DegAcc): per workgroup, LDS bins and wavefront
@@ -492,12 +501,10 @@ hipError_t expand_cols(hipStream_t s, const int32_t* CP, int32_t n,
   return hipGetLastError();
 }
 
-hipError_t histogram(hipStream_t s, const int32_t* keys, int32_t nz,
-                     int32_t* cnt) {
-  int grid = (int)(((int64_t)nz + 255) / 256);
-  if (grid > 65536) grid = 65536;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL(k_histogram, dim3(grid), dim3(256), 0, s, keys, nz, cnt);
+hipError_t offsets_sorted(hipStream_t s, const int32_t* keys, int32_t nz, int32_t n,
+                          int32_t* ptr) {
+  const int grid = (int)(((int64_t)n + 1 + 255) / 256);
+  hipLaunchKernelGGL(k_offsets_sorted, dim3(grid), dim3(256), 0, s, keys, nz, n, ptr);
   return hipGetLastError();
 }
 
