@@ -4,6 +4,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -239,7 +240,22 @@ static hipError_t download_d2h(void* dst, const void* src, size_t bytes, hipStre
   return e;
 }
 
-int graph_build_bigc(tbfs_graph* g, const int32_t* cp_host);
+int graph_build_bigc(tbfs_graph* g, hipStream_t s);
+
+// TBFS_VERBOSE=1: host time of each stage of a graph upload on stdout.
+namespace {
+struct StageClock {
+  using clk = std::chrono::steady_clock;
+  bool on = std::getenv("TBFS_VERBOSE") != nullptr;
+  clk::time_point t = clk::now();
+  double lap() {
+    const clk::time_point u = clk::now();
+    const double ms = std::chrono::duration<double, std::milli>(u - t).count();
+    t = u;
+    return ms;
+  }
+};
+}  // namespace
 
 int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
                      int32_t nz, uint32_t flags, tbfs_graph** out) {
@@ -257,6 +273,8 @@ int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
   g->flags = flags & (TBFS_GRAPH_SYMMETRIC);
   hipStream_t s = nullptr;
   int rc = TBFS_OK;
+  StageClock clock;
+  double ms_up = 0, ms_check = 0, ms_tr = 0;
   hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
   if (e != hipSuccess) { rc = hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); goto fail; }
   rc = dev_alloc(&g->CP, (size_t)n + 1);
@@ -269,6 +287,10 @@ int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
       e = on_dev ? hipMemcpyAsync(g->IC, IC, (size_t)nz * sizeof(int32_t), kind, s)
                  : upload_h2d(g->IC, IC, (size_t)nz * sizeof(int32_t), s);
     if (e != hipSuccess) { rc = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__); goto fail; }
+    if (clock.on) {
+      (void)hipStreamSynchronize(s);
+      ms_up = clock.lap();
+    }
   }
   {
     int32_t* bad = nullptr;
@@ -283,6 +305,7 @@ int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
     if (e != hipSuccess) { rc = hip_fail(e, "check_csc", __FILE__, __LINE__); goto fail; }
     if (hbad) { rc = TBFS_ERR_INVALID; goto fail; }
   }
+  ms_check = clock.lap();
   if (g->flags & TBFS_GRAPH_SYMMETRIC) {
     g->RP = g->CP;
     g->JR = g->IC;
@@ -292,8 +315,12 @@ int graph_create_csc(const int32_t* CP, const int32_t* IC, int32_t n,
   }
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__); goto fail; }
-  rc = graph_build_bigc(g, on_dev ? nullptr : CP);
+  ms_tr = clock.lap();
+  rc = graph_build_bigc(g, s);
   if (rc) goto fail;
+  if (clock.on)
+    std::printf("graph_create_csc ms: alloc + upload %.3f, check %.3f, transpose %.3f, "
+                "slices %.3f\n", ms_up, ms_check, ms_tr, clock.lap());
   (void)hipStreamDestroy(s);
   *out = g;
   return TBFS_OK;
@@ -350,7 +377,7 @@ int graph_create_cooc(const int32_t* I, const int32_t* J, int32_t n, int32_t nz,
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__); goto fail; }
   if (sorted) {
-    rc = graph_build_bigc(g, nullptr);
+    rc = graph_build_bigc(g, s);
     if (rc) goto fail;
   }
   (void)hipStreamDestroy(s);
@@ -373,64 +400,59 @@ int graph_ensure_cooc(tbfs_graph* g, hipStream_t s) {
   return TBFS_OK;
 }
 
-// Static slices (kPullChunk entries) of every list longer than kBigIn: in-lists
-// for the pull levels, out-lists for the push levels.
-// Static slices of the lists longer than kBigIn, from the list offsets ptr
-// (host memory).
-static int build_slices(const int32_t* ptr, int32_t n, uint64_t** out, int32_t* count) {
-  std::vector<uint64_t> big;
-  for (int32_t v = 0; v < n; ++v) {
-    const int64_t deg = (int64_t)ptr[v + 1] - ptr[v];
-    if (deg <= kBigIn) continue;
-    const int64_t nch = (deg + kPullChunk - 1) / kPullChunk;
-    for (int64_t c = 0; c < nch; ++c)
-      big.push_back((uint64_t)(uint32_t)v | ((uint64_t)c << 32));
+// Static slices (kPullChunk entries) of every list longer than kBigIn, from
+// the device list offsets ptr: slices per vertex, their exclusive scan, then
+// the entries in vertex order (in-lists for the pull levels, out-lists for
+// the push levels).  Returns with the table complete.
+static int build_slices(const int32_t* ptr, int32_t n, hipStream_t s, uint64_t** out,
+                        int32_t* count) {
+  int32_t *cnt = nullptr, *off = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int32_t total = 0;
+  hipError_t e = hipSuccess;
+  int rc = dev_alloc(&cnt, (size_t)n + 1);
+  if (!rc) rc = dev_alloc(&off, (size_t)n + 1);
+  if (!rc) {
+    e = kern::slice_count(s, ptr, n, cnt);
+    if (e == hipSuccess)
+      e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, off, n + 1, s);
+    if (e == hipSuccess) rc = dev_alloc((char**)&tmp, std::max<size_t>(tmp_bytes, 1));
   }
-  *count = (int32_t)big.size();
-  int rc = dev_alloc(out, big.size());
-  if (rc) return rc;
-  if (!big.empty())
-    HIPCHK(hipMemcpy(*out, big.data(), big.size() * sizeof(uint64_t),
-                     hipMemcpyHostToDevice));
-  return TBFS_OK;
+  if (!rc && e == hipSuccess) {
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, off, n + 1, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&total, off + n, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  if (!rc && e == hipSuccess) rc = dev_alloc(out, (size_t)total);
+  if (!rc && e == hipSuccess && total > 0) e = kern::slice_fill(s, cnt, off, n, *out);
+  if (!rc && e == hipSuccess) e = hipStreamSynchronize(s);
+  if (!rc && e != hipSuccess) rc = hip_fail(e, "build_slices", __FILE__, __LINE__);
+  if (!rc) *count = total;
+  dev_free(cnt);
+  dev_free(off);
+  if (tmp) (void)hipFree(tmp);
+  return rc;
 }
 
 // Slices of the long in-lists (and out-lists) and the bitmap of columns the
-// light-column pull skips.  cp_host: the in-list offsets in host memory when
-// the caller has them (else they are read back from the device).
-int graph_build_bigc(tbfs_graph* g, const int32_t* cp_host) {
-  std::vector<int32_t> cpv;
-  if (!cp_host) {
-    cpv.resize((size_t)g->n + 1);
-    HIPCHK(hipMemcpy(cpv.data(), g->CP, cpv.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-    cp_host = cpv.data();
-  }
-  int rc = build_slices(cp_host, g->n, &g->bigc, &g->nbigc);
+// light-column pull skips, built on the device from g->CP / g->RP (complete
+// on stream s or earlier).  Returns with every table complete.
+int graph_build_bigc(tbfs_graph* g, hipStream_t s) {
+  int rc = build_slices(g->CP, g->n, s, &g->bigc, &g->nbigc);
   if (rc) return rc;
-  {
-    const int32_t* cp = cp_host;
-    const size_t nw = ((size_t)g->n + 63) / 64;
-    std::vector<uint64_t> skip(nw, 0ull);
-    for (size_t v = 0; v < nw * 64; ++v) {
-      bool sk = true;  // bits past n count as skipped
-      if (v < (size_t)g->n) {
-        const int64_t deg = (int64_t)cp[v + 1] - cp[v];
-        sk = deg == 0 || deg > kBigIn;
-      }
-      if (sk) skip[v >> 6] |= 1ull << (v & 63);
-    }
-    rc = dev_alloc(&g->skip, nw);
-    if (rc) return rc;
-    HIPCHK(hipMemcpy(g->skip, skip.data(), nw * sizeof(uint64_t), hipMemcpyHostToDevice));
-  }
+  const int64_t nw = ((int64_t)g->n + 63) / 64;
+  rc = dev_alloc(&g->skip, (size_t)nw);
+  if (rc) return rc;
+  HIPCHK(kern::skip_bits(s, g->CP, g->n, g->skip, nw));
+  HIPCHK(hipStreamSynchronize(s));
   if (g->RP == g->CP) {
     g->bigo = g->bigc;
     g->nbigo = g->nbigc;
     return TBFS_OK;
   }
-  std::vector<int32_t> rp((size_t)g->n + 1);
-  HIPCHK(hipMemcpy(rp.data(), g->RP, rp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-  return build_slices(rp.data(), g->n, &g->bigo, &g->nbigo);
+  return build_slices(g->RP, g->n, s, &g->bigo, &g->nbigo);
 }
 
 // Guards the in-degree statistics kept in each graph (a server's dispatcher

@@ -186,6 +186,12 @@ hipError_t copy16(hipStream_t s, const void* src, void* dst, int64_t n16);
 // setup helpers
 hipError_t expand_cols(hipStream_t s, const int32_t* CP, int32_t n,
                        int32_t* col);
+// slice tables and skip bitmap of a graph, from device list offsets ptr
+hipError_t slice_count(hipStream_t s, const int32_t* ptr, int32_t n, int32_t* cnt);
+hipError_t slice_fill(hipStream_t s, const int32_t* cnt, const int32_t* off, int32_t n,
+                      uint64_t* out);
+hipError_t skip_bits(hipStream_t s, const int32_t* ptr, int32_t n, uint64_t* skip,
+                     int64_t nw);
 // ptr[r] = first k with keys[k] >= r, r = 0 .. n (keys non-decreasing)
 hipError_t offsets_sorted(hipStream_t s, const int32_t* keys, int32_t nz, int32_t n,
                           int32_t* ptr);

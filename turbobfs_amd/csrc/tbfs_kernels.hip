@@ -298,6 +298,45 @@ __global__ void k_expand_cols(const int32_t* CP, int32_t n, int32_t* col) {
   }
 }
 
+// Static slices of the lists longer than kBigIn (entry v | slice << 32, in
+// vertex then slice order): slices per vertex (cnt[n] = 0, so an exclusive
+// scan of n + 1 counts ends with the total), then the entries at the scanned
+// offsets.
+__global__ void k_slice_count(const int32_t* ptr, int32_t n, int32_t* cnt) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > n) return;
+  int32_t c = 0;
+  if (v < n) {
+    const int64_t deg = (int64_t)ptr[v + 1] - ptr[v];
+    if (deg > kBigIn) c = (int32_t)((deg + kPullChunk - 1) / kPullChunk);
+  }
+  cnt[v] = c;
+}
+
+__global__ void k_slice_fill(const int32_t* cnt, const int32_t* off, int32_t n,
+                             uint64_t* out) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int32_t c = cnt[v], o = off[v];
+  for (int32_t k = 0; k < c; ++k)
+    out[o + k] = (uint64_t)(uint32_t)v | ((uint64_t)(uint32_t)k << 32);
+}
+
+// Bitmap of the columns the light-column pull skips (no in-list, or one split
+// into slices); bits past n are set.  One wavefront per word.
+__global__ void k_skip_bits(const int32_t* ptr, int32_t n, uint64_t* skip, int64_t nw) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t w = v >> 6;
+  if (w >= nw) return;  // uniform per wavefront: 64 lanes share w
+  bool sk = true;
+  if (v < n) {
+    const int64_t deg = (int64_t)ptr[v + 1] - ptr[v];
+    sk = deg == 0 || deg > kBigIn;
+  }
+  const uint64_t bits = __ballot(sk);
+  if ((threadIdx.x & 63) == 0) skip[w] = bits;
+}
+
 // Offsets of a non-decreasing key array with keys in [0, n): ptr[r] = the
 // first k with keys[k] >= r, for r = 0 .. n (ptr[n] = nz).  One binary search
 // per offset and no atomics: counting the keys with atomics serialised on the
@@ -498,6 +537,26 @@ hipError_t expand_cols(hipStream_t s, const int32_t* CP, int32_t n,
   if (grid > 65536) grid = 65536;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(k_expand_cols, dim3(grid), dim3(256), 0, s, CP, n, col);
+  return hipGetLastError();
+}
+
+hipError_t slice_count(hipStream_t s, const int32_t* ptr, int32_t n, int32_t* cnt) {
+  const int grid = (int)(((int64_t)n + 1 + 255) / 256);
+  hipLaunchKernelGGL(k_slice_count, dim3(grid), dim3(256), 0, s, ptr, n, cnt);
+  return hipGetLastError();
+}
+
+hipError_t slice_fill(hipStream_t s, const int32_t* cnt, const int32_t* off, int32_t n,
+                      uint64_t* out) {
+  const int grid = (int)(((int64_t)n + 255) / 256);
+  hipLaunchKernelGGL(k_slice_fill, dim3(grid), dim3(256), 0, s, cnt, off, n, out);
+  return hipGetLastError();
+}
+
+hipError_t skip_bits(hipStream_t s, const int32_t* ptr, int32_t n, uint64_t* skip,
+                     int64_t nw) {
+  const int grid = (int)((nw * 64 + 255) / 256);
+  hipLaunchKernelGGL(k_skip_bits, dim3(grid), dim3(256), 0, s, ptr, n, skip, nw);
   return hipGetLastError();
 }
 
