@@ -154,38 +154,122 @@ def bench_dropin(tb, g, sources, nsrc: int = 4):
     return out
 
 
+def _time_variant(tb, ctx, gr, roots, n, o):
+    ctx.run_many(gr, roots[:1], o)
+    ctx.run_many(gr, roots[:2], o)  # launch plan history for this graph
+    ctx.mark(0)
+    st = ctx.run_many(gr, roots, o)
+    ctx.mark(1)
+    ms = ctx.elapsed(0, 1)
+    edges = sum(x["edges"] for x in st)
+    b_alg = len(roots) * (4 * (n + 1) + 12 * n) + 4 * edges
+    return {"gteps": edges / 2 / (ms * 1e-3) / 1e9, "ms_per_bfs": ms / len(roots),
+            "alg_gbs": b_alg / (ms * 1e-3) / 1e9, "sources": len(roots)}
+
+
 def bench_variants(tb, g, gr, ctx, sources, nsrc: int = 16, names=None):
-    """Every reference variant's kernel mapping on the same resident graph:
-    GTEPS and algorithmic GB/s (B_alg / time) per variant, sources in turn
-    (`names`: only these variants)."""
+    """GTEPS and algorithmic GB/s (B_alg / time) on the resident R-MAT graph,
+    sources in turn (`names`: only these rows).  `entry_points`: the kernels
+    the seven reference entry points run by default (each distinct
+    configuration once, with the entry points it serves; the COOC entry point
+    on a COOC upload of the graph); `reference_mappings`: the reference's own
+    thread / warp mappings, which tbfs_dropin_mapping(TBFS_DROPIN_REFERENCE)
+    selects.  `measured_gbs`: HBM bytes per BFS from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes of this table, divided by the time."""
     from turbobfs_amd import _lib
-    H = _lib.LOOP_HOST
-    variants = [
+    H, CH, LB = _lib.LOOP_HOST, _lib.LOOP_CHAINED, _lib.PULL_LB
+    roots = np.asarray(sources[:nsrc], np.int32)
+    n = g.n
+    traffic = _variant_traffic()
+    out = {"entry_points": {}, "reference_mappings": {}}
+    fast = [
+        ("tdbu_sigma", "bfs_gpu_td_csc_sc, bfs_gpu_td_csc_wa, bfs_gpu_tdbu_csc_sc, "
+                       "bfs_gpu_tdbu_csc_wa",
+         dict(mode=_lib.MODE_AUTO, pull=LB, loop=CH), False),
+        ("tdbu_depth_only", "bfs_gpu_bu_csc_sc, bfs_gpu_bu_csc_wa",
+         dict(mode=_lib.MODE_AUTO, pull=LB, loop=CH, sigma=0, bbreak=1), False),
+        ("tdbu_sigma_cooc_upload", "bfs_gpu_td_cooc_sc",
+         dict(mode=_lib.MODE_AUTO, pull=LB, loop=CH), True),
+    ]
+    for name, eps, kw, cooc in fast:
+        if names and name not in names:
+            continue
+        if cooc:
+            I, J = _cooc_of(g)
+            with tb.Graph(I=I, J=J, n=n) as gc:
+                row = _time_variant(tb, ctx, gc, roots, n, tb.default_opts(**kw))
+        else:
+            row = _time_variant(tb, ctx, gr, roots, n, tb.default_opts(**kw))
+        row["entry_points"] = eps
+        _add_measured(row, traffic.get(name))
+        out["entry_points"][name] = row
+    ref = [
         ("td_csc_sc", dict(mode=_lib.MODE_PULL_SC, loop=H)),
         ("td_csc_wa", dict(mode=_lib.MODE_PULL_WA, loop=H)),
         ("bu_csc_sc_bbreak", dict(mode=_lib.MODE_PULL_SC, loop=H, sigma=0, bbreak=1)),
         ("bu_csc_wa_bbreak", dict(mode=_lib.MODE_PULL_WA, loop=H, sigma=0, bbreak=1)),
         ("td_cooc", dict(mode=_lib.MODE_COOC, loop=H)),
-        ("tdbu_push_pull_host", dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB, loop=H)),
-        ("tdbu_push_pull_chained", dict(mode=_lib.MODE_AUTO, pull=_lib.PULL_LB,
-                                        loop=_lib.LOOP_CHAINED)),
+        ("tdbu_push_pull_host", dict(mode=_lib.MODE_AUTO, pull=LB, loop=H)),
     ]
-    roots = np.asarray(sources[:nsrc], np.int32)
-    n = g.n
-    out = {}
-    for name, kw in variants:
+    for name, kw in ref:
         if names and name not in names:
             continue
-        o = tb.default_opts(**kw)
-        ctx.run_many(gr, roots[:1], o)
+        row = _time_variant(tb, ctx, gr, roots, n, tb.default_opts(**kw))
+        _add_measured(row, traffic.get(name))
+        out["reference_mappings"][name] = row
+    return out
+
+
+def _cooc_of(g):
+    """COOC of a CSC graph (edge IC[k] -> its column), ordered by column."""
+    J = np.repeat(np.arange(g.n, dtype=np.int32), np.diff(g.CP))
+    return np.ascontiguousarray(g.IC), J
+
+
+VARIANT_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "variants_rmat20_traffic.json")
+
+
+def _variant_traffic():
+    try:
+        with open(VARIANT_TRAFFIC_FILE) as f:
+            return json.load(f).get("hbm_bytes_per_bfs", {})
+    except (OSError, ValueError):
+        return {}
+
+
+def _add_measured(row, bytes_per_bfs):
+    if bytes_per_bfs:
+        row["hbm_bytes_per_bfs"] = bytes_per_bfs
+        row["measured_gbs"] = bytes_per_bfs / (row["ms_per_bfs"] * 1e-3) / 1e9
+
+
+def bench_er4096(tb, graphs, reps: int = 256, cpu: bool = True):
+    """The CPU-runnable case: undirected G(n=4096, p=8/4095), seed 1, source
+    0.  `reps` BFS from source 0 back to back on the resident graph; the
+    reference entry point bfs_gpu_td_csc_sc timed as its callers use it; the
+    CPU baseline (the reference's sequential BFS) timed in full."""
+    g = graphs.erdos_renyi(4096, 8.0 / 4095.0, seed=1)
+    roots = np.zeros(reps, np.int32)
+    with tb.Graph(g.CP, g.IC, symmetric=True) as gr, tb.Context(g.n) as ctx:
+        ctx.run_many(gr, roots[:8])
         ctx.mark(0)
-        st = ctx.run_many(gr, roots, o)
+        st = ctx.run_many(gr, roots)
         ctx.mark(1)
         ms = ctx.elapsed(0, 1)
-        edges = sum(x["edges"] for x in st)
-        b_alg = len(roots) * (4 * (n + 1) + 12 * n) + 4 * edges
-        out[name] = {"gteps": edges / 2 / (ms * 1e-3) / 1e9, "ms_per_bfs": ms / len(roots),
-                     "alg_gbs": b_alg / (ms * 1e-3) / 1e9, "sources": len(roots)}
+    edges = sum(x["edges"] for x in st)
+    out = {"workload": "Erdos-Renyi G(4096, 8/4095) seed 1, source 0",
+           "n": g.n, "nz": g.nz, "levels": st[0]["levels"], "bfs": reps,
+           "gteps": edges / 2 / (ms * 1e-3) / 1e9, "us_per_bfs": 1e3 * ms / reps}
+    S = np.zeros(g.n, np.int32)
+    sig = np.zeros(g.n, np.float32)
+    tb.bfs_gpu_td_csc_sc(g.IC, g.CP, S, sig, 0, g.nz, g.n, 1)
+    t0 = time.perf_counter()
+    k = 20
+    for _ in range(k):
+        tb.bfs_gpu_td_csc_sc(g.IC, g.CP, S, sig, 0, g.nz, g.n, 1)
+    out["dropin_bfs_gpu_td_csc_sc_ms_per_call"] = 1e3 * (time.perf_counter() - t0) / k
+    if cpu:  # source 0 repeated until ~2 s of CPU time, every BFS timed in full
+        out["cpu_baseline"] = cpu_baseline(g, [0] * 1000, 2.0)
     return out
 
 
@@ -195,7 +279,11 @@ def bench_batched(tb, graphs, reps: int = 3, tune: bool = True, loop: bool = Tru
     of nonzero degree (seed 4) in one batch, 64 per frontier word."""
     g = graphs.rmat(18, 16, 0.57, 0.19, 0.19, seed=4, relabel=True)
     src = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 1024, seed=4)
-    gr = tb.Graph(g.CP, g.IC, symmetric=True)
+    with tb.Graph(g.CP, g.IC, symmetric=True) as gr:
+        return _bench_batched(tb, g, gr, src, reps, tune, loop)
+
+
+def _bench_batched(tb, g, gr, src, reps, tune, loop):
     b = tb.Batch(g.n, len(src))
     b.run(gr, src)  # warm-up
     runs = [b.run(gr, src) for _ in range(reps)]
@@ -256,21 +344,22 @@ def bench_multigraph(tb, graphs, reps: int = 3):
     bd, off = graphs.block_diagonal(parts)
     roots = np.array([off[i] + int(rng.integers(0, parts[i].n)) for i in range(len(parts))],
                      np.int32)
-    gb = tb.Graph(bd.CP, bd.IC, symmetric=True)
-    ctx = tb.Context(bd.n)
-    ctx.run_multi(gb, roots)
-    runs = [ctx.run_multi(gb, roots) for _ in range(reps)]
+    with tb.Graph(bd.CP, bd.IC, symmetric=True) as gb, tb.Context(bd.n) as ctx:
+        ctx.run_multi(gb, roots)
+        runs = [ctx.run_multi(gb, roots) for _ in range(reps)]
     ms = float(np.median([r["ms"] for r in runs]))
-    ctx.close()
     grs = [tb.Graph(p.CP, p.IC, symmetric=True) for p in parts]
-    cs = tb.Context(max(p.n for p in parts))
-    for i in range(8):
-        cs.run(grs[i], int(roots[i] - off[i]))
-    t0 = time.perf_counter()
-    for i in range(len(parts)):
-        cs.run(grs[i], int(roots[i] - off[i]))
-    loop_ms = (time.perf_counter() - t0) * 1e3
-    cs.close()
+    try:
+        with tb.Context(max(p.n for p in parts)) as cs:
+            for i in range(8):
+                cs.run(grs[i], int(roots[i] - off[i]))
+            t0 = time.perf_counter()
+            for i in range(len(parts)):
+                cs.run(grs[i], int(roots[i] - off[i]))
+            loop_ms = (time.perf_counter() - t0) * 1e3
+    finally:
+        for x in grs:
+            x.close()
     return {
         "workload": "512 Watts-Strogatz graphs n~U[1000,64000] k=10 p=0.1 block-diagonal, "
                     "one source each",
@@ -290,12 +379,16 @@ def bench_grid(tb, graphs, nsrc: int = 16, names=("chained", "graph", "persisten
     component (seed 3).  Per-level cost of each level loop; with cpu_passes,
     the restated sequential BFS timed over its first cpu_passes passes and
     its whole run extrapolated (one thread)."""
-    from turbobfs_amd import _lib
     g = graphs.grid(2048, 2048, 0.1, seed=3, relabel=True)
     lc = graphs.largest_component_vertices(g)
     src = graphs.sample_sources(lc, nsrc, seed=3)
-    gr = tb.Graph(g.CP, g.IC, symmetric=True)
-    ctx = tb.Context(g.n)
+    with tb.Graph(g.CP, g.IC, symmetric=True) as gr, tb.Context(g.n) as ctx, \
+            tb.Batch(g.n, nsrc) as b:
+        return _bench_grid(tb, g, gr, ctx, b, src, nsrc, names, cpu_passes)
+
+
+def _bench_grid(tb, g, gr, ctx, b, src, nsrc, names, cpu_passes):
+    from turbobfs_amd import _lib
     loops = [
         ("chained", dict(loop=_lib.LOOP_CHAINED)),
         ("graph", dict(loop=_lib.LOOP_GRAPH)),
@@ -317,7 +410,6 @@ def bench_grid(tb, graphs, nsrc: int = 16, names=("chained", "graph", "persisten
         out["loops"][name] = {"sources": k, "ms_per_bfs": ms / k, "levels_per_bfs": lv / k,
                               "us_per_level": 1e3 * ms / lv,
                               "gteps": sum(x["edges"] for x in st) / 2 / (ms * 1e-3) / 1e9}
-    b = tb.Batch(g.n, nsrc)
     b.run(gr, src)
     st = b.run(gr, src)
     out["batch_of_sources"] = {"sources": nsrc, "ms": st["ms"], "levels": st["levels"],
@@ -334,8 +426,6 @@ def bench_grid(tb, graphs, nsrc: int = 16, names=("chained", "graph", "persisten
             "sample": f"first {made} of the {levels0 + 1} passes of the restated bfs_seq_td_csc "
                       f"(each pass sweeps every column) from the first source, single thread, "
                       f"{per_pass * made:.1f} s; the whole BFS = passes x time per pass"}
-    ctx.close()
-    b.close()
     return out
 
 
@@ -352,6 +442,14 @@ def bench_serving(tb, graphs, seconds: float = 1.5, rates=(0.25, 0.5, 0.8)):
     cs.append(graphs.grid(1024, 1024, 0.1, seed=3, relabel=True))
     names = [f"rmat{sc}" for sc in range(16, 23)] + ["grid1024"]
     gs = [tb.Graph(g.CP, g.IC, symmetric=True) for g in cs]
+    try:
+        return _bench_serving(tb, cs, names, gs, seconds, rates)
+    finally:
+        for x in gs:
+            x.close()
+
+
+def _bench_serving(tb, cs, names, gs, seconds, rates):
     # result buffers: a ring of pinned rows per graph, reused round-robin
     rings = []
     for g in cs:
@@ -360,14 +458,25 @@ def bench_serving(tb, graphs, seconds: float = 1.5, rates=(0.25, 0.5, 0.8)):
                       tb.host_array(k * g.n, np.uint8)))
     use = [0] * len(cs)
     graph_of = {}  # ticket -> graph
+    # a seeded sample of the queries (every 97th submitted) gets result
+    # buffers of its own, kept for the check at the end
+    sampled = []
+    nsub = [0]
 
     def submit(srv, gi, r):
-        k, S, sig, ovf = rings[gi]
-        j = use[gi] % k
-        use[gi] += 1
         n = cs[gi].n
-        t = srv.submit(gi, r, S[j * n:(j + 1) * n], sig[j * n:(j + 1) * n],
-                       ovf[j * n:(j + 1) * n])
+        nsub[0] += 1
+        if nsub[0] % 97 == 0:
+            bufs = (tb.host_array(n, np.int32), tb.host_array(n, np.uint64),
+                    tb.host_array(n, np.uint8))
+            t = srv.submit(gi, r, *bufs)
+            sampled.append((gi, r) + bufs)
+        else:
+            k, S, sig, ovf = rings[gi]
+            j = use[gi] % k
+            use[gi] += 1
+            t = srv.submit(gi, r, S[j * n:(j + 1) * n], sig[j * n:(j + 1) * n],
+                           ovf[j * n:(j + 1) * n])
         graph_of[t] = gi
         return t
 
@@ -438,10 +547,23 @@ def bench_serving(tb, graphs, seconds: float = 1.5, rates=(0.25, 0.5, 0.8)):
     loop_srv = tb.Server(gs, slots=1, max_batch=1)
     loop = run(loop_srv, 256, [batched["poisson"][0]["offered_qps"]])
     loop_srv.close()
+    # the sampled served results against the single-source engine (tbfs_bfs:
+    # the chained single-source kernels, not the batch kernels that served
+    # most queries); the tests check served results against the oracle
+    exact = 0
+    with tb.Context(max(g.n for g in cs)) as ctx:
+        for gi, r, S, sig, ovf in sampled:
+            ctx.run(gs[gi], r)
+            S2, sig2, ovf2 = ctx.read(cs[gi].n)
+            exact += int(np.array_equal(S, S2) and np.array_equal(sig, sig2)
+                         and np.array_equal(ovf, ovf2))
     result = {
         "workload": "8 resident graphs: R-MAT scales 16-22 (ef16) and a 1024x1024 grid with "
                     "10% of edges removed; (graph, source) uniform, seed 5; results copied to "
                     "pinned host buffers",
+        "results_checked": {"checked": len(sampled), "exact": exact,
+                            "against": "single-source tbfs_bfs on the same resident graph "
+                                       "(every 97th submitted query)"},
         "graphs": {nm: {"n": g.n, "nz": g.nz, "batch_cap": c} for nm, g, c in zip(names, cs, caps)},
         "saturated": batched["saturated"],
         "poisson": batched["poisson"],
@@ -473,13 +595,16 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-sources and multi-graph measurements")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-er", action="store_true",
+                    help="skip the Erdos-Renyi n=4096 (CPU-runnable case) measurement")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the drop-in entry point's PCIe-inclusive timing")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the per-variant (TD / BU / TDBU / COOC) GTEPS table")
     ap.add_argument("--variants", default="",
                     help="comma-separated subset of the per-variant table (default: all)")
-    ap.add_argument("--only", choices=["batch", "multigraph", "grid", "serve", "variants"],
+    ap.add_argument("--only", choices=["batch", "multigraph", "grid", "serve", "variants",
+                                       "er4096"],
                     default=None,
                     help="run only that measurement and print its JSON object (for profiling)")
     ap.add_argument("--no-tune", action="store_true", help="batched run with default options only")
@@ -506,12 +631,12 @@ def main():
     if args.only == "variants":
         g = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)
         sources = graphs.sample_sources(graphs.nonzero_degree_vertices(g), args.sources, seed=2)
-        gr = tb.Graph(g.CP, g.IC, symmetric=True)
-        ctx = tb.Context(g.n)
-        if args.calib_mib > 0:  # reference dispatch for byte counters
-            ctx.calibrate_copy(args.calib_mib << 20, 1)
-        names = [x for x in args.variants.split(",") if x]
-        print(json.dumps({"variants": bench_variants(tb, g, gr, ctx, sources, names=names)}))
+        with tb.Graph(g.CP, g.IC, symmetric=True) as gr, tb.Context(g.n) as ctx:
+            if args.calib_mib > 0:  # reference dispatch for byte counters
+                ctx.calibrate_copy(args.calib_mib << 20, 1)
+            names = [x for x in args.variants.split(",") if x]
+            out = {"variants": bench_variants(tb, g, gr, ctx, sources, names=names)}
+        print(json.dumps(out))
         return
     if args.only:
         fn = {"batch": lambda: bench_batched(tb, graphs, tune=not args.no_tune,
@@ -519,7 +644,8 @@ def main():
               "multigraph": lambda: bench_multigraph(tb, graphs),
               "grid": lambda: bench_grid(tb, graphs, names=tuple(args.grid_loops.split(",")),
                                          cpu_passes=0 if args.no_cpu else 24),
-              "serve": lambda: bench_serving(tb, graphs)}[args.only]
+              "serve": lambda: bench_serving(tb, graphs),
+              "er4096": lambda: bench_er4096(tb, graphs, cpu=not args.no_cpu)}[args.only]
         print(json.dumps({args.only: fn()}))
         return
     g = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)
@@ -548,9 +674,27 @@ def main():
     if args.beta:
         kw["beta"] = args.beta
     opts = tb.default_opts(**kw)
-    gr = tb.Graph(g.CP, g.IC, symmetric=True)
-    ctx = tb.Context(g.n)
+    # every handle is closed before the interpreter exits (the HIP runtime,
+    # and a profiler's library when one is loaded, tear down at exit)
+    with tb.Graph(g.CP, g.IC, symmetric=True) as gr, tb.Context(g.n) as ctx:
+        line = bench_single(tb, graphs, args, g, sources, gr, ctx, opts)
+    if not args.no_er:
+        line["er4096"] = bench_er4096(tb, graphs, cpu=not args.no_cpu)
+    if not args.no_batch:
+        line["batched"] = bench_batched(tb, graphs)
+        line["multi_graph"] = bench_multigraph(tb, graphs)
+        line["high_diameter"] = bench_grid(tb, graphs, cpu_passes=0 if args.no_cpu else 24)
+    if not args.no_serve:
+        line["serving"] = bench_serving(tb, graphs)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(g, sources, args.cpu_budget)
+    print(json.dumps(line))
 
+
+def bench_single(tb, graphs, args, g, sources, gr, ctx, opts):
+    """The headline measurement: args.steps single-source BFS on the resident
+    R-MAT graph, sources in turn; plus the per-variant table and the drop-in
+    entry point's PCIe-inclusive timing."""
     # a plain copy of known size: achievable HBM bandwidth, and the reference
     # dispatch for the FETCH_SIZE / WRITE_SIZE counters of a profiled run
     calib_bytes = args.calib_mib << 20
@@ -592,6 +736,9 @@ def main():
             "workload": f"single-source BFS with exact sigma, R-MAT scale {args.scale} "
                         f"ef16 a.57 b.19 c.19 symmetrized relabelled seed 2",
             "n": n, "nz": g.nz, "sources": len(sources), "source_seed": 2,
+            # the timed steps take the sources in turn: fewer steps than
+            # sources time only the first ones
+            "sources_timed": int(len(set(roots.tolist()))),
             "mode": args.mode, "mean_levels": float(np.mean(levels)),
             "push_levels": float(np.mean([st["push_levels"] for st in stats])),
             "pull_levels": float(np.mean([st["pull_levels"] for st in stats])),
@@ -620,15 +767,7 @@ def main():
                                           names=[x for x in args.variants.split(",") if x])
     if not args.no_dropin:
         line["dropin_pcie_inclusive"] = bench_dropin(tb, g, sources)
-    if not args.no_batch:
-        line["batched"] = bench_batched(tb, graphs)
-        line["multi_graph"] = bench_multigraph(tb, graphs)
-        line["high_diameter"] = bench_grid(tb, graphs, cpu_passes=0 if args.no_cpu else 24)
-    if not args.no_serve:
-        line["serving"] = bench_serving(tb, graphs)
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(g, sources, args.cpu_budget)
-    print(json.dumps(line))
+    return line
 
 
 if __name__ == "__main__":

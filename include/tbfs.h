@@ -296,7 +296,16 @@ int tbfs_batch_read(tbfs_batch *b, int32_t first, int32_t count, int32_t n,
 typedef struct tbfs_server tbfs_server;
 
 typedef struct tbfs_server_opts {
-  int64_t mem_budget; /* device bytes for batch slots (0: min(free/4, 64 GiB)) */
+  int64_t mem_budget; /* device bytes for the slots (0: min(free/4, 64 GiB)),
+                         split evenly over them; each slot's share covers its
+                         batch state (828 bytes per vertex per 64-source word),
+                         its staging of finished rows for the host copies (up
+                         to 64 rows of 13 bytes per vertex, at most 256 MiB)
+                         and its single-source context (~14 bytes per vertex
+                         plus 16 bytes per 64 list entries), all sized for the
+                         largest graph.  Resident graphs and their per-graph
+                         tables (slices, degree-ordered copies) are not
+                         counted: they belong to the graphs */
   int32_t slots;      /* batches in flight at once (0: 4) */
   int32_t max_batch;  /* sources per batch, <= 1024 (0: 1024) */
   int32_t reserved[4];
@@ -316,6 +325,11 @@ typedef struct tbfs_query_done {
 /* graphs[i] stay owned by the caller and must outlive the server. */
 int tbfs_server_create(const tbfs_graph *const *graphs, int32_t ngraphs,
                        const tbfs_server_opts *opts, tbfs_server **out);
+/* The mem_budget that gives `slots` slots room for `words` 64-source words
+ * of the largest of graphs[0..ngraphs-1] each (what tbfs_server_create would
+ * size from it). */
+int tbfs_server_budget_for(const tbfs_graph *const *graphs, int32_t ngraphs,
+                           int32_t slots, int32_t words, int64_t *bytes);
 /* Stops the dispatcher (queries still queued are dropped) and frees slots. */
 int tbfs_server_destroy(tbfs_server *s);
 /* Queue a BFS from root on graphs[graph].  S (n int32), sigma (n uint64) and
@@ -334,9 +348,34 @@ int tbfs_server_drain(tbfs_server *s);
 int tbfs_server_info(tbfs_server *s, int32_t graph, int32_t *batch_cap,
                      int64_t *batches, int64_t *batched, double *now_us);
 
+/* ---- the reference's entry points -------------------------------------
+ * Kernels behind bfs_gpu_td_csc_sc / _wa, bfs_gpu_bu_csc_sc / _wa,
+ * bfs_gpu_tdbu_csc_sc / _wa and bfs_gpu_td_cooc_sc (process-wide; every
+ * mapping gives the same results):
+ *   TBFS_DROPIN_FAST (default): direction-optimizing, load-balanced levels
+ *     on the chained loop, for every entry point (a COOC input whose J is
+ *     not sorted runs the edge-parallel sweep);
+ *   TBFS_DROPIN_REFERENCE: the reference's own mappings on a host-driven
+ *     loop -- TD and BU sweep every unreached column every level, one lane
+ *     per column (_sc) or one wavefront per column (_wa); TDBU switches
+ *     direction (load-balanced levels for _sc, wavefront-per-column pulls
+ *     for _wa); COOC runs the edge-parallel sweep.
+ * Returns the previous mapping (mapping -1: only query it). */
+#define TBFS_DROPIN_FAST 0
+#define TBFS_DROPIN_REFERENCE 1
+int tbfs_dropin_mapping(int mapping);
+
 /* Pinned host memory (hipHostMalloc) for result buffers. */
 void *tbfs_host_alloc(int64_t bytes);
 int tbfs_host_free(void *p);
+
+/* Device memory for callers without a HIP allocator of their own (for
+ * instance the output buffers of tbfs_bfs_noalloc): allocate, free, copy in
+ * any direction (hipMemcpy, synchronous), fill bytes. */
+void *tbfs_device_alloc(int64_t bytes);
+int tbfs_device_free(void *p);
+int tbfs_device_copy(void *dst, const void *src, int64_t bytes);
+int tbfs_device_fill(void *dst, int value, int64_t bytes);
 
 #ifdef __cplusplus
 }

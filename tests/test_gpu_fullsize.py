@@ -59,6 +59,35 @@ def test_rmat20_single_source_every_loop(gpu, rmat20):
     ctx.close()
 
 
+def test_rmat20_sixteen_sources_many(gpu, rmat20):
+    """16 of the 64 benchmark sources, run back to back as the benchmark
+    times them (tbfs_bfs_many, chained loop, launch shapes planned from the
+    previous call), each checked against the oracle; the last source's
+    arrays, and every source's level / vertex / edge counts."""
+    g, src = rmat20
+    gr = tb.Graph(g.CP, g.IC, symmetric=True)
+    ctx = tb.Context(g.n)
+    deg = np.diff(g.CP).astype(np.int64)
+    roots = np.asarray(src[:16], np.int32)
+    ctx.run_many(gr, roots[:4])  # history for the launch plan
+    stats = ctx.run_many(gr, roots)
+    for r, st in zip(roots, stats):
+        S, sig, ovf, lv = oracle.textbook(g.CP, g.IC, int(r))
+        assert st["levels"] == lv, int(r)
+        assert st["reached"] == int((S >= 0).sum()), int(r)
+        assert st["edges"] == int(deg[S >= 0].sum()), int(r)
+    S, sig, ovf = ctx.read(g.n)
+    exp = oracle.textbook(g.CP, g.IC, int(roots[-1]))
+    np.testing.assert_array_equal(S, exp[0])
+    np.testing.assert_array_equal(sig, exp[1])
+    np.testing.assert_array_equal(ovf, exp[2])
+    # and each of the 16 alone, arrays checked
+    for r in roots:
+        print(f"[rmat20] root={r} single", flush=True)
+        _same(ctx.bfs(gr, int(r)), oracle.textbook(g.CP, g.IC, int(r)), f"root {r}")
+    ctx.close()
+
+
 def test_rmat20_reference_mappings(gpu, rmat20):
     """The reference's kernel mappings (one lane per column, one wavefront per
     column, edge-parallel COOC, push) and the depth-only bottom-up sweep."""
@@ -79,7 +108,27 @@ def test_rmat20_reference_mappings(gpu, rmat20):
                                          loop=L.LOOP_CHAINED, alpha=1e9, sigma=0))
     np.testing.assert_array_equal(res.S, exp[0])
     assert res.stats["levels"] == exp[3]
+    # the bottom-up entry points' mappings with bbreak (stop a column at its
+    # first parent): depths only
+    for name, mode in (("bu_csc_sc", L.MODE_PULL_SC), ("bu_csc_wa", L.MODE_PULL_WA)):
+        for rr in (r, int(src[1])):
+            e = exp if rr == r else oracle.textbook(g.CP, g.IC, rr)
+            print(f"[rmat20] root={rr} mode={name} bbreak=1", flush=True)
+            res = ctx.bfs(gr, rr, tb.default_opts(mode=mode, loop=L.LOOP_HOST, sigma=0,
+                                                  bbreak=1))
+            np.testing.assert_array_equal(res.S, e[0], err_msg=name)
+            assert res.stats["levels"] == e[3], name
     ctx.close()
+
+
+def test_rmat20_dropin_entry_points(gpu, rmat20):
+    """The seven entry points at full size (host arrays in and out, the
+    default kernels): S, and sigma in the reference's type, against the
+    oracle, with the inexact status exactly where a count leaves it."""
+    from tests.test_gpu_parity import _check_dropin
+    g, src = rmat20
+    print(f"[rmat20] drop-in root={int(src[0])}", flush=True)
+    _check_dropin(g, int(src[0]), mappings=(L.DROPIN_FAST,))
 
 
 def test_grid2048_high_diameter(gpu):
@@ -90,10 +139,13 @@ def test_grid2048_high_diameter(gpu):
     for r in src[:2]:
         exp = oracle.textbook(g.CP, g.IC, int(r))
         assert exp[3] > 2000  # thousands of levels, most of them tiny
-        for name, loop in (("chained", L.LOOP_CHAINED), ("graph", L.LOOP_GRAPH)):
+        loops = [("chained", dict(loop=L.LOOP_CHAINED)), ("graph", dict(loop=L.LOOP_GRAPH))]
+        if r == src[0]:  # the benchmark's other two loops, on one source
+            loops += [("persistent_1024x64", dict(loop=L.LOOP_PERSISTENT, block=1024, grid=64)),
+                      ("host", dict(loop=L.LOOP_HOST))]
+        for name, kw in loops:
             print(f"[grid2048] root={r} loop={name} levels={exp[3]}", flush=True)
-            res = ctx.bfs(gr, int(r), tb.default_opts(mode=L.MODE_AUTO, pull=L.PULL_LB,
-                                                     loop=loop))
+            res = ctx.bfs(gr, int(r), tb.default_opts(mode=L.MODE_AUTO, pull=L.PULL_LB, **kw))
             _same(res, exp, f"{name} root {r}")
     ctx.close()
 
@@ -187,3 +239,65 @@ def test_serving_mix_at_scale(gpu):
         np.testing.assert_array_equal(sig, esig, err_msg=f"sigma graph {gi} root {r}")
         np.testing.assert_array_equal(ovf, eovf, err_msg=f"ovf graph {gi} root {r}")
     srv.close()
+
+
+def _serving_graphs():
+    """The serving benchmark's eight resident graphs: R-MAT scales 16..22
+    (ef16, a=.57 b=.19 c=.19, symmetrized, relabelled, seed = scale) and a
+    1024 x 1024 grid with 10% of its edges removed (seed 3)."""
+    cs = [graphs.rmat(sc, 16, 0.57, 0.19, 0.19, seed=sc, relabel=True) for sc in range(16, 23)]
+    cs.append(graphs.grid(1024, 1024, 0.1, seed=3, relabel=True))
+    return cs
+
+
+@pytest.mark.timeout(600)
+def test_serving_benchmark_graph_set(gpu):
+    """The serving benchmark's graph set under the default server: every graph
+    is queried (seed 5) with results checked bit for bit against the oracle,
+    and the graphs whose batch cap is below 1024 sources (R-MAT 21 and 22:
+    512 and 256 with the default budget) get bursts larger than the cap, so
+    batches at the cap run too; a checked query is the same in a full batch
+    as alone."""
+    cs = _serving_graphs()
+    gs = [tb.Graph(g.CP, g.IC, symmetric=True) for g in cs]
+    srv = tb.Server(gs)
+    caps = [srv.info(i)["batch_cap"] for i in range(len(gs))]
+    print(f"[serve8] caps={caps}", flush=True)
+    rng = np.random.default_rng(5)
+    store = {}
+    capped = {}
+    for gi, g in enumerate(cs):
+        n = g.n
+        extra = 2 * caps[gi] if caps[gi] < 1024 else 0  # unchecked: fill the cap
+        roots = [int(rng.integers(0, n)) for _ in range(4)]
+        for r in roots[:2]:
+            bufs = (tb.host_array(n, np.int32), tb.host_array(n, np.uint64),
+                    tb.host_array(n, np.uint8))
+            store[srv.submit(gi, r, *bufs)] = (gi, r) + bufs
+        for _ in range(extra):
+            capped[srv.submit(gi, int(rng.integers(0, n)))] = gi
+        for r in roots[2:]:
+            bufs = (tb.host_array(n, np.int32), tb.host_array(n, np.uint64),
+                    tb.host_array(n, np.uint8))
+            store[srv.submit(gi, r, *bufs)] = (gi, r) + bufs
+    total = len(store) + len(capped)
+    done = []
+    while len(done) < total:
+        done += srv.poll(8192, wait_us=100000)
+    assert all(d["status"] == 0 for d in done)
+    for gi in range(len(cs)):
+        if caps[gi] < 1024:
+            sizes = [d["batch"] for d in done if d["graph"] == gi]
+            assert max(sizes) == caps[gi], (gi, caps[gi], max(sizes))
+    srv.close()
+    for d in done:
+        if d["ticket"] not in store:
+            continue
+        gi, r, S, sig, ovf = store[d["ticket"]]
+        print(f"[serve8] check graph={gi} root={r} batch={d['batch']}", flush=True)
+        eS, esig, eovf, _ = oracle.textbook(cs[gi].CP, cs[gi].IC, r)
+        np.testing.assert_array_equal(S, eS, err_msg=f"S graph {gi} root {r}")
+        np.testing.assert_array_equal(sig, esig, err_msg=f"sigma graph {gi} root {r}")
+        np.testing.assert_array_equal(ovf, eovf, err_msg=f"ovf graph {gi} root {r}")
+    for x in gs:
+        x.close()

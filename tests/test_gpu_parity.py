@@ -49,15 +49,36 @@ def _dropin_all(g, r, directed_ok=True):
     return out
 
 
-def _check_dropin(g, r):
+MAPPINGS = (tb._lib.DROPIN_FAST, tb._lib.DROPIN_REFERENCE)
+
+
+def _dropin_with(mapping, g, r):
+    """_dropin_all under one kernel mapping of the entry points."""
+    prev = tb.dropin_mapping(mapping)
+    try:
+        return _dropin_all(g, r)
+    finally:
+        tb.dropin_mapping(prev)
+
+
+def _check_dropin(g, r, mappings=MAPPINGS):
+    """Every entry point under each kernel mapping against the oracle: the
+    reference's output conventions and types, TBFS_SIGMA_INEXACT (5) exactly
+    when oracle.count_exceed finds a count past the type's exact range."""
     S0, sig0, ovf0, _ = oracle.seq_td_csc(g.CP, g.IC, r, s_unreached=0)
     S1, sig1, ovf1, _ = oracle.seq_td_csc(g.CP, g.IC, r, s_unreached=-1)
     exp_f32 = np.where(ovf1 == 1, np.float32(np.inf), sig1.astype(np.float32))
-    f32_inexact = bool((ovf1 == 1).any() or (sig1 > F32_EXACT).any())
-    i32_inexact = bool((ovf1 == 1).any() or (sig1 > I32_MAX).any())
+    f32_inexact = oracle.count_exceed(sig1, ovf1, F32_EXACT) > 0
+    i32_inexact = oracle.count_exceed(sig1, ovf1, I32_MAX) > 0
     exp_i32 = (sig1 & np.uint64(0xFFFFFFFF)).astype(np.uint32).view(np.int32)
-    res = _dropin_all(g, r)
+    for m in mappings:
+        _check_outputs(_dropin_with(m, g, r), S0, S1, exp_f32, exp_i32, f32_inexact,
+                       i32_inexact, m)
+
+
+def _check_outputs(res, S0, S1, exp_f32, exp_i32, f32_inexact, i32_inexact, m):
     for name, (rc, S, sig) in res.items():
+        name = f"{name} (mapping {m})"
         if name.startswith("bfs_gpu_td_csc"):
             assert rc == (5 if f32_inexact else 0), name
             np.testing.assert_array_equal(S, S0, err_msg=name)
@@ -335,16 +356,21 @@ def test_dropin_equals_reference_golden_vectors(gpu, fixture):
     else:
         g = mtx.read_csc(golden("luxembourg_osm.mtx.gz"))
     for i, r in enumerate(fx["roots"]):
-        res = _dropin_all(g, int(r))
-        for name, (rc, S, sig) in res.items():
-            assert rc == 0, name
-            if name.startswith("bfs_gpu_td_csc"):
-                np.testing.assert_array_equal(S, fx["S_td"][i], err_msg=name)
-                np.testing.assert_array_equal(sig, fx["sigma_td"][i], err_msg=name)
-            elif name.startswith("bfs_gpu_bu_csc"):
-                np.testing.assert_array_equal(S, fx["S_bu"][i], err_msg=name)
-            elif name.startswith("bfs_gpu_tdbu_csc"):
-                np.testing.assert_array_equal(S, fx["S_bu"][i], err_msg=name)
-                np.testing.assert_array_equal(sig, fx["sigma_tdbu"][i], err_msg=name)
-            else:
-                np.testing.assert_array_equal(sig, fx["sigma_cooc"][i], err_msg=name)
+        for m in MAPPINGS:
+            _check_golden(_dropin_with(m, g, int(r)), fx, i, m)
+
+
+def _check_golden(res, fx, i, m):
+    for name, (rc, S, sig) in res.items():
+        name = f"{name} (mapping {m})"
+        assert rc == 0, name
+        if name.startswith("bfs_gpu_td_csc"):
+            np.testing.assert_array_equal(S, fx["S_td"][i], err_msg=name)
+            np.testing.assert_array_equal(sig, fx["sigma_td"][i], err_msg=name)
+        elif name.startswith("bfs_gpu_bu_csc"):
+            np.testing.assert_array_equal(S, fx["S_bu"][i], err_msg=name)
+        elif name.startswith("bfs_gpu_tdbu_csc"):
+            np.testing.assert_array_equal(S, fx["S_bu"][i], err_msg=name)
+            np.testing.assert_array_equal(sig, fx["sigma_tdbu"][i], err_msg=name)
+        else:
+            np.testing.assert_array_equal(sig, fx["sigma_cooc"][i], err_msg=name)

@@ -114,9 +114,8 @@ def test_serve_alone_equals_in_mix(gpu):
 
 def test_serve_small_budget_limits_batches(gpu):
     cs, gs = _graph_set()
-    # room for one 64-source word of the largest graph per slot
-    nmax = max(g.n for g in cs)
-    srv = tb.Server(gs, mem_budget=2 * nmax * 828 * 2, slots=2)
+    # room for two 64-source words of the largest graph per slot
+    srv = tb.Server(gs, mem_budget=tb.server_budget_for(gs, 2, 2), slots=2)
     assert srv.info(0)["batch_cap"] == 128
     store = {}
     rng = np.random.default_rng(3)

@@ -300,3 +300,26 @@ def test_reference_int_counts_break_discovery_past_int32():
     np.testing.assert_array_equal(S_i[ok], S[ok])
     np.testing.assert_array_equal(sig_i[ok].astype(np.uint64), sig[ok])
     assert not np.array_equal(S_i, S)  # the reference's own result breaks
+
+
+def test_count_exceed_boundaries():
+    """oracle.count_exceed(sigma, ovf, limit) = number of vertices whose exact
+    count is above `limit` (sigma > limit, or the 2^64 overflow flag): the
+    test the drop-in entry points' TBFS_SIGMA_INEXACT status follows, at the
+    float limit 2^24 and the int32 limit 2^31 - 1."""
+    for limit in (2**24, 2**31 - 1):
+        sig = np.array([0, 1, limit - 1, limit, limit + 1, 2**63, 2**64 - 1, 5], np.uint64)
+        ovf = np.array([0, 0, 0, 0, 0, 0, 0, 1], np.uint8)
+        # above the limit: limit + 1, 2^63, 2^64 - 1, and the flagged 5
+        assert oracle.count_exceed(sig, ovf, limit) == 4
+        assert oracle.count_exceed(sig[:4], ovf[:4], limit) == 0  # == limit is exact
+        assert oracle.count_exceed(sig, np.zeros_like(ovf), limit) == 3
+    # closed form: a k x k grid from its corner has C(i + j, i) paths to (i, j)
+    from math import comb
+    k = 24
+    g = graphs.grid(k, k, 0.0, seed=0, relabel=False)
+    S, sig, ovf, _ = oracle.textbook(g.CP, g.IC, 0)
+    for limit in (2**24, 2**31 - 1):
+        want = sum(1 for i in range(k) for j in range(k) if comb(i + j, i) > limit)
+        assert want > 0
+        assert oracle.count_exceed(sig, ovf, limit) == want

@@ -23,15 +23,17 @@ from .bfs import (  # noqa: F401
     batch_tune_clear,
     batch_tuned,
     default_opts,
+    dropin_mapping,
     host_array,
+    server_budget_for,
 )
 
 __all__ = [
     "Batch", "Context", "Graph", "Server", "host_array", "Result", "default_opts", "graphs", "load", "LIB_PATH",
-    "batch_tuned", "batch_tune_clear",
+    "batch_tuned", "batch_tune_clear", "server_budget_for",
     "device_count", "device_info",
     "TbfsError",
     "bfs_gpu_td_csc_sc", "bfs_gpu_td_csc_wa", "bfs_gpu_bu_csc_sc",
     "bfs_gpu_bu_csc_wa", "bfs_gpu_tdbu_csc_sc", "bfs_gpu_tdbu_csc_wa",
-    "bfs_gpu_td_cooc_sc",
+    "bfs_gpu_td_cooc_sc", "dropin_mapping",
 ]

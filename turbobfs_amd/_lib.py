@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(_HERE, "libturbobfs.so")
 
 # status codes (include/tbfs.h)
 TBFS_OK = 0
+DROPIN_FAST = 0
+DROPIN_REFERENCE = 1
 TBFS_ERR_INVALID = 1
 TBFS_ERR_HIP = 2
 TBFS_ERR_NOMEM = 3
@@ -160,12 +162,19 @@ SIGNATURES = {
                                       C.POINTER(Opts), C.POINTER(C.c_double)]),
     "tbfs_batch_read": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _i32p, _u64p, _u8p]),
     "tbfs_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
+    "tbfs_server_budget_for": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32,
+                                         C.POINTER(C.c_int64)]),
     "tbfs_server_destroy": (C.c_int, [C.c_void_p]),
     "tbfs_server_submit": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _i32p, _u64p, _u8p, C.POINTER(C.c_uint64)]),
     "tbfs_server_poll": (C.c_int, [C.c_void_p, C.POINTER(QueryDone), C.c_int32, C.POINTER(C.c_int32), C.c_int64]),
     "tbfs_server_drain": (C.c_int, [C.c_void_p]),
     "tbfs_server_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "tbfs_dropin_mapping": (C.c_int, [C.c_int]),
+    "tbfs_device_alloc": (C.c_void_p, [C.c_int64]),
+    "tbfs_device_free": (C.c_int, [C.c_void_p]),
+    "tbfs_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "tbfs_device_fill": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
     "tbfs_host_alloc": (C.c_void_p, [C.c_int64]),
     "tbfs_host_free": (C.c_int, [C.c_void_p]),
     # generators (tbfs_gen.h)

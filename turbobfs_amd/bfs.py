@@ -11,6 +11,8 @@
 from __future__ import annotations
 
 import ctypes as C
+import sys
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -34,6 +36,31 @@ def _req(a, dtype, n=None, name="array"):
     if n is not None and a.shape[0] < n:
         raise ValueError(f"{name} has {a.shape[0]} entries, needs {n}")
     return a
+
+
+def _release(obj) -> None:
+    """Garbage-collection path of the handle classes.  During interpreter
+    shutdown the HIP runtime may already be tearing down (its own exit
+    handlers, or a profiler's), so no native call is made then: the process
+    is ending and the driver reclaims the device memory.  Close handles
+    explicitly (close() or a with-block) to free them earlier."""
+    if sys.is_finalizing():
+        return
+    try:
+        obj.close()
+    except Exception:
+        pass
+
+
+class _Handle:
+    """Context-manager support shared by the handle classes."""
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
 
 # ---------------------------------------------------------------- drop-in ---
@@ -90,6 +117,17 @@ def bfs_gpu_td_cooc_sc(I_h, J_h, sigma_h, r, nz, n, repetition) -> int:
         _ptr(_req(sigma_h, np.float32, n, "sigma_h"), _f32p), r, nz, n, repetition)
 
 
+def dropin_mapping(mapping: int = -1) -> int:
+    """Select the kernels behind the seven entry points above (process-wide):
+    _lib.DROPIN_FAST (default) or _lib.DROPIN_REFERENCE (the reference's
+    one-lane / one-wavefront mappings); -1 only queries.  Returns the
+    previous mapping."""
+    rc = _lib.load().tbfs_dropin_mapping(int(mapping))
+    if rc not in (_lib.DROPIN_FAST, _lib.DROPIN_REFERENCE):
+        check(rc, "tbfs_dropin_mapping")
+    return rc
+
+
 # ------------------------------------------------------- resident graphs ---
 def default_opts(**kw) -> Opts:
     o = Opts()
@@ -101,7 +139,7 @@ def default_opts(**kw) -> Opts:
     return o
 
 
-class Graph:
+class Graph(_Handle):
     """A CSC (or COOC) graph resident in HBM."""
 
     def __init__(self, CP=None, IC=None, *, I=None, J=None, n=None,
@@ -149,10 +187,7 @@ class Graph:
             self._h = C.c_void_p()
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _release(self)
 
 
 @dataclass
@@ -163,7 +198,7 @@ class Result:
     stats: dict
 
 
-class Context:
+class Context(_Handle):
     """Stream + per-BFS device buffers for graphs of up to n_cap vertices."""
 
     def __init__(self, n_cap: int):
@@ -242,13 +277,10 @@ class Context:
             self._h = C.c_void_p()
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _release(self)
 
 
-class Batch:
+class Batch(_Handle):
     """Many sources per BFS: up to b_cap (<= 1024) roots on one graph at once,
     frontiers bit-packed 64 sources per word.  Each source's result equals
     Context.bfs from that root."""
@@ -310,10 +342,7 @@ class Batch:
             self._h = C.c_void_p()
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _release(self)
 
 
 def batch_tuned(g: Graph, count: int) -> Opts | None:
@@ -341,12 +370,27 @@ def host_array(count: int, dtype) -> np.ndarray:
         raise MemoryError("tbfs_host_alloc failed")
     buf = (C.c_uint8 * nbytes).from_address(ptr)
     arr = np.frombuffer(buf, dtype=dt, count=int(count))
-    import weakref
-    weakref.finalize(buf, _lib.load().tbfs_host_free, C.c_void_p(ptr))
+    fin = weakref.finalize(buf, _host_free, ptr)
+    fin.atexit = False  # at exit the process's pinned memory goes with it
     return arr
 
 
-class Server:
+def _host_free(ptr: int) -> None:
+    if not sys.is_finalizing():
+        _lib.load().tbfs_host_free(C.c_void_p(ptr))
+
+
+def server_budget_for(graphs, slots: int, words: int) -> int:
+    """The Server mem_budget that gives `slots` slots room for `words`
+    64-source words of the largest of `graphs` each."""
+    arr = (C.c_void_p * len(graphs))(*[g.handle for g in graphs])
+    b = C.c_int64()
+    check(_lib.load().tbfs_server_budget_for(arr, len(graphs), int(slots), int(words),
+                                             C.byref(b)), "tbfs_server_budget_for")
+    return int(b.value)
+
+
+class Server(_Handle):
     """Resident graphs + a native dispatcher that batches queries as they
     arrive (continuous batching over concurrent batch slots)."""
 
@@ -363,14 +407,26 @@ class Server:
         self._bufs = {}
 
     def submit(self, graph: int, root: int, S=None, sigma=None, ovf=None) -> int:
-        """Queue a query; S / sigma / ovf (numpy int32 / uint64 / uint8 of n
-        entries, or None) receive the result and must stay alive until done."""
+        """Queue a query; S / sigma / ovf (C-contiguous numpy int32 / uint64 /
+        uint8 of at least n entries, n = that graph's vertex count, or None)
+        receive the result.  The server keeps a reference to them until poll()
+        reports the query done, since the device writes them asynchronously."""
+        # (a graph index out of range is the library's to reject)
+        n = self._graphs[int(graph)].n if 0 <= int(graph) < len(self._graphs) else 0
+        if S is not None:
+            _req(S, np.int32, n, "S")
+        if sigma is not None:
+            _req(sigma, np.uint64, n, "sigma")
+        if ovf is not None:
+            _req(ovf, np.uint8, n, "ovf")
         t = C.c_uint64()
         check(_lib.load().tbfs_server_submit(
             self._h, int(graph), int(root),
             _ptr(S, _i32p) if S is not None else None,
             _ptr(sigma, _u64p) if sigma is not None else None,
             _ptr(ovf, _u8p) if ovf is not None else None, C.byref(t)), "tbfs_server_submit")
+        if S is not None or sigma is not None or ovf is not None:
+            self._bufs[int(t.value)] = (S, sigma, ovf)
         return int(t.value)
 
     def poll(self, max_items: int = 1024, wait_us: int = 0) -> list[dict]:
@@ -378,7 +434,10 @@ class Server:
         k = C.c_int32()
         check(_lib.load().tbfs_server_poll(self._h, buf, int(max_items), C.byref(k), int(wait_us)),
               "tbfs_server_poll")
-        return [{f: getattr(buf[i], f) for f, _ in _lib.QueryDone._fields_} for i in range(k.value)]
+        out = [{f: getattr(buf[i], f) for f, _ in _lib.QueryDone._fields_} for i in range(k.value)]
+        for d in out:  # the device no longer writes these buffers
+            self._bufs.pop(d["ticket"], None)
+        return out
 
     def drain(self) -> None:
         check(_lib.load().tbfs_server_drain(self._h), "tbfs_server_drain")
@@ -397,9 +456,7 @@ class Server:
         if self._h:
             _lib.load().tbfs_server_destroy(self._h)
             self._h = C.c_void_p()
+        self._bufs.clear()  # destroy waited for the device's last copies
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _release(self)

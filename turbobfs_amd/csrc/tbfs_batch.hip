@@ -74,6 +74,7 @@ namespace {
 using namespace dev;
 
 constexpr int kBatchSlice = 256;  // in-list entries per pull slice
+static_assert(kBatchSlice == kPullChunk, "batch slices come from the shared slice builder");
 constexpr int kBatchBlock = 256;  // threads per workgroup (4 wavefronts)
 constexpr int kBatchMaxW = 16;    // words per vertex: up to 1024 sources
 constexpr float kBatchAlpha = 32.0f;       // default push / pull switch
@@ -982,30 +983,20 @@ hipStream_t batch_stream(const tbfs_batch* b) { return b->stream; }
 
 namespace {
 
-// Pull slices of the in-lists (built on first use, kept with the graph).
-int graph_ensure_batch_slices(tbfs_graph* g) {
+// Pull slices of every in-list (built on first use on stream s, kept with
+// the graph).  A server's dispatcher and a caller's thread may start batches
+// on the same graph: the lock makes one of them build the table.
+std::mutex g_bsl_mu;
+
+int graph_ensure_batch_slices(tbfs_graph* g, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_bsl_mu);
   if (g->bsl) return TBFS_OK;
-  std::vector<int32_t> cp((size_t)g->n + 1);
-  HIPCHK(hipMemcpy(cp.data(), g->CP, cp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-  std::vector<uint64_t> sl;
-  sl.reserve((size_t)g->n);
-  for (int32_t v = 0; v < g->n; ++v) {
-    const int64_t deg = (int64_t)cp[v + 1] - cp[v];
-    const int64_t ns = (deg + kBatchSlice - 1) / kBatchSlice;
-    for (int64_t c = 0; c < ns; ++c) sl.push_back((uint64_t)(uint32_t)v | ((uint64_t)c << 32));
-  }
   uint64_t* d = nullptr;
-  int rc = dev_alloc(&d, sl.size());
+  int32_t count = 0;
+  int rc = build_slices(g->CP, g->n, 0, s, &d, &count);
   if (rc) return rc;
-  if (!sl.empty()) {
-    hipError_t e = hipMemcpy(d, sl.data(), sl.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      dev_free(d);
-      return hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
-    }
-  }
+  g->nbsl = count;
   g->bsl = d;
-  g->nbsl = (int32_t)sl.size();
   return TBFS_OK;
 }
 
@@ -1207,7 +1198,7 @@ int batch_start(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32
     }
     g = gv;
   }
-  rc = graph_ensure_batch_slices(g);
+  rc = graph_ensure_batch_slices(g, b->stream);
   if (rc) return rc;
   b->g = g;
   b->n = g->n;

@@ -9,6 +9,7 @@
 // nonzero status codes instead of exit().
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -29,6 +30,27 @@ namespace {
 bool verbose() {
   const char* v = std::getenv("TBFS_VERBOSE");
   return v && v[0] && v[0] != '0';
+}
+
+// Kernels behind the seven entry points (tbfs_dropin_mapping): the
+// load-balanced direction-optimizing levels on the chained loop, or the
+// reference's own thread / warp mappings on a host-driven loop.  Results are
+// the same either way.
+std::atomic<int> g_mapping{TBFS_DROPIN_FAST};
+
+bool fast() { return g_mapping.load(std::memory_order_relaxed) == TBFS_DROPIN_FAST; }
+
+// The fast engine's options for an entry point: direction-optimizing,
+// load-balanced levels, chained loop; sigma off for the depth-only BU
+// entry points.
+tbfs_opts fast_opts(bool sigma) {
+  tbfs_opts o;
+  tbfs_opts_default(&o);
+  o.mode = TBFS_MODE_AUTO;
+  o.pull = TBFS_PULL_LB;
+  o.loop = TBFS_LOOP_CHAINED;
+  o.sigma = sigma ? 1 : 0;
+  return o;
 }
 
 struct DropinResult {
@@ -69,9 +91,16 @@ int run_dropin(const char* name, const int* A, const int* B, int nz, int n,
     graph_free(g);
     return rc;
   }
+  // a COOC graph whose columns are not sorted has no CSC view: only the
+  // edge-parallel sweep runs on it
+  tbfs_opts oo = o;
+  if (cooc && !g->CP) {
+    oo.mode = TBFS_MODE_COOC;
+    oo.loop = TBFS_LOOP_HOST;
+  }
   const auto t2 = clk::now();
   for (int i = 0; i < repetition && !rc; ++i) {
-    rc = run_bfs(c, g, r, o, &res->st);
+    rc = run_bfs(c, g, r, oo, &res->st);
     res->total_ms += res->st.ms;
   }
   if (!rc) {
@@ -147,11 +176,12 @@ int to_int(const char* name, const DropinResult& res, int* out, int n) {
 int td_csc(const char* name, int* IC_h, int* CP_h, int* S_h, float* sigma_h,
            int r, int nz, int n, int repetition, int mode) {
   if (!S_h || !sigma_h) return TBFS_ERR_INVALID;
-  tbfs_opts o;
-  tbfs_opts_default(&o);
-  o.mode = mode;
-  o.loop = TBFS_LOOP_HOST;
-  o.pull = mode == TBFS_MODE_PULL_WA ? TBFS_PULL_WA : TBFS_PULL_SC;
+  tbfs_opts o = fast_opts(true);
+  if (!fast()) {
+    o.mode = mode;
+    o.loop = TBFS_LOOP_HOST;
+    o.pull = mode == TBFS_MODE_PULL_WA ? TBFS_PULL_WA : TBFS_PULL_SC;
+  }
   DropinResult res;
   int rc = run_dropin(name, CP_h, IC_h, nz, n, r, repetition, o, false, true, &res);
   if (rc) return rc;
@@ -162,12 +192,12 @@ int td_csc(const char* name, int* IC_h, int* CP_h, int* S_h, float* sigma_h,
 int bu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int r, int nz,
            int n, int repetition, int bbreak, int mode) {
   if (!S_h) return TBFS_ERR_INVALID;
-  tbfs_opts o;
-  tbfs_opts_default(&o);
-  o.mode = mode;
-  o.loop = TBFS_LOOP_HOST;
-  o.pull = mode == TBFS_MODE_PULL_WA ? TBFS_PULL_WA : TBFS_PULL_SC;
-  o.sigma = 0;
+  tbfs_opts o = fast_opts(false);
+  if (!fast()) {
+    o.mode = mode;
+    o.loop = TBFS_LOOP_HOST;
+    o.pull = mode == TBFS_MODE_PULL_WA ? TBFS_PULL_WA : TBFS_PULL_SC;
+  }
   o.bbreak = bbreak ? 1 : 0;
   DropinResult res;
   int rc = run_dropin(name, CP_h, I_h, nz, n, r, repetition, o, false, false, &res);
@@ -179,11 +209,11 @@ int bu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int r, int nz,
 int tdbu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int* sigma_h,
              int r, int nz, int n, int repetition, int pull) {
   if (!S_h || !sigma_h) return TBFS_ERR_INVALID;
-  tbfs_opts o;
-  tbfs_opts_default(&o);
-  o.mode = TBFS_MODE_AUTO;
-  o.pull = pull;
-  o.loop = pull == TBFS_PULL_LB ? TBFS_LOOP_CHAINED : TBFS_LOOP_HOST;
+  tbfs_opts o = fast_opts(true);
+  if (!fast()) {
+    o.pull = pull;
+    o.loop = pull == TBFS_PULL_LB ? TBFS_LOOP_CHAINED : TBFS_LOOP_HOST;
+  }
   DropinResult res;
   int rc = run_dropin(name, CP_h, I_h, nz, n, r, repetition, o, false, true, &res);
   if (rc) return rc;
@@ -194,6 +224,13 @@ int tdbu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int* sigma_h,
 }  // namespace
 
 extern "C" {
+
+int tbfs_dropin_mapping(int mapping) {
+  if (mapping != TBFS_DROPIN_FAST && mapping != TBFS_DROPIN_REFERENCE && mapping != -1)
+    return TBFS_ERR_INVALID;
+  if (mapping == -1) return g_mapping.load();
+  return g_mapping.exchange(mapping);
+}
 
 int bfs_gpu_td_csc_sc(int* IC_h, int* CP_h, int* S_h, float* sigma_h, int r,
                       int nz, int n, int repetition) {
@@ -236,10 +273,11 @@ int bfs_gpu_tdbu_csc_wa(int* IC_h, int* CP_h, int* S_h, int* sigma_h, int r,
 int bfs_gpu_td_cooc_sc(int* I_h, int* J_h, float* sigma_h, int r, int nz,
                        int n, int repetition) {
   if (!sigma_h) return TBFS_ERR_INVALID;
-  tbfs_opts o;
-  tbfs_opts_default(&o);
-  o.mode = TBFS_MODE_COOC;
-  o.loop = TBFS_LOOP_HOST;
+  tbfs_opts o = fast_opts(true);
+  if (!fast()) {
+    o.mode = TBFS_MODE_COOC;
+    o.loop = TBFS_LOOP_HOST;
+  }
   DropinResult res;
   int rc = run_dropin("bfsgputdcooc_sc", I_h, J_h, nz, n, r, repetition, o,
                       true, true, &res);

@@ -25,6 +25,8 @@ struct tbfs_graph {
   int32_t nbigc = 0;
   uint64_t* bigo = nullptr;  // static push slices of long out-lists (may alias bigc)
   int32_t nbigo = 0;
+  int32_t nheavy_in = 0;     // columns cut into bigc slices (in-degree > kBigIn)
+  int32_t nheavy_out = 0;    // vertices cut into bigo slices (out-degree > kBigIn)
   uint64_t* skip = nullptr;  // static: in-degree 0 or > kBigIn (one bit per vertex)
   uint64_t* bsl = nullptr;   // batched pull slices of the in-lists (v | slice << 32),
   int32_t nbsl = 0;          // built on first use
@@ -69,6 +71,14 @@ struct tbfs_ctx {
   tbfs::LoopState* loop = nullptr;  // [2] chained-loop state slots
   tbfs::LoopState* loop_h = nullptr;
   int chain_k = 8;                  // levels enqueued per host check
+  // shapes of the chained level launches, learned from the last call on
+  // pred_graph: bit min(d, 63) of pred_stage set when level d staged the
+  // frontier bitmap for some source (those launches get the staging
+  // workgroups); launches past pred_levels (the one finding the frontier
+  // empty, spares) get a small grid.  No history: every launch stages
+  const tbfs_graph* pred_graph = nullptr;
+  uint64_t pred_stage = ~0ull;
+  int pred_levels = 0;
   size_t fbl_limit = 0;             // bytes of LDS for a staged frontier bitmap
   int cus = 0;
   int32_t last_n = 0;
@@ -79,6 +89,8 @@ struct tbfs_ctx {
   int gK = 0;
   int ggrid = 0;
   size_t gfbl = 0;
+  uint64_t gstage = 0;
+  int glevels = 0;
   tbfs::PersistParams gparams{};
   tbfs::RunStats* gstats = nullptr;  // the chunk's stats slot (device)
   int32_t* groots_h = nullptr;       // pinned roots, one per queued source
@@ -118,6 +130,12 @@ inline int grid_for(int64_t items, int block, int cap) {
   if (g > cap) g = cap;
   return (int)g;
 }
+
+// Static kPullChunk-entry slices of every list of ptr[0..n] longer than
+// min_deg (entries v | slice << 32, in vertex order), built on stream s
+// (returns with the table complete, *out allocated with *count entries).
+int build_slices(const int32_t* ptr, int32_t n, int32_t min_deg, hipStream_t s, uint64_t** out,
+                 int32_t* count, int32_t* nlists = nullptr);
 
 // The graph a batch runs on: g itself, or its degree-ordered copy (order:
 // the batch_order option) with *perm mapping g's ids to the copy's (else

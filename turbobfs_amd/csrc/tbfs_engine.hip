@@ -400,21 +400,23 @@ int graph_ensure_cooc(tbfs_graph* g, hipStream_t s) {
   return TBFS_OK;
 }
 
-// Static slices (kPullChunk entries) of every list longer than kBigIn, from
+// Static slices (kPullChunk entries) of every list longer than min_deg, from
 // the device list offsets ptr: slices per vertex, their exclusive scan, then
 // the entries in vertex order (in-lists for the pull levels, out-lists for
-// the push levels).  Returns with the table complete.
-static int build_slices(const int32_t* ptr, int32_t n, hipStream_t s, uint64_t** out,
-                        int32_t* count) {
+// the push levels; min_deg 0 cuts every nonempty list, as batches use).
+// Returns with the table complete.
+int build_slices(const int32_t* ptr, int32_t n, int32_t min_deg, hipStream_t s, uint64_t** out,
+                 int32_t* count, int32_t* nlists) {
   int32_t *cnt = nullptr, *off = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
-  int32_t total = 0;
+  int32_t total = 0, lists = 0;
   hipError_t e = hipSuccess;
-  int rc = dev_alloc(&cnt, (size_t)n + 1);
+  int rc = dev_alloc(&cnt, (size_t)n + 2);  // cnt[n + 1]: lists cut
   if (!rc) rc = dev_alloc(&off, (size_t)n + 1);
   if (!rc) {
-    e = kern::slice_count(s, ptr, n, cnt);
+    e = hipMemsetAsync(cnt + n + 1, 0, sizeof(int32_t), s);
+    if (e == hipSuccess) e = kern::slice_count(s, ptr, n, min_deg, cnt, cnt + n + 1);
     if (e == hipSuccess)
       e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, off, n + 1, s);
     if (e == hipSuccess) rc = dev_alloc((char**)&tmp, std::max<size_t>(tmp_bytes, 1));
@@ -423,13 +425,18 @@ static int build_slices(const int32_t* ptr, int32_t n, hipStream_t s, uint64_t**
     e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, off, n + 1, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(&total, off + n, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&lists, cnt + n + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
   }
   if (!rc && e == hipSuccess) rc = dev_alloc(out, (size_t)total);
   if (!rc && e == hipSuccess && total > 0) e = kern::slice_fill(s, cnt, off, n, *out);
   if (!rc && e == hipSuccess) e = hipStreamSynchronize(s);
   if (!rc && e != hipSuccess) rc = hip_fail(e, "build_slices", __FILE__, __LINE__);
-  if (!rc) *count = total;
+  if (!rc) {
+    *count = total;
+    if (nlists) *nlists = lists;
+  }
   dev_free(cnt);
   dev_free(off);
   if (tmp) (void)hipFree(tmp);
@@ -440,7 +447,7 @@ static int build_slices(const int32_t* ptr, int32_t n, hipStream_t s, uint64_t**
 // light-column pull skips, built on the device from g->CP / g->RP (complete
 // on stream s or earlier).  Returns with every table complete.
 int graph_build_bigc(tbfs_graph* g, hipStream_t s) {
-  int rc = build_slices(g->CP, g->n, s, &g->bigc, &g->nbigc);
+  int rc = build_slices(g->CP, g->n, kBigIn, s, &g->bigc, &g->nbigc, &g->nheavy_in);
   if (rc) return rc;
   const int64_t nw = ((int64_t)g->n + 63) / 64;
   rc = dev_alloc(&g->skip, (size_t)nw);
@@ -450,9 +457,10 @@ int graph_build_bigc(tbfs_graph* g, hipStream_t s) {
   if (g->RP == g->CP) {
     g->bigo = g->bigc;
     g->nbigo = g->nbigc;
+    g->nheavy_out = g->nheavy_in;
     return TBFS_OK;
   }
-  return build_slices(g->RP, g->n, s, &g->bigo, &g->nbigo);
+  return build_slices(g->RP, g->n, kBigIn, s, &g->bigo, &g->nbigo, &g->nheavy_out);
 }
 
 // Guards the in-degree statistics kept in each graph (a server's dispatcher
@@ -781,6 +789,7 @@ static PersistParams make_params(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   p.stats = stats_slot;
   p.bigc = g->bigc;
   p.nbigc = g->nbigc;
+  p.nheavy = g->nheavy_in;
   p.bigo = g->bigo;
   p.nbigo = g->nbigo;
   p.root = root;
@@ -827,19 +836,53 @@ static size_t chain_fbl(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts&
 // level costs ~10 us instead of ~16 us on the full grid (any grid is correct).
 constexpr int kChainWideLevels = 16;
 
-static int level_grid(const tbfs_ctx* c, int grid, size_t fbl, int d) {
-  if (fbl || d < kChainWideLevels) return grid;
-  return std::min(grid, std::max(1, c->cus));
-}
+// Shape of one chained level launch: workgroups and dynamic LDS (0: the
+// kLbBlock-thread kernel without a staged bitmap).
+struct LevelShape {
+  int grid;
+  size_t fbl;
+};
 
-static int chain_grid(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o) {
-  if (chain_fbl(c, g, o)) return o.grid > 0 ? std::min(o.grid, 65536) : c->cus;
+// The launch shapes of a chained BFS on g: every shape gives the same
+// results; they differ in speed.  Staging workgroups (1024 threads, one per
+// CU, the bitmap in LDS) only for the levels that staged in the context's
+// last call on g; 256-thread workgroups, several per CU, for the others;
+// launches past the levels that call needed (the one that finds the frontier
+// empty, spare launches) on a small grid.  o.grid > 0 fixes the grid.
+struct LevelPlan {
+  int grid_stage = 0, grid_lb = 0, grid_spare = 0;
+  size_t fbl = 0;
+  uint64_t stage = ~0ull;
+  int levels = 0;
+  bool fixed = false;
+
+  LevelShape at(const tbfs_ctx* c, int d) const {
+    const bool st = fbl && ((stage >> (d < 63 ? d : 63)) & 1ull);
+    if (fixed) return {st ? grid_stage : grid_lb, st ? fbl : 0};
+    if (st) return {grid_stage, fbl};
+    if (levels > 0 && d >= levels) return {grid_spare, 0};
+    return {d < kChainWideLevels ? grid_lb : std::min(grid_lb, std::max(1, c->cus)), 0};
+  }
+};
+
+static LevelPlan level_plan(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o) {
+  LevelPlan pl;
+  pl.fbl = chain_fbl(c, g, o);
   const int64_t nwords = ((int64_t)g->n + 63) / 64;
   // about four frontier words per wavefront: fewer workgroups to dispatch
   // per level and fewer counter atomics, still >> 256 CUs' worth of waves
-  int grid = grid_for(nwords * 16, kLbBlock, 2048);
-  if (o.grid > 0) grid = std::min(o.grid, 65536);
-  return grid;
+  pl.grid_lb = grid_for(nwords * 16, kLbBlock, 2048);
+  pl.grid_stage = std::max(1, c->cus);
+  pl.grid_spare = std::max(1, c->cus / 8);
+  if (o.grid > 0) {
+    pl.fixed = true;
+    pl.grid_lb = pl.grid_stage = std::min(o.grid, 65536);
+  }
+  if (c->pred_graph == g) {
+    pl.stage = c->pred_stage;
+    pl.levels = c->pred_levels;
+  }
+  return pl;
 }
 
 // Chained loop, enqueue only: init plus `levels` level kernels.  The BFS has
@@ -847,12 +890,13 @@ static int chain_grid(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o
 static int enqueue_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
                            const tbfs_opts& o, RunStats* stats_slot, int levels) {
   const PersistParams p = make_params(c, g, root, o, stats_slot);
-  const int grid = chain_grid(c, g, o);
-  const size_t fbl = chain_fbl(c, g, o);
+  const LevelPlan pl = level_plan(c, g, o);
   hipStream_t s = c->stream;
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
-  for (int d = 0; d < levels; ++d)
-    HIPCHK(kern::chain_level(s, o.grid > 0 ? grid : level_grid(c, grid, fbl, d), p, d, fbl));
+  for (int d = 0; d < levels; ++d) {
+    const LevelShape sh = pl.at(c, d);
+    HIPCHK(kern::chain_level(s, sh.grid, p, d, sh.fbl));
+  }
   return TBFS_OK;
 }
 
@@ -861,8 +905,8 @@ static int enqueue_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
 // stream drained.
 static int run_chained_p(tbfs_ctx* c, const tbfs_graph* g, const PersistParams& p,
                          const tbfs_opts& o) {
-  const int grid = chain_grid(c, g, o);
-  const size_t fbl = chain_fbl(c, g, o);
+  LevelPlan pl = level_plan(c, g, o);
+  pl.levels = 0;  // a run the host checks: no launch is taken for a spare
   hipStream_t s = c->stream;
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
   if (p.nroots > 0) HIPCHK(kern::chain_seed(s, p));
@@ -872,9 +916,10 @@ static int run_chained_p(tbfs_ctx* c, const tbfs_graph* g, const PersistParams& 
   int32_t d = 0;
   int chunk = std::max(c->chain_k, 8);
   for (;;) {
-    for (int i = 0; i < chunk; ++i)
-      HIPCHK(kern::chain_level(s, o.grid > 0 ? grid : level_grid(c, grid, fbl, d + i), p, d + i,
-                               fbl));
+    for (int i = 0; i < chunk; ++i) {
+      const LevelShape sh = pl.at(c, d + i);
+      HIPCHK(kern::chain_level(s, sh.grid, p, d + i, sh.fbl));
+    }
     d += chunk;
     chunk = std::min(chunk + chunk / 2, 256);
     HIPCHK(hipMemcpyAsync(c->loop_h, c->loop + (d & 1), sizeof(LoopState),
@@ -1053,9 +1098,11 @@ static int graph_chunk(tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o, int
   PersistParams p = make_params(c, g, -1, o, c->gstats);
   p.nroots = 1;
   p.roots = c->roots_d;
-  const int grid = chain_grid(c, g, o);
-  const size_t fbl = chain_fbl(c, g, o);
+  const LevelPlan pl = level_plan(c, g, o);
+  const int grid = pl.at(c, 0).grid;
+  const size_t fbl = pl.fbl;
   if (c->gexec && c->gK == K && c->ggrid == grid && c->gfbl == fbl &&
+      c->gstage == pl.stage && c->glevels == pl.levels &&
       std::memcmp(&p, &c->gparams, sizeof(p)) == 0)
     return TBFS_OK;
   if (c->gexec) {
@@ -1066,8 +1113,10 @@ static int graph_chunk(tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o, int
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   hipError_t e = kern::chain_init(s, grid_for(g->n, 256, 8192), p);
   if (e == hipSuccess) e = kern::chain_seed(s, p);
-  for (int i = 0; i < K && e == hipSuccess; ++i)
-    e = kern::chain_level(s, o.grid > 0 ? grid : level_grid(c, grid, fbl, i), p, i, fbl);
+  for (int i = 0; i < K && e == hipSuccess; ++i) {
+    const LevelShape sh = pl.at(c, i);
+    e = kern::chain_level(s, sh.grid, p, i, sh.fbl);
+  }
   hipGraph_t gr = nullptr;
   const hipError_t e2 = hipStreamEndCapture(s, &gr);
   if (e != hipSuccess || e2 != hipSuccess) {
@@ -1083,6 +1132,8 @@ static int graph_chunk(tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o, int
   c->gK = K;
   c->ggrid = grid;
   c->gfbl = fbl;
+  c->gstage = pl.stage;
+  c->glevels = pl.levels;
   c->gparams = p;
   return TBFS_OK;
 }
@@ -1191,6 +1242,16 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
     // spare, so a source needing one launch more than any seen here still
     // finishes without a re-run (a spare launch costs ~4 us, a re-run a BFS)
     c->chain_k = std::min(4096, std::max(2, maxl + 1));
+    // the next call's launch shapes: the levels that staged and the depth
+    uint64_t stage = 0;
+    int depth = 0;
+    for (int i = 0; i < count; ++i) {
+      stage |= c->stats_h[i].stage_mask;
+      depth = std::max(depth, (int)c->stats_h[i].levels);
+    }
+    c->pred_graph = g;
+    c->pred_stage = stage;
+    c->pred_levels = depth;
   }
   c->last_n = g->n;
   if (st) {

@@ -46,8 +46,10 @@ struct alignas(64) LevelCtr {
   unsigned long long mout;  // sum of their out-degrees (push work next level)
   unsigned long long min;   // sum of their in-degrees (leaves the unexplored set)
   unsigned int ovf;         // 1 if any of them has a count >= 2^64
+  unsigned int hin;         // how many of them have in-degree > kBigIn
+  unsigned int hout;        // how many of them have out-degree > kBigIn
   unsigned int pad0;
-  unsigned long long pad1[3];
+  unsigned long long pad1[2];
 };
 
 // Level counters of the load-balanced kernels, striped over kStripes cache
@@ -60,9 +62,10 @@ constexpr int kStripes = 16;
 struct alignas(128) CtrStripe {
   unsigned long long qm;
   unsigned long long min;
+  unsigned long long hubs;  // hout << 32 | hin (see LevelCtr), both < 2^31
   unsigned int ovf;
   unsigned int pad0;
-  unsigned long long pad1[13];
+  unsigned long long pad1[12];
 };
 struct CtrSet {
   CtrStripe s[kStripes];
@@ -78,7 +81,9 @@ struct alignas(64) RunStats {
   unsigned long long edges;
   int32_t launches;  // chained loop: level kernels used, the last one included
   int32_t pad0;
-  unsigned long long pad[3];
+  unsigned long long stage_mask;  // chained loop: bit min(d, 63) set when level d
+                                  // was a pull that stages the frontier bitmap
+  unsigned long long pad[2];
 };
 
 struct GraphView {
@@ -117,7 +122,8 @@ struct alignas(64) LoopState {
   unsigned long long mu;       // in-list entries of the unreached vertices
   unsigned long long reached;  // vertices reached before this level
   unsigned long long edges;    // their in-list entries
-  unsigned long long pad[2];
+  unsigned long long stage_mask;  // levels so far that were bitmap-staging pulls
+  unsigned long long heavy_reached;  // reached vertices with in-degree > kBigIn
 };
 
 // Everything the device-resident level loops need.
@@ -134,6 +140,7 @@ struct PersistParams {
   RunStats* stats;
   const uint64_t* bigc;
   int32_t nbigc;
+  int32_t nheavy;   // columns in bigc (in-degree > kBigIn)
   const uint64_t* bigo;
   int32_t nbigo;
   int32_t root;     // single root; -1 when the roots come from `roots`
@@ -187,7 +194,10 @@ hipError_t copy16(hipStream_t s, const void* src, void* dst, int64_t n16);
 hipError_t expand_cols(hipStream_t s, const int32_t* CP, int32_t n,
                        int32_t* col);
 // slice tables and skip bitmap of a graph, from device list offsets ptr
-hipError_t slice_count(hipStream_t s, const int32_t* ptr, int32_t n, int32_t* cnt);
+// (lists longer than min_deg, cut into kPullChunk-entry slices; *nlists, zero
+// on entry, receives the number of lists cut)
+hipError_t slice_count(hipStream_t s, const int32_t* ptr, int32_t n, int32_t min_deg,
+                       int32_t* cnt, int32_t* nlists);
 hipError_t slice_fill(hipStream_t s, const int32_t* cnt, const int32_t* off, int32_t n,
                       uint64_t* out);
 hipError_t skip_bits(hipStream_t s, const int32_t* ptr, int32_t n, uint64_t* skip,

@@ -72,6 +72,7 @@ __global__ void k_level_prologue(uint64_t* fnext, int32_t nwords,
   if (tid < kStripes) {
     cset->s[tid].qm = 0;
     cset->s[tid].min = 0;
+    cset->s[tid].hubs = 0;
     cset->s[tid].ovf = 0;
   }
 }
@@ -302,15 +303,18 @@ __global__ void k_expand_cols(const int32_t* CP, int32_t n, int32_t* col) {
 // vertex then slice order): slices per vertex (cnt[n] = 0, so an exclusive
 // scan of n + 1 counts ends with the total), then the entries at the scanned
 // offsets.
-__global__ void k_slice_count(const int32_t* ptr, int32_t n, int32_t* cnt) {
+__global__ void k_slice_count(const int32_t* ptr, int32_t n, int32_t min_deg, int32_t* cnt,
+                              int32_t* nlists) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v > n) return;
   int32_t c = 0;
   if (v < n) {
     const int64_t deg = (int64_t)ptr[v + 1] - ptr[v];
-    if (deg > kBigIn) c = (int32_t)((deg + kPullChunk - 1) / kPullChunk);
+    if (deg > min_deg) c = (int32_t)((deg + kPullChunk - 1) / kPullChunk);
   }
-  cnt[v] = c;
+  if (v <= n) cnt[v] = c;
+  // lists cut: one atomic per wavefront
+  const uint64_t m = __ballot(c > 0);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(nlists, (int32_t)__popcll(m));
 }
 
 __global__ void k_slice_fill(const int32_t* cnt, const int32_t* off, int32_t n,
@@ -540,9 +544,10 @@ hipError_t expand_cols(hipStream_t s, const int32_t* CP, int32_t n,
   return hipGetLastError();
 }
 
-hipError_t slice_count(hipStream_t s, const int32_t* ptr, int32_t n, int32_t* cnt) {
+hipError_t slice_count(hipStream_t s, const int32_t* ptr, int32_t n, int32_t min_deg,
+                        int32_t* cnt, int32_t* nlists) {
   const int grid = (int)(((int64_t)n + 1 + 255) / 256);
-  hipLaunchKernelGGL(k_slice_count, dim3(grid), dim3(256), 0, s, ptr, n, cnt);
+  hipLaunchKernelGGL(k_slice_count, dim3(grid), dim3(256), 0, s, ptr, n, min_deg, cnt, nlists);
   return hipGetLastError();
 }
 

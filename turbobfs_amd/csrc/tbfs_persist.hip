@@ -49,9 +49,12 @@ struct WaveLds {
 };
 
 // Per-lane tallies of the vertices this lane discovered in the current level.
+// Also counted: the vertices with lists cut into static slices (in- and
+// out-degree above kBigIn), so that levels can skip the slice tables when no
+// such vertex is open (pull) or in the frontier (push).
 struct LevelAcc {
   uint64_t q = 0, mout = 0, min = 0;
-  uint32_t ovf = 0;
+  uint32_t ovf = 0, hin = 0, hout = 0;
 
   __device__ __forceinline__ void found(const GraphView& g, int v) {
     found_in(g, v, indeg(g, v));
@@ -59,36 +62,48 @@ struct LevelAcc {
   // v's in-degree already known: on a symmetric graph (out-lists alias the
   // in-lists) nothing more is loaded
   __device__ __forceinline__ void found_in(const GraphView& g, int v, uint32_t di) {
+    const uint32_t dout = (g.RP == g.CP) ? di : outdeg(g, v);
     q += 1;
     min += di;
-    mout += (g.RP == g.CP) ? di : outdeg(g, v);
+    mout += dout;
+    hin += di > (uint32_t)kBigIn ? 1u : 0u;
+    hout += dout > (uint32_t)kBigIn ? 1u : 0u;
   }
 };
+
+// The hub counts of one vertex v in CtrStripe::hubs form.
+__device__ __forceinline__ unsigned long long hub_bits(const GraphView& g, int v) {
+  return ((unsigned long long)(outdeg(g, v) > (uint32_t)kBigIn ? 1u : 0u) << 32) |
+         (indeg(g, v) > (uint32_t)kBigIn ? 1u : 0u);
+}
 
 // Sum the tallies of the workgroup and add them to its stripe of the level
 // counters (two 64-bit atomics).  Every thread of the workgroup must call it.
 __device__ __forceinline__ void flush_level(CtrSet* set, LevelAcc& a) {
-  __shared__ unsigned long long red[kMaxBlock / 64][3];
+  __shared__ unsigned long long red[kMaxBlock / 64][4];
   __shared__ unsigned int redo[kMaxBlock / 64];
   const uint64_t q = wave_sum_u64(a.q);
   const uint64_t mo = wave_sum_u64(a.mout);
   const uint64_t mi = wave_sum_u64(a.min);
+  const uint64_t hb = wave_sum_u64(((uint64_t)a.hout << 32) | a.hin);
   const bool ov = __ballot(a.ovf != 0) != 0;
   const int wv = threadIdx.x >> 6;
   if (lane() == 0) {
     red[wv][0] = q;
     red[wv][1] = mo;
     red[wv][2] = mi;
+    red[wv][3] = hb;
     redo[wv] = ov ? 1u : 0u;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long t0 = 0, t1 = 0, t2 = 0;
+    unsigned long long t0 = 0, t1 = 0, t2 = 0, t3 = 0;
     unsigned int to = 0;
     for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
       t0 += red[i][0];
       t1 += red[i][1];
       t2 += red[i][2];
+      t3 += red[i][3];
       to |= redo[i];
     }
     CtrStripe* st = &set->s[blockIdx.x & (kStripes - 1)];
@@ -96,6 +111,7 @@ __device__ __forceinline__ void flush_level(CtrSet* set, LevelAcc& a) {
       atomicAdd(&st->qm, (t0 << 32) + t1);
       atomicAdd(&st->min, t2);
     }
+    if (t3) atomicAdd(&st->hubs, t3);
     if (to) atomicOr(&st->ovf, 1u);
   }
   __syncthreads();
@@ -106,7 +122,7 @@ __device__ __forceinline__ void flush_level(CtrSet* set, LevelAcc& a) {
 // stripe each).  `agent`: the loads bypass caches that may hold values from
 // before the last grid barrier.
 struct StripeVals {
-  uint64_t qm = 0, mi = 0;
+  uint64_t qm = 0, mi = 0, hb = 0;
   uint32_t ov = 0;
 };
 __device__ __forceinline__ StripeVals load_set(const CtrSet* set, bool agent) {
@@ -117,10 +133,12 @@ __device__ __forceinline__ StripeVals load_set(const CtrSet* set, bool agent) {
     if (agent) {
       r.qm = __hip_atomic_load(&st->qm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       r.mi = __hip_atomic_load(&st->min, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      r.hb = __hip_atomic_load(&st->hubs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       r.ov = __hip_atomic_load(&st->ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       r.qm = st->qm;
       r.mi = st->min;
+      r.hb = st->hubs;
       r.ov = st->ovf;
     }
   }
@@ -134,6 +152,9 @@ __device__ __forceinline__ LevelCtr sum_set(const StripeVals& r) {
   c.qe = 0;
   c.mout = wave_sum_u64(r.qm & 0xffffffffull);
   c.min = wave_sum_u64(r.mi);
+  const uint64_t hb = wave_sum_u64(r.hb);
+  c.hin = (uint32_t)(hb & 0xffffffffull);
+  c.hout = (uint32_t)(hb >> 32);
   c.ovf = __ballot(r.ov != 0) ? 1u : 0u;
   return c;
 }
@@ -142,31 +163,71 @@ __device__ __forceinline__ LevelCtr read_set(const CtrSet* set, bool agent) {
   return sum_set(load_set(set, agent));
 }
 
-// One edge u -> v of a push level, u at depth d with count x (overflow ou).
-// "Reached at depth <= d" is read from the reached bitmap or the current
-// frontier (both small and cached); otherwise v is reached at depth d+1 and
-// the count is added at once.  A vertex not reached before this level holds
-// count 0, so only an adder that finds 0 can be the first: those (normally
-// one per vertex; more only when a count is 0 mod 2^64) claim v's bit in the
-// next frontier, and the one that sets it records the depth.  Device-scope
-// atomics go out to memory, so the second one is skipped where it can be.
-__device__ __forceinline__ void push_edge(const GraphView& g, const StateView& st,
-                                          int dn, int v, uint64_t x, bool ou,
-                                          LevelAcc& a) {
-  const uint64_t bit = 1ull << (v & 63);
-  if ((st.vis[v >> 6] | st.fcur[v >> 6]) & bit) return;
-  const unsigned long long old =
-      atomicAdd((unsigned long long*)&st.sigma[v], (unsigned long long)x);
-  if (old + x < old || ou) {
-    st.ovf[v] = 1;
-    a.ovf = 1;
+// R edges u -> v[t] of a push level (v[t] < 0: no edge), u at depth d with
+// count x[t] (overflow ou[t]).  "Reached at depth <= d" is read from the
+// reached bitmap or the current frontier (both small and cached); otherwise
+// v is reached at depth d+1 and the count is added at once.  A vertex not
+// reached before this level holds count 0, so only an adder that finds 0 can
+// be the first: those (normally one per vertex; more only when a count is 0
+// mod 2^64) claim v's bit in the next frontier, and the one that sets it
+// records the depth.  Device-scope atomics are performed at memory, a round
+// trip of a microsecond or more, so the R edges go through each step
+// together: all bitmap loads, then all count additions (with the targets'
+// in-degrees loaded alongside), then all claims -- three round trips for the
+// lane's R edges instead of four per edge.  Lanes without an edge read word 0
+// of the bitmaps and vertex 0's degree (no load behind a branch) and are
+// masked out.
+template <int R>
+__device__ __forceinline__ void push_edges(const GraphView& g, const StateView& st, int dn,
+                                           const int (&v)[R], const uint64_t (&x)[R],
+                                           const bool (&ou)[R], LevelAcc& a) {
+  uint64_t bw[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int w = v[t] < 0 ? 0 : (v[t] >> 6);
+    bw[t] = st.vis[w] | st.fcur[w];
   }
-  if (old == 0) {
-    const unsigned long long oldb =
-        atomicOr((unsigned long long*)&st.fnext[v >> 6], (unsigned long long)bit);
-    if ((oldb & bit) == 0) {
-      st.S[v] = dn;
-      a.found(g, v);
+  bool live[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) live[t] = v[t] >= 0 && !((bw[t] >> (v[t] & 63)) & 1ull);
+  unsigned long long old[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    old[t] = 1;
+    if (live[t])
+      old[t] = atomicAdd((unsigned long long*)&st.sigma[v[t]], (unsigned long long)x[t]);
+  }
+  // in-degrees of every target still open, in flight with the additions
+  // (only the claimed ones are used): one round trip less per level
+  int32_t c0[R], c1[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int vv = live[t] ? v[t] : 0;
+    c0[t] = g.CP[vv];
+    c1[t] = g.CP[vv + 1];
+  }
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    if (live[t] && (old[t] + x[t] < old[t] || ou[t])) {
+      st.ovf[v[t]] = 1;
+      a.ovf = 1;
+    }
+  }
+  bool first[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) first[t] = live[t] && old[t] == 0;
+  unsigned long long ob[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    ob[t] = ~0ull;
+    if (first[t])
+      ob[t] = atomicOr((unsigned long long*)&st.fnext[v[t] >> 6], 1ull << (v[t] & 63));
+  }
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    if (!((ob[t] >> (v[t] & 63)) & 1ull)) {
+      st.S[v[t]] = dn;
+      a.found_in(g, v[t], (uint32_t)(c1[t] - c0[t]));
     }
   }
 }
@@ -205,13 +266,20 @@ __device__ __forceinline__ int select_bit(uint64_t x, int r) {
   return pos;
 }
 
+// fclr (device-resident loops; null for the host-driven kernels, whose
+// prologue kernel does it): the level's housekeeping rides on the frontier
+// word loads -- each word of the frontier bitmap due for clearing is zeroed
+// and the current frontier is ORed into the reached bitmap as its words are
+// read.  Every word belongs to exactly one wave's share, and readers of the
+// reached bitmap in this level test it together with the current frontier,
+// so the order of the OR does not matter.
 __device__ __forceinline__ void push_lb_level(const GraphView& g,
                                               const StateView& st, int dn,
                                               uint32_t ovf_any,
                                               const uint64_t* bigo,
                                               int32_t nbigo, int64_t wave,
                                               int64_t nwaves, WaveLds* L,
-                                              LevelAcc& a) {
+                                              LevelAcc& a, uint64_t* fclr) {
   // this wave's share: entries wave, wave + nwaves, ...; 64 examined at once.
   // Each lane loads the bounds and count of its own entry; the wavefront then
   // takes the active slices in turn, kSliceRegs out-list entries per lane, all
@@ -243,14 +311,16 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
       const uint64_t sx = __shfl(x, j);
       const bool sou = __shfl(ou, j) != 0;
       int v[kSliceRegs];
+      uint64_t xs[kSliceRegs];
+      bool os[kSliceRegs];
 #pragma unroll
       for (int t = 0; t < kSliceRegs; ++t) {
         const int k = scb + lane() + 64 * t;
         v[t] = k < sce ? g.JR[k] : -1;
+        xs[t] = sx;
+        os[t] = sou;
       }
-#pragma unroll
-      for (int t = 0; t < kSliceRegs; ++t)
-        if (v[t] >= 0) push_edge(g, st, dn, v[t], sx, sou, a);
+      push_edges<kSliceRegs>(g, st, dn, v, xs, os, a);
     }
   }
   const int64_t nwords = ((int64_t)g.n + 63) >> 6;
@@ -261,7 +331,13 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
   const int64_t nm = nmine_of(nwords, wave, nwaves);
   for (int64_t b = 0; b < nm; b += 64) {
     const int64_t wl = wave + (b + lane()) * nwaves;
-    const uint64_t bits_l = (b + lane() < nm) ? st.fcur[wl] : 0ull;
+    const bool mine = b + lane() < nm;
+    const uint64_t bits_l = mine ? st.fcur[wl] : 0ull;
+    if (fclr && mine) {
+      const uint64_t vw = st.vis[wl];
+      fclr[wl] = 0ull;
+      if (bits_l) st.vis[wl] = vw | bits_l;
+    }
     const uint32_t cnt = (uint32_t)__popcll(bits_l);
     const uint32_t wincl = wave_incl_scan(cnt);
     const int nv = (int)__shfl(wincl, 63);
@@ -282,13 +358,15 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
       int beg = 0, deg = 0;
       uint64_t x = 0;
       uint32_t ou = 0;
-      if (u >= 0) {
+      if (u >= 0) {  // bounds and count of u loaded together
         beg = g.RP[u];
-        const int dg = g.RP[u + 1] - beg;
-        if (dg <= kBigIn) {
-          deg = dg;
-          x = st.sigma[u];
-          if (ovf_any) ou = st.ovf[u];
+        const int end = g.RP[u + 1];
+        const uint64_t xu = st.sigma[u];
+        const uint32_t ouu = ovf_any ? st.ovf[u] : 0u;
+        if (end - beg <= kBigIn) {  // longer lists go by their static slices
+          deg = end - beg;
+          x = xu;
+          ou = ouu;
         }
       }
       const uint32_t incl = wave_incl_scan((uint32_t)deg);
@@ -301,20 +379,22 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
       L->x[lane()] = x;
       wave_lds_sync();
       for (int e0 = lane(); e0 < total; e0 += 64 * kUnroll) {
-        int v[kUnroll], o[kUnroll];
+        int v[kUnroll];
+        uint64_t xs[kUnroll];
+        bool os[kUnroll];
 #pragma unroll
         for (int t = 0; t < kUnroll; ++t) {
           const int e = e0 + 64 * t;
-          o[t] = -1;
-          if (e < total) {
-            o[t] = owner_of(L->excl, e);
-            v[t] = g.JR[L->beg[o[t]] + (e - L->excl[o[t]])];
-          }
+          const bool ok = e < total;
+          const int o = owner_of(L->excl, ok ? e : 0);
+          // lanes past the end read entry 0 (total > 0, so nz > 0)
+          const int k = ok ? L->beg[o] + (e - L->excl[o]) : 0;
+          const int vk = g.JR[k];
+          v[t] = ok ? vk : -1;
+          xs[t] = L->x[o];
+          os[t] = (oumask >> o) & 1ull;
         }
-#pragma unroll
-        for (int t = 0; t < kUnroll; ++t)
-          if (o[t] >= 0)
-            push_edge(g, st, dn, v[t], L->x[o[t]], (oumask >> o[t]) & 1ull, a);
+        push_edges<kUnroll>(g, st, dn, v, xs, os, a);
       }
     }
   }
@@ -371,6 +451,92 @@ __device__ __forceinline__ void pull_light_pass(const GraphView& g, const StateV
   }
 }
 
+// One group of up to 64 columns of a pull level, column v with in-list
+// [beg, beg + deg) on its lane (deg 0: none): the in-lists are walked in
+// passes of 4 or 8 entries per lane (parents' counts summed per column in
+// LDS), and the columns that found a parent get their depth, count and
+// overflow flag and are tallied.  Returns the wavefront's mask of those
+// columns (lane bits).  All lanes must take part.
+template <bool SIGMA>
+__device__ __forceinline__ uint64_t pull_group(const GraphView& g, const StateView& st,
+                                               WaveLds* L, const uint64_t* fpar,
+                                               uint32_t ovf_any, int dn, int v, int beg,
+                                               int deg, LevelAcc& a) {
+  const uint32_t incl = wave_incl_scan((uint32_t)deg);
+  const int total = (int)__shfl(incl, 63);
+  if (total == 0) return 0;
+  wave_lds_sync();
+  L->excl[lane()] = (int)incl - deg;
+  L->beg[lane()] = beg;
+  if (SIGMA) {
+    L->x[lane()] = 0;
+    L->xh[lane()] = 0;
+  }
+  wave_lds_sync();
+  uint64_t fmask = 0, omask = 0;
+  // one pass of 4 entries per lane for short groups, else passes of 8
+  if (total <= 64 * 4) {
+    pull_light_pass<4, SIGMA>(g, st, L, fpar, ovf_any, lane(), total, fmask, omask);
+  } else {
+    for (int e0 = lane(); e0 < total; e0 += 64 * kPullRegs)
+      pull_light_pass<kPullRegs, SIGMA>(g, st, L, fpar, ovf_any, e0, total, fmask, omask);
+  }
+  fmask = wave_or_u64(fmask);
+  if (fmask == 0) return 0;
+  if (SIGMA) omask = wave_or_u64(omask);
+  wave_lds_sync();
+  if ((fmask >> lane()) & 1ull) {
+    st.S[v] = dn;
+    if (SIGMA) {
+      // count = xh * 2^32 + x; at least 2^64 iff xh >= 2^32 or the sum
+      // of the low 64 bits carries
+      const uint64_t xh = L->xh[lane()];
+      const uint64_t hs = xh << 32;
+      const uint64_t sum = hs + L->x[lane()];
+      st.sigma[v] = sum;
+      if ((xh >> 32) != 0 || sum < hs || ((omask >> lane()) & 1ull)) {
+        st.ovf[v] = 1;
+        a.ovf = 1;
+      }
+    }
+    a.found_in(g, v, (uint32_t)deg);
+  }
+  return fmask;
+}
+
+// The next kSlicesPerPass active slices of a pull level's slice batch (lane
+// bits of todo, taken lowest first; js[h] = -1 past the last): their lane
+// and entry range [scb, sce).  A slot without a slice repeats the first
+// slot's start with an empty range.
+__device__ __forceinline__ void pick_slices(uint64_t& todo, int cb, int ce,
+                                            int (&js)[kSlicesPerPass],
+                                            int (&scb)[kSlicesPerPass],
+                                            int (&sce)[kSlicesPerPass]) {
+#pragma unroll
+  for (int h = 0; h < kSlicesPerPass; ++h) {
+    js[h] = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
+    todo &= todo - 1;
+    const int jj = js[h] >= 0 ? js[h] : (js[0] >= 0 ? js[0] : 0);
+    scb[h] = __shfl(cb, jj);
+    sce[h] = js[h] >= 0 ? __shfl(ce, jj) : scb[h];
+  }
+}
+
+// In-list entries of the picked slices, kSliceRegs per lane of each; entries
+// past a slice's end read the first slice's first entry (masked later).
+__device__ __forceinline__ void load_slices(const GraphView& g, const int (&scb)[kSlicesPerPass],
+                                            const int (&sce)[kSlicesPerPass],
+                                            int (&u)[kSlicesPerPass * kSliceRegs]) {
+#pragma unroll
+  for (int h = 0; h < kSlicesPerPass; ++h) {
+#pragma unroll
+    for (int t = 0; t < kSliceRegs; ++t) {
+      const int k = scb[h] + lane() + 64 * t;
+      u[h * kSliceRegs + t] = g.IC[k < sce[h] ? k : scb[0]];
+    }
+  }
+}
+
 // Pull level: (a) static slices of the high in-degree columns, reduced with
 // global atomics; (b) the other unreached columns: words are dealt to
 // wavefronts round-robin; a wavefront loads 64 of its words of the
@@ -385,7 +551,7 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
                                               int32_t nbigc, int64_t wave,
                                               int64_t nwaves, WaveLds* L,
                                               LevelAcc& a,
-                                              const uint64_t* fpar) {
+                                              const uint64_t* fpar, uint64_t* fclr) {
   // Slices: each lane loads the bounds of its own entry; the wavefront takes
   // the active slices in turn (kSliceRegs in-list entries per lane, loads in
   // flight together) and the lane owning a slice keeps its sum; the atomics
@@ -410,31 +576,32 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
     uint64_t mine_sum = 0;
     uint32_t mine_ov = 0;
     // kSlicesPerPass active slices per pass (kSliceRegs entries per lane of
-    // each); a slot without a slice repeats the first one's start, empty
-    while (todo) {
-      int js[kSlicesPerPass], scb[kSlicesPerPass], sce[kSlicesPerPass];
-#pragma unroll
-      for (int h = 0; h < kSlicesPerPass; ++h) {
-        js[h] = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
-        todo &= todo - 1;
-        const int jj = js[h] >= 0 ? js[h] : js[0];
-        scb[h] = __shfl(cb, jj);
-        sce[h] = js[h] >= 0 ? __shfl(ce, jj) : scb[h];
-      }
-      constexpr int R = kSlicesPerPass * kSliceRegs;
-      int u[R];
+    // each); a slot without a slice repeats the first one's start, empty.
+    // The next pass's in-list entries are loaded as soon as this pass's
+    // count gathers are issued, so the two latencies overlap.
+    constexpr int R = kSlicesPerPass * kSliceRegs;
+    int js[kSlicesPerPass], scb[kSlicesPerPass], sce[kSlicesPerPass];
+    int u[R];
+    pick_slices(todo, cb, ce, js, scb, sce);
+    if (js[0] >= 0) load_slices(g, scb, sce, u);
+    while (js[0] >= 0) {
       bool par[R];
 #pragma unroll
-      for (int h = 0; h < kSlicesPerPass; ++h) {
+      for (int h = 0; h < kSlicesPerPass; ++h)
 #pragma unroll
-        for (int t = 0; t < kSliceRegs; ++t) {
-          const int k = scb[h] + lane() + 64 * t;
-          u[h * kSliceRegs + t] = g.IC[k < sce[h] ? k : scb[0]];
-          par[h * kSliceRegs + t] = k < sce[h];
-        }
-      }
+        for (int t = 0; t < kSliceRegs; ++t)
+          par[h * kSliceRegs + t] = scb[h] + lane() + 64 * t < sce[h];
 #pragma unroll
       for (int t = 0; t < R; ++t) par[t] = par[t] && fbit(fpar, u[t]);
+      uint64_t xs[R];
+      if (SIGMA) {
+#pragma unroll
+        for (int t = 0; t < R; ++t) xs[t] = st.sigma[par[t] ? u[t] : 0];  // no load behind a branch
+      }
+      int jn[kSlicesPerPass], scbn[kSlicesPerPass], scen[kSlicesPerPass];
+      int un[R];
+      pick_slices(todo, cb, ce, jn, scbn, scen);
+      if (jn[0] >= 0) load_slices(g, scbn, scen, un);
       bool f[kSlicesPerPass];
       uint64_t sum[kSlicesPerPass];
       uint32_t ov[kSlicesPerPass];
@@ -445,17 +612,11 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
         ov[h] = 0;
       }
       if (SIGMA) {
-        uint64_t xs[R];
-#pragma unroll
-        for (int t = 0; t < R; ++t) {
-          const uint64_t xv = st.sigma[par[t] ? u[t] : 0];  // no load behind a branch
-          xs[t] = par[t] ? xv : 0ull;
-        }
 #pragma unroll
         for (int t = 0; t < R; ++t) {
           const int h = t / kSliceRegs;
           f[h] |= par[t];
-          add_carry(sum[h], ov[h], xs[t]);
+          add_carry(sum[h], ov[h], par[t] ? xs[t] : 0ull);
           if (ovf_any && par[t]) ov[h] |= (uint32_t)st.ovf[u[t]];
         }
       } else {
@@ -473,6 +634,14 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
           }
         }
       }
+#pragma unroll
+      for (int h = 0; h < kSlicesPerPass; ++h) {
+        js[h] = jn[h];
+        scb[h] = scbn[h];
+        sce[h] = scen[h];
+      }
+#pragma unroll
+      for (int t = 0; t < R; ++t) u[t] = un[t];
     }
     if (mine_f) {
       if (SIGMA) {
@@ -498,14 +667,62 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
   for (int64_t b = 0; b < nmine; b += 64) {
     const int64_t wl = wave + (b + lane()) * nwaves;
     // reached (any frontier so far, the current one included) or not a
-    // light column: nothing to do for that vertex
-    const uint64_t done_l =
-        (b + lane() < nmine) ? (st.vis[wl] | st.skip[wl] | st.fcur[wl]) : ~0ull;
+    // light column: nothing to do for that vertex.  The housekeeping of
+    // these words rides along (as in push_lb_level).
+    uint64_t done_l = ~0ull;
+    if (b + lane() < nmine) {
+      const uint64_t vw = st.vis[wl], fw = st.fcur[wl];
+      done_l = vw | st.skip[wl] | fw;
+      if (fclr) {
+        fclr[wl] = 0ull;
+        if (fw) st.vis[wl] = vw | fw;
+      }
+    }
     uint64_t todo = __ballot(done_l != ~0ull);
+    if (!todo) continue;
+    const uint64_t open_l = ~done_l;
+    const uint32_t cnt = (uint32_t)__popcll(open_l);
+    const uint32_t wincl = wave_incl_scan(cnt);
+    const int ncol = (int)__shfl(wincl, 63);
+    if (2 * ncol < 64 * __popcll(todo)) {
+      // Sparse words (under half their columns open on average): the open
+      // columns of the 64 words are numbered across the words (prefix sum
+      // of popcounts) and taken 64 at a time, so a few open columns per word
+      // cost one pass for many words instead of one per word.  Columns found
+      // are gathered per word in LDS: one next-frontier OR per word.
+      uint64_t wmask = 0;  // this lane's word: columns found
+      for (int k0 = 0; k0 < ncol; k0 += 64) {
+        wave_lds_sync();
+        L->excl[lane()] = (int)(wincl - cnt);
+        L->x[lane()] = open_l;
+        wave_lds_sync();
+        int v = -1, beg = 0, deg = 0, j = 0;
+        const int k = k0 + lane();
+        if (k < ncol) {
+          j = owner_of(L->excl, k);
+          const int bit = select_bit(L->x[j], k - L->excl[j]);
+          v = (int)((wave + (b + j) * nwaves) * 64) + bit;
+          beg = g.CP[v];
+          deg = g.CP[v + 1] - beg;
+        }
+        const uint64_t fm = pull_group<SIGMA>(g, st, L, fpar, ovf_any, dn, v, beg, deg, a);
+        if (fm) {
+          wave_lds_sync();
+          L->x[lane()] = 0;
+          wave_lds_sync();
+          if ((fm >> lane()) & 1ull)
+            atomicOr((unsigned long long*)&L->x[j], (unsigned long long)(1ull << (v & 63)));
+          wave_lds_sync();
+          wmask |= L->x[lane()];
+        }
+      }
+      if (wmask) atomicOr((unsigned long long*)&st.fnext[wl], (unsigned long long)wmask);
+      continue;
+    }
     // column bounds of the next word with work are loaded one word ahead:
     // their latency hides behind the current word's passes
     int nbeg = 0, nend = 0;
-    if (todo) {
+    {
       const int j = __ffsll((unsigned long long)todo) - 1;
       const int64_t w = wave + (b + j) * nwaves;
       if (!((__shfl(done_l, j) >> lane()) & 1ull)) {
@@ -532,45 +749,8 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
           nend = g.CP[vn + 1];
         }
       }
-      const uint32_t incl = wave_incl_scan((uint32_t)deg);
-      const int total = (int)__shfl(incl, 63);
-      wave_lds_sync();
-      L->excl[lane()] = (int)incl - deg;
-      L->beg[lane()] = beg;
-      if (SIGMA) {
-        L->x[lane()] = 0;
-        L->xh[lane()] = 0;
-      }
-      wave_lds_sync();
-      uint64_t fmask = 0, omask = 0;
-      // one pass of 4 entries per lane for short words, else passes of 8
-      if (total <= 64 * 4) {
-        pull_light_pass<4, SIGMA>(g, st, L, fpar, ovf_any, lane(), total, fmask, omask);
-      } else {
-        for (int e0 = lane(); e0 < total; e0 += 64 * kPullRegs)
-          pull_light_pass<kPullRegs, SIGMA>(g, st, L, fpar, ovf_any, e0, total, fmask, omask);
-      }
-      fmask = wave_or_u64(fmask);
-      if (fmask == 0) continue;
-      if (SIGMA) omask = wave_or_u64(omask);
-      wave_lds_sync();
-      if ((fmask >> lane()) & 1ull) {
-        st.S[v] = dn;
-        if (SIGMA) {
-          // count = xh * 2^32 + x; at least 2^64 iff xh >= 2^32 or the sum
-          // of the low 64 bits carries
-          const uint64_t xh = L->xh[lane()];
-          const uint64_t hs = xh << 32;
-          const uint64_t sum = hs + L->x[lane()];
-          st.sigma[v] = sum;
-          if ((xh >> 32) != 0 || sum < hs || ((omask >> lane()) & 1ull)) {
-            st.ovf[v] = 1;
-            a.ovf = 1;
-          }
-        }
-        a.found_in(g, v, (uint32_t)deg);
-      }
-      if (lane() == 0) atomicOr((unsigned long long*)&st.fnext[w], fmask);
+      const uint64_t fm = pull_group<SIGMA>(g, st, L, fpar, ovf_any, dn, v, beg, deg, a);
+      if (fm && lane() == 0) atomicOr((unsigned long long*)&st.fnext[w], (unsigned long long)fm);
     }
   }
 }
@@ -587,7 +767,7 @@ __global__ __launch_bounds__(kLbBlock) void k_push_lb(GraphView g, StateView st,
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   LevelAcc a;
   push_lb_level(g, st, d + 1, (uint32_t)ovf_any, bigo, nbigo, wave, nwaves,
-                &lds[threadIdx.x >> 6], a);
+                &lds[threadIdx.x >> 6], a, nullptr);
   flush_level(st.cset, a);
 }
 
@@ -602,7 +782,7 @@ __global__ __launch_bounds__(kLbBlock) void k_pull_lb(GraphView g, StateView st,
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   LevelAcc a;
   pull_lb_level<SIGMA>(g, st, d + 1, (uint32_t)ovf_any, bigc, nbigc, wave,
-                       nwaves, &lds[threadIdx.x >> 6], a, st.fcur);
+                       nwaves, &lds[threadIdx.x >> 6], a, st.fcur, nullptr);
   flush_level(st.cset, a);
 }
 
@@ -663,6 +843,7 @@ __device__ __forceinline__ void init_state(const PersistParams& p, int64_t gtid,
     const bool root_slot = gtid == 0 && r >= 0;
     c->qm = root_slot ? ((1ull << 32) + outdeg(g, r)) : 0ull;
     c->min = root_slot ? indeg(g, r) : 0ull;
+    c->hubs = root_slot ? hub_bits(g, r) : 0ull;
     c->ovf = 0;
   }
 }
@@ -692,6 +873,7 @@ __device__ __forceinline__ bool step_state(const PersistParams& p,
   s.ovf_any |= c->ovf;
   s.reached += q;
   s.edges += mi;
+  s.heavy_reached += c->hin;
   s.mu = s.mu > mi ? s.mu - mi : 0ull;
   if (p.mode == TBFS_MODE_AUTO) {
     if (s.pushing && (double)mf * p.alpha > (double)s.mu) s.pushing = 0;
@@ -711,15 +893,28 @@ __device__ __forceinline__ void write_stats(const PersistParams& p,
   p.stats->ovf_any = s.ovf_any;
   p.stats->reached = s.reached;
   p.stats->edges = s.edges;
+  p.stats->stage_mask = s.stage_mask;
+}
+
+// A pull level stages the frontier bitmap (when the launch has the LDS for
+// it) if the unexplored in-list entries, one parent test each, outnumber the
+// vertices: every workgroup then reads the whole bitmap once.
+__device__ __forceinline__ bool wants_stage(const PersistParams& p, bool pushing,
+                                            unsigned long long mu) {
+  return !pushing && mu > (unsigned long long)p.g.n;
 }
 
 // Housekeeping for depth d+2 plus the level's work (frontier depth d).
 // fb_lds: when not null, pull levels test parents against a copy of the
 // frontier bitmap staged in this workgroup's LDS (random bitmap lookups stay
 // out of the L2); it must hold ceil(n/64) words.
+// frontier_hubs: some frontier vertex has out-degree > kBigIn (else a push
+// skips the out-list slices); heavy_open: some vertex with in-degree > kBigIn
+// is not reached yet (else a pull skips the in-list slices).
 __device__ __forceinline__ void run_level(const PersistParams& p, int d,
                                           bool pushing, uint32_t ovf_any,
-                                          unsigned long long mu,
+                                          unsigned long long mu, bool frontier_hubs,
+                                          bool heavy_open,
                                           int64_t gtid, int64_t gstride,
                                           WaveLds* L, LevelAcc& acc,
                                           uint64_t* fb_lds) {
@@ -728,28 +923,23 @@ __device__ __forceinline__ void run_level(const PersistParams& p, int d,
   const int64_t wave = gtid >> 6, nwaves = gstride >> 6;
   // buffers of depth d+2 were last used by depth d-1: clear them now; and
   // fold the current frontier into the reached bitmap
+  // (done by the level's own word loop, which reads these words anyway)
   uint64_t* fclr = p.fb[(d + 2) % 3];
-  const uint64_t* fcur = p.fb[d % 3];
-  for (int64_t w = gtid; w < nwords; w += gstride) {
-    fclr[w] = 0ull;
-    const uint64_t f = fcur[w];
-    if (f) p.vis[w] |= f;
-  }
   if (gtid < kStripes) {
     CtrStripe* z = &p.cset[(d + 2) % 3].s[gtid];
     z->qm = 0;
     z->min = 0;
+    z->hubs = 0;
     z->ovf = 0;
   }
   StateView st{p.S, p.sigma, p.ovf, p.fb[d % 3], p.fb[(d + 1) % 3],
                nullptr, nullptr, nullptr, &p.cset[(d + 1) % 3], p.vis, p.skip};
   if (pushing) {
-    push_lb_level(g, st, d + 1, ovf_any, p.bigo, p.nbigo, wave, nwaves, L, acc);
+    push_lb_level(g, st, d + 1, ovf_any, p.bigo, frontier_hubs ? p.nbigo : 0, wave, nwaves,
+                  L, acc, fclr);
   } else {
     const uint64_t* fpar = st.fcur;
-    // staging costs every workgroup a read of the whole bitmap: worth it
-    // only when the unexplored in-lists (parent tests) outnumber the vertices
-    if (fb_lds && mu > (unsigned long long)g.n) {
+    if (fb_lds && wants_stage(p, false, mu)) {
       const uint4* src = reinterpret_cast<const uint4*>(st.fcur);
       uint4* dst = reinterpret_cast<uint4*>(fb_lds);
       const int64_t n16 = nwords >> 1;
@@ -758,10 +948,13 @@ __device__ __forceinline__ void run_level(const PersistParams& p, int d,
       __syncthreads();
       fpar = fb_lds;
     }
+    const int32_t nbigc = heavy_open ? p.nbigc : 0;
     if (p.sigma_on)
-      pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc, fpar);
+      pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, nbigc, wave, nwaves, L, acc, fpar,
+                          fclr);
     else
-      pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, p.nbigc, wave, nwaves, L, acc, fpar);
+      pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, nbigc, wave, nwaves, L, acc, fpar,
+                           fclr);
   }
   flush_level(st.cset, acc);
 }
@@ -782,7 +975,8 @@ __global__ __launch_bounds__(BLOCK) void k_persist(PersistParams p) {
   for (;;) {
     const LevelCtr c = read_set(&p.cset[s.d % 3], true);
     if (!step_state(p, &c, s)) break;
-    run_level(p, s.d, s.pushing != 0, s.ovf_any, s.mu, gtid, gstride, L, acc, nullptr);
+    run_level(p, s.d, s.pushing != 0, s.ovf_any, s.mu, c.hout != 0,
+              s.heavy_reached < (unsigned long long)p.nheavy, gtid, gstride, L, acc, nullptr);
     ++s.d;
     grid_barrier(p.bar, ++bk);
   }
@@ -815,6 +1009,8 @@ __global__ __launch_bounds__(256) void k_chain_seed(PersistParams p) {
   CtrStripe* c = &p.cset[0].s[j & (kStripes - 1)];
   atomicAdd(&c->qm, (1ull << 32) + (unsigned long long)outdeg(p.g, r));
   atomicAdd(&c->min, (unsigned long long)indeg(p.g, r));
+  const unsigned long long hb = hub_bits(p.g, r);
+  if (hb) atomicAdd(&c->hubs, hb);
 }
 
 // Chained loop: level d.  Reads slot d & 1 and the counters of depth d (both
@@ -848,10 +1044,14 @@ __global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t 
   }
   LoopState nx = s;
   nx.d = d + 1;
+  // the host picks each launch's shape (staging workgroups or not) from the
+  // levels that staged in its last BFS on this graph
+  if (wants_stage(p, s.pushing != 0, s.mu)) nx.stage_mask |= 1ull << (d < 63 ? d : 63);
   if (gtid == 0) p.loop[(step + 1) & 1] = nx;
   LevelAcc acc;
-  run_level(p, d, s.pushing != 0, s.ovf_any, s.mu, gtid, gstride, &lds[threadIdx.x >> 6], acc,
-            FBL ? fb_dyn : nullptr);
+  run_level(p, d, s.pushing != 0, s.ovf_any, s.mu, c.hout != 0,
+            s.heavy_reached < (unsigned long long)p.nheavy, gtid, gstride,
+            &lds[threadIdx.x >> 6], acc, FBL ? fb_dyn : nullptr);
 }
 
 }  // namespace

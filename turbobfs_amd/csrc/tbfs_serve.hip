@@ -47,6 +47,19 @@ namespace {
 // depths (4) + 64 counts (8).
 constexpr int64_t kSlotBytesPerWord = 6 * 8 + 3 * 4 + 64 * (4 + 8);
 
+// Device bytes a slot holds besides its per-word batch state, for graphs of
+// up to nmax vertices and nzmax entries: the rows of finished queries staged
+// for their copies to the host (up to 64 rows of 13 bytes per vertex, at
+// most 256 MiB, at least one row) and the slot's single-source context
+// (depths 4, counts 8, flags 1 per vertex; five bitmaps; two work queues of
+// n + nz/64 entries of 8 bytes).
+int64_t slot_fixed_bytes(int64_t nmax, int64_t nzmax) {
+  const int64_t row = 13 * nmax;
+  const int64_t stage = row * std::max<int64_t>(1, std::min<int64_t>(64, ((int64_t)256 << 20) / row));
+  const int64_t ctx = 13 * nmax + 5 * 8 * ((nmax + 63) / 64) + 2 * 8 * (nmax + nzmax / 64 + 64);
+  return stage + ctx + ((int64_t)1 << 20);  // counters, loop state, alignment
+}
+
 struct Query {
   uint64_t ticket = 0;
   int32_t graph = 0;
@@ -97,7 +110,8 @@ struct tbfs_server {
   int device = 0;
   tbfs_opts bopts{};
   std::chrono::steady_clock::time_point t0;
-  uint64_t batches = 0, batched = 0;
+  // written by the dispatcher, read by tbfs_server_info on caller threads
+  std::atomic<uint64_t> batches{0}, batched{0};
 
   double now_us() const {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0)
@@ -351,10 +365,11 @@ int tbfs_server_create(const tbfs_graph* const* graphs, int32_t ngraphs,
     return TBFS_ERR_INVALID;
   const int nslots = so.slots ? so.slots : 4;
   const int max_batch = so.max_batch ? so.max_batch : 1024;
-  int32_t nmax = 0;
+  int32_t nmax = 0, nzmax = 0;
   for (int i = 0; i < ngraphs; ++i) {
     if (!graphs[i] || !graphs[i]->CP) return TBFS_ERR_INVALID;
     nmax = std::max(nmax, graphs[i]->n);
+    nzmax = std::max(nzmax, graphs[i]->nz);
   }
   int64_t budget = so.mem_budget;
   if (budget == 0) {
@@ -362,7 +377,10 @@ int tbfs_server_create(const tbfs_graph* const* graphs, int32_t ngraphs,
     HIPCHK(hipMemGetInfo(&fr, &tot));
     budget = std::min<int64_t>((int64_t)(fr / 4), (int64_t)64 << 30);
   }
-  const int64_t cap_words = budget / nslots / kSlotBytesPerWord;
+  // what is left of each slot's share after its fixed buffers holds its
+  // per-word batch state
+  const int64_t per_slot = budget / nslots - slot_fixed_bytes(nmax, nzmax);
+  const int64_t cap_words = per_slot > 0 ? per_slot / kSlotBytesPerWord : 0;
   if (cap_words < nmax) return TBFS_ERR_NOMEM;  // not even 64 sources of the largest graph
   tbfs_server* s = new (std::nothrow) tbfs_server();
   if (!s) return TBFS_ERR_NOMEM;
@@ -403,6 +421,19 @@ int tbfs_server_create(const tbfs_graph* const* graphs, int32_t ngraphs,
   }
   s->th = std::thread(dispatcher, s);
   *out = s;
+  return TBFS_OK;
+}
+
+int tbfs_server_budget_for(const tbfs_graph* const* graphs, int32_t ngraphs, int32_t slots,
+                           int32_t words, int64_t* bytes) {
+  if (!graphs || ngraphs < 1 || slots < 1 || words < 1 || !bytes) return TBFS_ERR_INVALID;
+  int64_t nmax = 0, nzmax = 0;
+  for (int i = 0; i < ngraphs; ++i) {
+    if (!graphs[i]) return TBFS_ERR_INVALID;
+    nmax = std::max<int64_t>(nmax, graphs[i]->n);
+    nzmax = std::max<int64_t>(nzmax, graphs[i]->nz);
+  }
+  *bytes = (int64_t)slots * (slot_fixed_bytes(nmax, nzmax) + (int64_t)words * nmax * kSlotBytesPerWord);
   return TBFS_OK;
 }
 
@@ -479,6 +510,35 @@ void* tbfs_host_alloc(int64_t bytes) {
 
 int tbfs_host_free(void* p) {
   if (p && hipHostFree(p) != hipSuccess) return TBFS_ERR_HIP;
+  return TBFS_OK;
+}
+
+void* tbfs_device_alloc(int64_t bytes) {
+  void* p = nullptr;
+  if (bytes <= 0) return nullptr;
+  if (hipMalloc(&p, (size_t)bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
+}
+
+int tbfs_device_free(void* p) {
+  if (p && hipFree(p) != hipSuccess) return TBFS_ERR_HIP;
+  return TBFS_OK;
+}
+
+int tbfs_device_copy(void* dst, const void* src, int64_t bytes) {
+  if (bytes < 0 || (bytes > 0 && (!dst || !src))) return TBFS_ERR_INVALID;
+  if (bytes == 0) return TBFS_OK;
+  HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDefault));
+  return TBFS_OK;
+}
+
+int tbfs_device_fill(void* dst, int value, int64_t bytes) {
+  if (bytes < 0 || (bytes > 0 && !dst)) return TBFS_ERR_INVALID;
+  if (bytes == 0) return TBFS_OK;
+  HIPCHK(hipMemset(dst, value, (size_t)bytes));
   return TBFS_OK;
 }
 
