@@ -198,6 +198,24 @@ int tbfs_bfs_many(tbfs_ctx *c, const tbfs_graph *g, const int32_t *roots,
 int tbfs_bfs_multi(tbfs_ctx *c, const tbfs_graph *g, const int32_t *roots,
                    int32_t nroots, const tbfs_opts *o, tbfs_stats *st);
 
+/* One BFS from root on g that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory during a call
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  The results go to the caller's device buffers S (n int32),
+ * sigma (n uint64) and ovf (n uint8), with the conventions of tbfs_bfs; the
+ * level loop's working state lives in a fixed device array of the library,
+ * TBFS_NOALLOC_SCRATCH_BYTES per GPU, held from the first call on a GPU until
+ * the process ends, so graphs of up to TBFS_NOALLOC_MAX_N vertices are
+ * taken (larger ones are rejected with TBFS_ERR_INVALID and a message).
+ * Calls on one GPU share that array: the library runs them one at a time
+ * per GPU, and each returns with its work on the device complete.  opts may
+ * be NULL (defaults); modes AUTO and PUSH, chained level loop.  stream is a
+ * hipStream_t (NULL: the null stream).  st->ms is the call's host time. */
+#define TBFS_NOALLOC_MAX_N (1 << 24)
+#define TBFS_NOALLOC_SCRATCH_BYTES (8 * 1024 * 1024 + 64 * 1024)
+int tbfs_bfs_noalloc(const tbfs_graph *g, int32_t root, int32_t *S, uint64_t *sigma,
+                     uint8_t *ovf, void *stream, const tbfs_opts *opts, tbfs_stats *st);
+
 /* Copy the last BFS's results to host memory (any pointer may be NULL;
  * n must equal that graph's vertex count). */
 int tbfs_ctx_read(tbfs_ctx *c, int32_t n, int32_t *S, uint64_t *sigma,

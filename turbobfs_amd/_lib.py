@@ -171,6 +171,8 @@ SIGNATURES = {
     "tbfs_server_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "tbfs_dropin_mapping": (C.c_int, [C.c_int]),
+    "tbfs_bfs_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.POINTER(Opts), C.POINTER(Stats)]),
     "tbfs_device_alloc": (C.c_void_p, [C.c_int64]),
     "tbfs_device_free": (C.c_int, [C.c_void_p]),
     "tbfs_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
