@@ -95,7 +95,16 @@ typedef struct tbfs_opts {
                           vertices numbered by decreasing in-degree when the
                           degrees are skewed (sd > 2 x mean), 1 = always,
                           -1 = never; results are in the caller's numbering */
-  int32_t reserved[3];
+  float stage_ratio;   /* chained loop: pull levels stage the frontier bitmap
+                          in LDS when the unexplored in-list entries exceed
+                          stage_ratio * n */
+  int32_t lb_grid;     /* chained loop: workgroups of the launches that do
+                          not stage the bitmap (0: about one wavefront per
+                          four frontier words) */
+  int32_t reserved[1];
+  /* alpha, beta, stage_ratio and lb_grid left at 0 take the autotuner's
+   * values for the graph's class (tbfs_autotune) when the table holds them,
+   * else alpha 14, beta 24, stage_ratio 1 and the default grid. */
 } tbfs_opts;
 
 typedef struct tbfs_stats {
@@ -114,7 +123,8 @@ typedef struct tbfs_stats {
 } tbfs_stats;
 
 /* Fill *o with the defaults: AUTO, load-balanced kernels, chained loop,
- * sigma on, alpha 14, beta 24. */
+ * sigma on; alpha, beta, stage_ratio, lb_grid 0 (tuned, else 14, 24, 1 and
+ * the default grid). */
 void tbfs_opts_default(tbfs_opts *o);
 
 /* Human-readable name of a status code. */
@@ -289,12 +299,42 @@ int tbfs_batch_autotune(tbfs_batch *b, const tbfs_graph *g, const int32_t *roots
                         int32_t count, const tbfs_opts *base, tbfs_opts *best,
                         double *best_ms);
 
+/* ---- autotuner -----------------------------------------------------------
+ * Single-source runs: times `count` BFS from roots on g with the default
+ * thresholds and launch shapes, then searches alpha {6,10,14,20,30}, beta
+ * {8,16,24,48,96}, stage_ratio {0.25,0.5,1,2,4} and lb_grid {1,2,4,8
+ * workgroups per CU, default} one at a time, twice round (best of two runs
+ * per candidate).  Every candidate gives the same results, which is checked
+ * on each source's level, vertex and edge counts.  *best gets the winner
+ * (other fields from base, or the defaults), *best_ms / *default_ms the time
+ * per BFS of the winner and of the defaults.  The winner is recorded in the
+ * table for g's class (the in-degree statistics key described at
+ * tbfs_batch_autotune): later runs on such graphs with those fields at 0 take
+ * it.  The context holds the last run's results afterwards. */
+int tbfs_autotune(tbfs_ctx *c, const tbfs_graph *g, const int32_t *roots, int32_t count,
+                  const tbfs_opts *base, tbfs_opts *best, double *best_ms,
+                  double *default_ms);
+
+/* The single-source values the table holds for g's class, written into *o
+ * (alpha, beta, stage_ratio, lb_grid; other fields untouched): TBFS_OK, or
+ * TBFS_ERR_UNSUPPORTED when no entry applies. */
+int tbfs_tuned(const tbfs_graph *g, tbfs_opts *o);
+
+/* The table (single-source and batch entries) as JSON: write it to `path`,
+ * merge the entries of `path` into it, empty it, count its entries.  With
+ * the environment variable TBFS_TUNE_CACHE set to a path, the table is read
+ * from that file on first use and written back after every autotune. */
+int tbfs_tune_save(const char *path);
+int tbfs_tune_load(const char *path);
+int tbfs_tune_clear(void);
+int tbfs_tune_count(void);
+
 /* The tuned grid / batch_alpha the table holds for a batch of `count`
  * sources on g, written into *o (other fields untouched).  Returns TBFS_OK
  * when an entry applies, TBFS_ERR_UNSUPPORTED when none does. */
 int tbfs_batch_tuned(const tbfs_graph *g, int32_t count, tbfs_opts *o);
 
-/* Empty the autotuner's table. */
+/* Empty the autotuner's table (as tbfs_tune_clear). */
 int tbfs_batch_tune_clear(void);
 
 /* Results of sources first .. first+count-1 of the last batch, one row of n

@@ -53,4 +53,6 @@ def test_status_strings_and_defaults():
     assert lib.tbfs_status_str(0) == b"ok"
     assert lib.tbfs_status_str(5).startswith(b"path count")
     o = turbobfs_amd.default_opts()
-    assert o.mode == 0 and o.sigma == 1 and o.alpha > 0 and o.beta > 0
+    assert o.mode == 0 and o.sigma == 1
+    # the tunables are left at 0: the autotuner's value, else the default
+    assert o.alpha == 0 and o.beta == 0 and o.stage_ratio == 0 and o.lb_grid == 0

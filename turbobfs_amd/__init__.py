@@ -26,11 +26,17 @@ from .bfs import (  # noqa: F401
     dropin_mapping,
     host_array,
     server_budget_for,
+    tune_clear,
+    tune_count,
+    tune_load,
+    tune_save,
+    tuned,
 )
 
 __all__ = [
     "Batch", "Context", "Graph", "Server", "host_array", "Result", "default_opts", "graphs", "load", "LIB_PATH",
     "batch_tuned", "batch_tune_clear", "server_budget_for",
+    "tuned", "tune_save", "tune_load", "tune_clear", "tune_count",
     "device_count", "device_info",
     "TbfsError",
     "bfs_gpu_td_csc_sc", "bfs_gpu_td_csc_wa", "bfs_gpu_bu_csc_sc",

@@ -62,7 +62,9 @@ class Opts(C.Structure):
         ("batch_alpha", C.c_float),
         ("batch_count_bits", C.c_int32),
         ("batch_order", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("stage_ratio", C.c_float),
+        ("lb_grid", C.c_int32),
+        ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -171,6 +173,13 @@ SIGNATURES = {
     "tbfs_server_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "tbfs_dropin_mapping": (C.c_int, [C.c_int]),
+    "tbfs_autotune": (C.c_int, [C.c_void_p, C.c_void_p, _i32p, C.c_int32, C.POINTER(Opts),
+                                C.POINTER(Opts), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "tbfs_tuned": (C.c_int, [C.c_void_p, C.POINTER(Opts)]),
+    "tbfs_tune_save": (C.c_int, [C.c_char_p]),
+    "tbfs_tune_load": (C.c_int, [C.c_char_p]),
+    "tbfs_tune_clear": (C.c_int, []),
+    "tbfs_tune_count": (C.c_int, []),
     "tbfs_bfs_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.POINTER(Opts), C.POINTER(Stats)]),
     "tbfs_device_alloc": (C.c_void_p, [C.c_int64]),

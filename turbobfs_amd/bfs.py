@@ -257,6 +257,21 @@ class Context(_Handle):
               "tbfs_calibrate_copy")
         return float(ms.value)
 
+    def autotune(self, g: Graph, roots, base: Opts | None = None):
+        """Tune the single-source switch thresholds and launch shapes on g
+        with these roots (tbfs_autotune); returns (best opts, ms per BFS of
+        the winner, ms per BFS of the defaults).  The winner is recorded in
+        the process's table for graphs of g's class."""
+        roots = np.ascontiguousarray(roots, dtype=np.int32)
+        best = Opts()
+        ms = C.c_double()
+        dms = C.c_double()
+        check(_lib.load().tbfs_autotune(self._h, g.handle, _ptr(roots, _i32p),
+                                        int(roots.shape[0]),
+                                        C.byref(base) if base is not None else None,
+                                        C.byref(best), C.byref(ms), C.byref(dms)), "tbfs_autotune")
+        return best, float(ms.value), float(dms.value)
+
     def read(self, n: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         S = np.empty(n, dtype=np.int32)
         sigma = np.empty(n, dtype=np.uint64)
@@ -358,6 +373,33 @@ def batch_tuned(g: Graph, count: int) -> Opts | None:
 
 def batch_tune_clear() -> None:
     check(_lib.load().tbfs_batch_tune_clear(), "tbfs_batch_tune_clear")
+
+
+def tuned(g: Graph) -> Opts | None:
+    """The single-source values the autotuner's table holds for g's class
+    (alpha, beta, stage_ratio, lb_grid on top of the defaults), or None."""
+    o = default_opts()
+    rc = _lib.load().tbfs_tuned(g.handle, C.byref(o))
+    if rc == _lib.TBFS_ERR_UNSUPPORTED:
+        return None
+    check(rc, "tbfs_tuned")
+    return o
+
+
+def tune_save(path: str) -> None:
+    check(_lib.load().tbfs_tune_save(str(path).encode()), "tbfs_tune_save")
+
+
+def tune_load(path: str) -> None:
+    check(_lib.load().tbfs_tune_load(str(path).encode()), "tbfs_tune_load")
+
+
+def tune_clear() -> None:
+    check(_lib.load().tbfs_tune_clear(), "tbfs_tune_clear")
+
+
+def tune_count() -> int:
+    return int(_lib.load().tbfs_tune_count())
 
 
 def host_array(count: int, dtype) -> np.ndarray:

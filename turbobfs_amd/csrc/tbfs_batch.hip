@@ -1094,62 +1094,6 @@ int enqueue(tbfs_batch* b, int d0, int levels) {
 
 }  // namespace
 
-// Autotuner table: tuned launch grid (as workgroups per CU; 0 = the default
-// work-sized grid) and push threshold, keyed by the batch's number of words
-// and the graph's in-degree statistics (floor log2 of n, of the maximum and
-// of twice the mean in-degree, each plus one).  Process-wide.
-struct TuneKey {
-  int32_t w, ln, lmax, lmean;
-  bool operator==(const TuneKey& o) const {
-    return w == o.w && ln == o.ln && lmax == o.lmax && lmean == o.lmean;
-  }
-};
-struct TuneEntry {
-  TuneKey key;
-  int32_t grid_per_cu;
-  float alpha;
-};
-std::mutex g_tune_mu;
-std::vector<TuneEntry> g_tune;
-
-static int ilog2p1(double x) { return x < 1.0 ? 0 : (int)std::floor(std::log2(x)) + 1; }
-
-static int tune_key(tbfs_graph* g, int W, hipStream_t s, TuneKey* k) {
-  tbfs_degree_stats ds;
-  const int rc = graph_degree_stats(g, TBFS_DEGREE_IN, s, &ds);
-  if (rc) return rc;
-  k->w = W;
-  k->ln = ilog2p1((double)g->n);
-  k->lmax = ilog2p1((double)ds.max);
-  k->lmean = ilog2p1(2.0 * ds.mean);
-  return TBFS_OK;
-}
-
-static bool tune_find(const TuneKey& k, TuneEntry* e) {
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  for (const TuneEntry& t : g_tune)
-    if (t.key == k) {
-      *e = t;
-      return true;
-    }
-  return false;
-}
-
-static void tune_put(const TuneEntry& e) {
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  for (TuneEntry& t : g_tune)
-    if (t.key == e.key) {
-      t = e;
-      return;
-    }
-  g_tune.push_back(e);
-}
-
-static bool tune_empty() {
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  return g_tune.empty();
-}
-
 // Start a batch without waiting: roots are copied, the first chain of levels
 // is enqueued on the batch's stream.  32-bit counts first (half the bytes of
 // every count gather); a batch in which some count reaches 2^32 is run again
@@ -1177,7 +1121,7 @@ int batch_start(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32
     if (rc) return rc;
     if (tune_find(k, &e)) {
       o.grid = e.grid_per_cu * std::max(1, b->cus);
-      o.batch_alpha = e.alpha;
+      o.batch_alpha = e.batch_alpha;
     }
   }
   // the graph the batch runs on: g, or its degree-ordered copy (roots and
@@ -1406,7 +1350,8 @@ int batch_autotune(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int
   rc = tune_key(const_cast<tbfs_graph*>(g), batch_words_for(count), b->stream, &e.key);
   if (rc) return rc;
   e.grid_per_cu = bo.grid / cus;
-  e.alpha = bo.batch_alpha;
+  e.batch_alpha = bo.batch_alpha;
+  e.ms = bt;
   tune_put(e);
   return TBFS_OK;
 }
@@ -1426,7 +1371,7 @@ int batch_tuned(const tbfs_graph* g, int32_t count, tbfs_opts* o) {
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   o->grid = e.grid_per_cu * std::max(1, cus);
-  o->batch_alpha = e.alpha;
+  o->batch_alpha = e.batch_alpha;
   return TBFS_OK;
 }
 
@@ -1443,11 +1388,7 @@ int tbfs_batch_tuned(const tbfs_graph* g, int32_t count, tbfs_opts* o) {
   return tbfs::batch_tuned(g, count, o);
 }
 
-int tbfs_batch_tune_clear(void) {
-  std::lock_guard<std::mutex> lk(tbfs::g_tune_mu);
-  tbfs::g_tune.clear();
-  return TBFS_OK;
-}
+int tbfs_batch_tune_clear(void) { return tbfs_tune_clear(); }
 
 int tbfs_batch_create(int32_t n_cap, int32_t b_cap, tbfs_batch** out) {
   return tbfs::batch_create(n_cap, b_cap, out);

@@ -11,10 +11,8 @@
 
 #include <atomic>
 #include <chrono>
-#include <cmath>
 #include <cstdio>
 #include <cstdlib>
-#include <vector>
 
 #include "bfsbucsc.h"
 #include "bfstdbucsc.h"
@@ -53,19 +51,19 @@ tbfs_opts fast_opts(bool sigma) {
   return o;
 }
 
-struct DropinResult {
-  std::vector<int32_t> S;
-  std::vector<uint64_t> sigma;
-  std::vector<uint8_t> ovf;
-  tbfs_stats st{};
-  double total_ms = 0.0;
-};
+// Conversion of the exact counts at the boundary (k_dropin_convert, on the
+// device): float for the TD CSC and COOC entry points (counts above 2^24, or
+// >= 2^64, are not exact in float), int32 modulo 2^32 for TDBU (the
+// reference's int arithmetic; counts above 2^31 - 1 are not exact).
+constexpr int32_t kNoCounts = 0, kFloatCounts = 1, kIntCounts = 2;
 
-// Upload, run `repetition` BFS from r, read back.  `cooc` selects the COOC
-// upload (A = I, B = J) instead of CSC (A = CP, B = IC).
-int run_dropin(const char* name, const int* A, const int* B, int nz, int n,
-               int r, int repetition, const tbfs_opts& o, bool cooc,
-               bool want_sigma, DropinResult* res) {
+// Upload, run `repetition` BFS from r, write S (unreached as s_unreached;
+// S_h may be null) and the counts in the reference's type into the caller's
+// arrays.  `cooc` selects the COOC upload (A = I, B = J) instead of CSC
+// (A = CP, B = IC).
+int run_dropin(const char* name, const int* A, const int* B, int nz, int n, int r,
+               int repetition, const tbfs_opts& o, bool cooc, int32_t s_unreached,
+               int32_t kind, int* S_h, void* val_h) {
   if (n <= 0 || nz < 0 || r < 0 || r >= n || repetition < 1 || !A ||
       (nz > 0 && !B)) {
     std::fprintf(stderr,
@@ -99,75 +97,41 @@ int run_dropin(const char* name, const int* A, const int* B, int nz, int n,
     oo.loop = TBFS_LOOP_HOST;
   }
   const auto t2 = clk::now();
+  tbfs_stats st{};
+  double total_ms = 0.0;
   for (int i = 0; i < repetition && !rc; ++i) {
-    rc = run_bfs(c, g, r, oo, &res->st);
-    res->total_ms += res->st.ms;
-  }
-  if (!rc) {
-    res->S.resize((size_t)n);
-    if (want_sigma) {
-      res->sigma.resize((size_t)n);
-      res->ovf.resize((size_t)n);
-    }
-    rc = ctx_read(c, n, res->S.data(), want_sigma ? res->sigma.data() : nullptr,
-                  want_sigma ? res->ovf.data() : nullptr);
+    rc = run_bfs(c, g, r, oo, &st);
+    total_ms += st.ms;
   }
   const auto t3 = clk::now();
+  int64_t inexact = 0;
+  if (!rc) rc = ctx_read_converted(c, n, s_unreached, kind, S_h, val_h, &inexact);
+  const auto t4 = clk::now();
   ctx_free(c);
   graph_free(g);
-  const auto t4 = clk::now();
+  const auto t5 = clk::now();
   auto ms = [](clk::time_point a, clk::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
   };
   if (rc) {
-    std::fprintf(stderr, "turbobfs: %s: BFS failed: %s\n", name,
-                 tbfs_status_str(rc));
+    std::fprintf(stderr, "turbobfs: %s: BFS failed: %s\n", name, tbfs_status_str(rc));
     return rc;
   }
   if (verbose()) {
     std::printf("\n%s::d = %d,r = %d,t_sum=%lfms \n%s::average time BFS = %lfms \n",
-                name, res->st.levels, r, res->total_ms, name,
-                res->total_ms / repetition);
-    std::printf("%s::host ms: graph upload + checks %.3f, context %.3f, BFS + read %.3f, "
-                "free %.3f\n", name, ms(t0, t1), ms(t1, t2), ms(t2, t3), ms(t3, t4));
-  }
-  return TBFS_OK;
-}
-
-// Exact counts -> float; counts above 2^24 (or >= 2^64) are inexact in float.
-int to_float(const char* name, const DropinResult& res, float* out, int n) {
-  long long inexact = 0;
-  for (int i = 0; i < n; ++i) {
-    if (res.ovf[i]) {
-      out[i] = INFINITY;
-      ++inexact;
-    } else {
-      out[i] = (float)res.sigma[i];
-      if (res.sigma[i] > (1ull << 24)) ++inexact;
-    }
+                name, st.levels, r, total_ms, name, total_ms / repetition);
+    std::printf("%s::host ms: graph upload + checks %.3f, context %.3f, BFS %.3f, "
+                "convert + read %.3f, free %.3f\n", name, ms(t0, t1), ms(t1, t2), ms(t2, t3),
+                ms(t3, t4), ms(t4, t5));
   }
   if (inexact) {
-    std::fprintf(stderr,
-                 "turbobfs: %s: %lld path counts exceed 2^24 and are not "
-                 "exact in float\n",
-                 name, inexact);
-    return TBFS_SIGMA_INEXACT;
-  }
-  return TBFS_OK;
-}
-
-// Exact counts -> int32 modulo 2^32 (the reference's int arithmetic).
-int to_int(const char* name, const DropinResult& res, int* out, int n) {
-  long long inexact = 0;
-  for (int i = 0; i < n; ++i) {
-    out[i] = (int)(uint32_t)res.sigma[i];
-    if (res.ovf[i] || res.sigma[i] > 0x7fffffffull) ++inexact;
-  }
-  if (inexact) {
-    std::fprintf(stderr,
-                 "turbobfs: %s: %lld path counts exceed 2^31-1; written "
-                 "modulo 2^32\n",
-                 name, inexact);
+    if (kind == kFloatCounts)
+      std::fprintf(stderr,
+                   "turbobfs: %s: %lld path counts exceed 2^24 and are not exact in float\n",
+                   name, (long long)inexact);
+    else
+      std::fprintf(stderr, "turbobfs: %s: %lld path counts exceed 2^31-1; written modulo 2^32\n",
+                   name, (long long)inexact);
     return TBFS_SIGMA_INEXACT;
   }
   return TBFS_OK;
@@ -182,11 +146,9 @@ int td_csc(const char* name, int* IC_h, int* CP_h, int* S_h, float* sigma_h,
     o.loop = TBFS_LOOP_HOST;
     o.pull = mode == TBFS_MODE_PULL_WA ? TBFS_PULL_WA : TBFS_PULL_SC;
   }
-  DropinResult res;
-  int rc = run_dropin(name, CP_h, IC_h, nz, n, r, repetition, o, false, true, &res);
-  if (rc) return rc;
-  for (int i = 0; i < n; ++i) S_h[i] = res.S[i] < 0 ? 0 : res.S[i];
-  return to_float(name, res, sigma_h, n);
+  // TD CSC: unreached vertices keep S = 0 (the reference clears S_d to 0)
+  return run_dropin(name, CP_h, IC_h, nz, n, r, repetition, o, false, 0, kFloatCounts, S_h,
+                    sigma_h);
 }
 
 int bu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int r, int nz,
@@ -199,11 +161,7 @@ int bu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int r, int nz,
     o.pull = mode == TBFS_MODE_PULL_WA ? TBFS_PULL_WA : TBFS_PULL_SC;
   }
   o.bbreak = bbreak ? 1 : 0;
-  DropinResult res;
-  int rc = run_dropin(name, CP_h, I_h, nz, n, r, repetition, o, false, false, &res);
-  if (rc) return rc;
-  for (int i = 0; i < n; ++i) S_h[i] = res.S[i];
-  return TBFS_OK;
+  return run_dropin(name, CP_h, I_h, nz, n, r, repetition, o, false, -1, kNoCounts, S_h, nullptr);
 }
 
 int tdbu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int* sigma_h,
@@ -214,11 +172,8 @@ int tdbu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int* sigma_h,
     o.pull = pull;
     o.loop = pull == TBFS_PULL_LB ? TBFS_LOOP_CHAINED : TBFS_LOOP_HOST;
   }
-  DropinResult res;
-  int rc = run_dropin(name, CP_h, I_h, nz, n, r, repetition, o, false, true, &res);
-  if (rc) return rc;
-  for (int i = 0; i < n; ++i) S_h[i] = res.S[i];
-  return to_int(name, res, sigma_h, n);
+  return run_dropin(name, CP_h, I_h, nz, n, r, repetition, o, false, -1, kIntCounts, S_h,
+                    sigma_h);
 }
 
 }  // namespace
@@ -278,11 +233,8 @@ int bfs_gpu_td_cooc_sc(int* I_h, int* J_h, float* sigma_h, int r, int nz,
     o.mode = TBFS_MODE_COOC;
     o.loop = TBFS_LOOP_HOST;
   }
-  DropinResult res;
-  int rc = run_dropin("bfsgputdcooc_sc", I_h, J_h, nz, n, r, repetition, o,
-                      true, true, &res);
-  if (rc) return rc;
-  return to_float("bfsgputdcooc_sc", res, sigma_h, n);
+  return run_dropin("bfsgputdcooc_sc", I_h, J_h, nz, n, r, repetition, o, true, -1,
+                    kFloatCounts, nullptr, sigma_h);
 }
 
 }  // extern "C"

@@ -131,6 +131,31 @@ inline int grid_for(int64_t items, int block, int cap) {
   return (int)g;
 }
 
+// The autotuner's table (tbfs_tune.hip).  Key: w = 0 for single-source
+// runs, else the batch's 64-source words; the graph's in-degree statistics
+// as floor(log2) + 1 of n, of the maximum and of twice the mean.
+struct TuneKey {
+  int32_t w, ln, lmax, lmean;
+  bool operator==(const TuneKey& o) const {
+    return w == o.w && ln == o.ln && lmax == o.lmax && lmean == o.lmean;
+  }
+};
+struct TuneEntry {
+  TuneKey key;
+  int32_t grid_per_cu = 0;   // batches: workgroups per CU (0: work-sized)
+  float batch_alpha = 0.f;   // batches: push threshold
+  float alpha = 0.f;         // single source: push -> pull threshold
+  float beta = 0.f;          // single source: pull -> push threshold
+  float stage_ratio = 0.f;   // single source: stage the bitmap above ratio * n
+  int32_t lb_grid = 0;       // single source: workgroups of non-staging levels
+  double ms = 0.0;           // the winner's time and the defaults' (per run)
+  double default_ms = 0.0;
+};
+int tune_key(tbfs_graph* g, int32_t words, hipStream_t s, TuneKey* k);
+bool tune_find(const TuneKey& k, TuneEntry* e);
+void tune_put(const TuneEntry& e);
+bool tune_empty();
+
 // Static kPullChunk-entry slices of every list of ptr[0..n] longer than
 // min_deg (entries v | slice << 32, in vertex order), built on stream s
 // (returns with the table complete, *out allocated with *count entries).
@@ -189,4 +214,6 @@ int batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* 
                uint64_t* sigma, uint8_t* ovf);
 int ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma,
              uint8_t* ovf);
+int ctx_read_converted(tbfs_ctx* c, int32_t n, int32_t s_unreached, int32_t kind,
+                       int32_t* S_out, void* val_out, int64_t* inexact);
 }  // namespace tbfs

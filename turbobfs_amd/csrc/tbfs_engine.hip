@@ -739,6 +739,32 @@ static int ctx_reserve_events(tbfs_ctx* c, size_t count) {
 
 // ----------------------------------------------------------------------------
 // argument checks shared by both level loops
+// Defaults of the single-source tunables (tbfs_opts fields left at 0 when
+// the autotuner's table has no entry for the graph).
+constexpr float kDefaultAlpha = 14.0f;
+constexpr float kDefaultBeta = 24.0f;
+constexpr float kDefaultStageRatio = 1.0f;
+
+// Options with the tunable fields left at 0 filled from the autotuner's
+// table entry for g's class, when there is one (the others keep 0: defaults).
+static tbfs_opts resolve_opts(const tbfs_graph* g, const tbfs_opts& o, hipStream_t s) {
+  tbfs_opts r = o;
+  if (r.alpha > 0 && r.beta > 0 && r.stage_ratio > 0 && r.lb_grid > 0) return r;
+  if (tune_empty() || !g->CP) return r;
+  TuneKey k;
+  TuneEntry e;
+  if (tune_key(const_cast<tbfs_graph*>(g), 0, s, &k) != TBFS_OK) {
+    (void)hipGetLastError();
+    return r;
+  }
+  if (!tune_find(k, &e)) return r;
+  if (r.alpha <= 0) r.alpha = e.alpha;
+  if (r.beta <= 0) r.beta = e.beta;
+  if (r.stage_ratio <= 0) r.stage_ratio = e.stage_ratio;
+  if (r.lb_grid <= 0) r.lb_grid = e.lb_grid;
+  return r;
+}
+
 static int check_run(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
                      const tbfs_opts& o) {
   if (!c || !g) return TBFS_ERR_INVALID;
@@ -795,8 +821,9 @@ static PersistParams make_params(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   p.root = root;
   p.mode = o.mode;
   p.sigma_on = o.sigma ? 1 : 0;
-  p.alpha = o.alpha > 0 ? o.alpha : 14.0f;
-  p.beta = o.beta > 0 ? o.beta : 24.0f;
+  p.alpha = o.alpha > 0 ? o.alpha : kDefaultAlpha;
+  p.beta = o.beta > 0 ? o.beta : kDefaultBeta;
+  p.stage_ratio = o.stage_ratio > 0 ? o.stage_ratio : kDefaultStageRatio;
   p.loop = c->loop;
   return p;
 }
@@ -865,13 +892,16 @@ struct LevelPlan {
   }
 };
 
+// about four frontier words per wavefront: fewer workgroups to dispatch per
+// level and fewer counter atomics, still >> 256 CUs' worth of waves
+static int default_lb_grid(const tbfs_graph* g) {
+  return grid_for((((int64_t)g->n + 63) / 64) * 16, kLbBlock, 2048);
+}
+
 static LevelPlan level_plan(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o) {
   LevelPlan pl;
   pl.fbl = chain_fbl(c, g, o);
-  const int64_t nwords = ((int64_t)g->n + 63) / 64;
-  // about four frontier words per wavefront: fewer workgroups to dispatch
-  // per level and fewer counter atomics, still >> 256 CUs' worth of waves
-  pl.grid_lb = grid_for(nwords * 16, kLbBlock, 2048);
+  pl.grid_lb = o.lb_grid > 0 ? std::min(o.lb_grid, 65536) : default_lb_grid(g);
   pl.grid_stage = std::max(1, c->cus);
   pl.grid_spare = std::max(1, c->cus / 8);
   if (o.grid > 0) {
@@ -987,8 +1017,8 @@ static int run_host_loop(tbfs_ctx* c, tbfs_graph* g, int32_t root,
   int32_t d = 0;
   int push_levels = 0, pull_levels = 0;
   bool pushing = (o.mode == TBFS_MODE_PUSH || o.mode == TBFS_MODE_AUTO);
-  const float alpha = o.alpha > 0 ? o.alpha : 14.0f;
-  const float beta = o.beta > 0 ? o.beta : 24.0f;
+  const float alpha = o.alpha > 0 ? o.alpha : kDefaultAlpha;
+  const float beta = o.beta > 0 ? o.beta : kDefaultBeta;
   const int sig = o.sigma ? 1 : 0;
 
   for (;;) {
@@ -1161,9 +1191,10 @@ static int ctx_reserve_graph(tbfs_ctx* c, int count) {
 }
 
 int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
-                 int count, const tbfs_opts& o, tbfs_stats* st) {
+                 int count, const tbfs_opts& o_in, tbfs_stats* st) {
   tbfs_graph* g = const_cast<tbfs_graph*>(gc);
-  if (count < 1 || !roots) return TBFS_ERR_INVALID;
+  if (count < 1 || !roots || !c || !g) return TBFS_ERR_INVALID;
+  const tbfs_opts o = resolve_opts(g, o_in, c->stream);
   for (int i = 0; i < count; ++i) {
     int rc = check_run(c, g, roots[i], o);
     if (rc) return rc;
@@ -1242,12 +1273,20 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
     // spare, so a source needing one launch more than any seen here still
     // finishes without a re-run (a spare launch costs ~4 us, a re-run a BFS)
     c->chain_k = std::min(4096, std::max(2, maxl + 1));
-    // the next call's launch shapes: the levels that staged and the depth
+    // the next call's launch shapes: the levels that staged for any source,
+    // kept across calls on this graph (on R-MAT scale 20 a staging pull on
+    // the non-staging shape costs ~40 us, a hub push on the staging shape
+    // ~10 us: a level that stages for some sources stages for all), and the
+    // depth
     uint64_t stage = 0;
     int depth = 0;
     for (int i = 0; i < count; ++i) {
       stage |= c->stats_h[i].stage_mask;
       depth = std::max(depth, (int)c->stats_h[i].levels);
+    }
+    if (c->pred_graph == g && c->pred_stage != ~0ull) {
+      stage |= c->pred_stage;
+      depth = std::max(depth, c->pred_levels);
     }
     c->pred_graph = g;
     c->pred_stage = stage;
@@ -1267,7 +1306,7 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
 int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots,
               const tbfs_opts& o_in, tbfs_stats* st) {
   if (!c || !g || !roots || nroots < 1) return TBFS_ERR_INVALID;
-  tbfs_opts o = o_in;
+  tbfs_opts o = resolve_opts(g, o_in, c->stream);
   if (o.mode != TBFS_MODE_AUTO && o.mode != TBFS_MODE_PUSH) return TBFS_ERR_UNSUPPORTED;
   o.pull = TBFS_PULL_LB;
   o.loop = TBFS_LOOP_CHAINED;
@@ -1316,9 +1355,10 @@ int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots
 // most level launches a finished source used.
 int ctx_async_begin(tbfs_ctx* c, int count) { return ctx_reserve_stats(c, count); }
 
-int ctx_async_enqueue(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs_opts& o,
+int ctx_async_enqueue(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs_opts& o_in,
                       int idx, int levels, int32_t* S, uint64_t* sigma, uint8_t* ovf) {
   if (idx < 0 || idx >= c->stats_cap || levels < 1) return TBFS_ERR_INVALID;
+  const tbfs_opts o = resolve_opts(g, o_in, c->stream);
   int rc = check_run(c, g, root, o);
   if (rc) return rc;
   rc = enqueue_chained(c, g, root, o, c->stats + idx, levels);
@@ -1361,6 +1401,144 @@ int run_bfs(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs_opts& o,
   return run_bfs_many(c, g, &root, 1, o, st);
 }
 
+// Single-source autotuner: coordinate search over the switch thresholds,
+// the staging ratio and the non-staging grid, timing `count` BFS per
+// candidate (device time, best of two runs).  Every candidate must give the
+// same per-source level, vertex and edge counts as the defaults.
+int autotune_single(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int count,
+                    const tbfs_opts* base, tbfs_opts* best, double* best_ms,
+                    double* default_ms) {
+  if (!c || !g || !roots || count < 1 || !best) return TBFS_ERR_INVALID;
+  if (!g->CP) return TBFS_ERR_UNSUPPORTED;
+  tbfs_opts o;
+  if (base) o = *base;
+  else tbfs_opts_default(&o);
+  o.mode = TBFS_MODE_AUTO;
+  o.pull = TBFS_PULL_LB;
+  if (o.loop != TBFS_LOOP_GRAPH) o.loop = TBFS_LOOP_CHAINED;
+  o.grid = 0;
+  const int cus = std::max(1, c->cus);
+  // the defaults, explicit (so no earlier table entry takes part)
+  o.alpha = kDefaultAlpha;
+  o.beta = kDefaultBeta;
+  o.stage_ratio = kDefaultStageRatio;
+  o.lb_grid = default_lb_grid(g);
+  std::vector<tbfs_stats> st((size_t)count), ref((size_t)count);
+  auto eval = [&](const tbfs_opts& t, double* ms, bool is_ref) -> int {
+    double best_t = -1;
+    for (int rep = 0; rep < 2; ++rep) {
+      const int rc = run_bfs_many(c, g, roots, count, t, st.data());
+      if (rc) return rc;
+      double sum = 0;
+      for (int i = 0; i < count; ++i) {
+        sum += st[i].ms;
+        const tbfs_stats& a = st[i];
+        const tbfs_stats& b = is_ref ? a : ref[i];
+        if (a.levels != b.levels || a.reached != b.reached || a.edges != b.edges) {
+          std::fprintf(stderr, "turbobfs: autotune: a candidate changed the result of source %d\n",
+                       roots[i]);
+          return TBFS_ERR_HIP;
+        }
+      }
+      if (is_ref) ref = st;
+      if (best_t < 0 || sum < best_t) best_t = sum;
+    }
+    *ms = best_t / count;
+    return TBFS_OK;
+  };
+  double bt = 0;
+  int rc = run_bfs_many(c, g, roots, count, o, st.data());  // warm-up: plan history
+  if (!rc) rc = eval(o, &bt, true);
+  if (rc) return rc;
+  const double dt = bt;
+  tbfs_opts bo = o;
+  const float alphas[] = {6.f, 10.f, 14.f, 20.f, 30.f};
+  const float betas[] = {8.f, 16.f, 24.f, 48.f, 96.f};
+  const float ratios[] = {0.25f, 0.5f, 1.f, 2.f, 4.f};
+  const int grids[] = {cus, 2 * cus, 4 * cus, 8 * cus, default_lb_grid(g)};
+  for (int round = 0; round < 2; ++round) {
+    for (int dim = 0; dim < 4; ++dim) {
+      for (int k = 0; k < 5; ++k) {
+        tbfs_opts t = bo;
+        if (dim == 0) t.alpha = alphas[k];
+        else if (dim == 1) t.beta = betas[k];
+        else if (dim == 2) t.stage_ratio = ratios[k];
+        else t.lb_grid = std::min(grids[k], 65536);
+        if (t.alpha == bo.alpha && t.beta == bo.beta && t.stage_ratio == bo.stage_ratio &&
+            t.lb_grid == bo.lb_grid)
+          continue;
+        double ms = 0;
+        rc = eval(t, &ms, false);
+        if (rc) return rc;
+        if (ms < bt * 0.995) {  // a real gain, not noise
+          bt = ms;
+          bo = t;
+        }
+      }
+    }
+  }
+  *best = bo;
+  if (best_ms) *best_ms = bt;
+  if (default_ms) *default_ms = dt;
+  TuneEntry e;
+  rc = tune_key(const_cast<tbfs_graph*>(g), 0, c->stream, &e.key);
+  if (rc) return rc;
+  e.alpha = bo.alpha;
+  e.beta = bo.beta;
+  e.stage_ratio = bo.stage_ratio;
+  e.lb_grid = bo.lb_grid;
+  e.ms = bt;
+  e.default_ms = dt;
+  tune_put(e);
+  return TBFS_OK;
+}
+
+int tuned_single(const tbfs_graph* g, tbfs_opts* o) {
+  if (!g || !o || !g->CP) return TBFS_ERR_INVALID;
+  if (tune_empty()) return TBFS_ERR_UNSUPPORTED;
+  hipStream_t s = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  TuneKey k;
+  const int rc = tune_key(const_cast<tbfs_graph*>(g), 0, s, &k);
+  (void)hipStreamDestroy(s);
+  if (rc) return rc;
+  TuneEntry e;
+  if (!tune_find(k, &e)) return TBFS_ERR_UNSUPPORTED;
+  o->alpha = e.alpha;
+  o->beta = e.beta;
+  o->stage_ratio = e.stage_ratio;
+  o->lb_grid = e.lb_grid;
+  return TBFS_OK;
+}
+
+// The last BFS's results in the reference entry points' types, converted on
+// the device and copied into host arrays: S_out (unreached as s_unreached)
+// (unless null) and, with kind 1 / 2, val_out as float / int32 (see
+// k_dropin_convert); *inexact gets the number of counts beyond the type's
+// exact range.
+int ctx_read_converted(tbfs_ctx* c, int32_t n, int32_t s_unreached, int32_t kind,
+                       int32_t* S_out, void* val_out, int64_t* inexact) {
+  if (!c || n != c->last_n || n <= 0 || (kind != 0 && !val_out)) return TBFS_ERR_INVALID;
+  hipStream_t s = c->stream;
+  int32_t* tmp = nullptr;  // converted S, then converted counts, then the counter
+  int rc = dev_alloc(&tmp, 2 * (size_t)n + 2);
+  if (rc) return rc;
+  uint32_t* vals = reinterpret_cast<uint32_t*>(tmp + n);
+  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(tmp + 2 * (size_t)n);
+  unsigned long long bad = 0;
+  hipError_t e = hipMemsetAsync(cnt, 0, sizeof(unsigned long long), s);
+  if (e == hipSuccess)
+    e = kern::dropin_convert(s, c->S, c->sigma, c->ovf, n, s_unreached, kind, tmp, vals, cnt);
+  if (e == hipSuccess && S_out) e = download_d2h(S_out, tmp, (size_t)n * sizeof(int32_t), s);
+  if (e == hipSuccess && kind != 0) e = download_d2h(val_out, vals, (size_t)n * sizeof(uint32_t), s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, cnt, sizeof(bad), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dev_free(tmp);
+  if (e != hipSuccess) return hip_fail(e, "ctx_read_converted", __FILE__, __LINE__);
+  if (inexact) *inexact = (int64_t)bad;
+  return TBFS_OK;
+}
+
 int ctx_read(tbfs_ctx* c, int32_t n, int32_t* S, uint64_t* sigma, uint8_t* ovf) {
   if (!c || n != c->last_n || n <= 0) return TBFS_ERR_INVALID;
   hipStream_t s = c->stream;
@@ -1383,10 +1561,9 @@ void tbfs_opts_default(tbfs_opts* o) {
   o->mode = TBFS_MODE_AUTO;
   o->sigma = 1;
   o->bbreak = 0;
-  o->alpha = 14.0f;
-  o->beta = 24.0f;
   o->pull = TBFS_PULL_LB;
   o->loop = TBFS_LOOP_CHAINED;
+  // alpha, beta, stage_ratio, lb_grid: 0, the tuned values or the defaults
 }
 
 const char* tbfs_status_str(int s) {
@@ -1460,6 +1637,13 @@ int tbfs_bfs_many(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots,
   tbfs_opts_default(&def);
   return run_bfs_many(c, g, roots, count, o ? *o : def, st);
 }
+
+int tbfs_autotune(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int32_t count,
+                  const tbfs_opts* base, tbfs_opts* best, double* best_ms, double* default_ms) {
+  return autotune_single(c, g, roots, count, base, best, best_ms, default_ms);
+}
+
+int tbfs_tuned(const tbfs_graph* g, tbfs_opts* o) { return tuned_single(g, o); }
 
 int tbfs_bfs_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots,
                    int32_t nroots, const tbfs_opts* o, tbfs_stats* st) {

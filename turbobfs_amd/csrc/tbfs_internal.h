@@ -150,6 +150,7 @@ struct PersistParams {
   int32_t sigma_on;
   float alpha;
   float beta;
+  float stage_ratio;  // pull levels stage the bitmap when mu > stage_ratio * n
   LoopState* loop;  // [2] (chained loop only)
 };
 
@@ -202,6 +203,10 @@ hipError_t slice_fill(hipStream_t s, const int32_t* cnt, const int32_t* off, int
                       uint64_t* out);
 hipError_t skip_bits(hipStream_t s, const int32_t* ptr, int32_t n, uint64_t* skip,
                      int64_t nw);
+// the reference entry points' output types (see k_dropin_convert)
+hipError_t dropin_convert(hipStream_t s, const int32_t* S, const uint64_t* sigma,
+                          const uint8_t* ovf, int32_t n, int32_t s_unreached, int32_t kind,
+                          int32_t* S_out, uint32_t* val_out, unsigned long long* inexact);
 // ptr[r] = first k with keys[k] >= r, r = 0 .. n (keys non-decreasing)
 hipError_t offsets_sorted(hipStream_t s, const int32_t* keys, int32_t nz, int32_t n,
                           int32_t* ptr);

@@ -326,6 +326,36 @@ __global__ void k_slice_fill(const int32_t* cnt, const int32_t* off, int32_t n,
     out[o + k] = (uint64_t)(uint32_t)v | ((uint64_t)(uint32_t)k << 32);
 }
 
+// The reference entry points' output types, converted on the device: S with
+// unreached vertices as s_unreached (0 for top-down CSC, -1 otherwise); with
+// kind 1 the counts as float (INFINITY when >= 2^64), kind 2 as int32 modulo
+// 2^32 (the reference's int arithmetic), kind 0 none; *inexact counts the
+// vertices whose count is beyond the type's exact range (float: above 2^24,
+// int32: above 2^31 - 1, or >= 2^64).  One atomic per wavefront.
+__global__ void k_dropin_convert(const int32_t* S, const uint64_t* sigma, const uint8_t* ovf,
+                                 int32_t n, int32_t s_unreached, int32_t kind, int32_t* S_out,
+                                 uint32_t* val_out, unsigned long long* inexact) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (i < n) {
+    const int32_t sv = S[i];
+    S_out[i] = sv < 0 ? s_unreached : sv;
+    if (kind != 0) {
+      const uint64_t x = sigma[i];
+      const bool o = ovf[i] != 0;
+      if (kind == 1) {
+        val_out[i] = __float_as_uint(o ? INFINITY : (float)x);
+        bad = o || x > (1ull << 24);
+      } else {
+        val_out[i] = (uint32_t)x;
+        bad = o || x > 0x7fffffffull;
+      }
+    }
+  }
+  const uint64_t m = __ballot(bad);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(inexact, (unsigned long long)__popcll(m));
+}
+
 // Bitmap of the columns the light-column pull skips (no in-list, or one split
 // into slices); bits past n are set.  One wavefront per word.
 __global__ void k_skip_bits(const int32_t* ptr, int32_t n, uint64_t* skip, int64_t nw) {
@@ -555,6 +585,15 @@ hipError_t slice_fill(hipStream_t s, const int32_t* cnt, const int32_t* off, int
                       uint64_t* out) {
   const int grid = (int)(((int64_t)n + 255) / 256);
   hipLaunchKernelGGL(k_slice_fill, dim3(grid), dim3(256), 0, s, cnt, off, n, out);
+  return hipGetLastError();
+}
+
+hipError_t dropin_convert(hipStream_t s, const int32_t* S, const uint64_t* sigma,
+                          const uint8_t* ovf, int32_t n, int32_t s_unreached, int32_t kind,
+                          int32_t* S_out, uint32_t* val_out, unsigned long long* inexact) {
+  const int grid = (int)(((int64_t)n + 255) / 256);
+  hipLaunchKernelGGL(k_dropin_convert, dim3(grid), dim3(256), 0, s, S, sigma, ovf, n, s_unreached,
+                     kind, S_out, val_out, inexact);
   return hipGetLastError();
 }
 

@@ -138,8 +138,11 @@ int tbfs_bfs_noalloc(const tbfs_graph* g, int32_t root, int32_t* S, uint64_t* si
   p.roots = nullptr;
   p.mode = o.mode;
   p.sigma_on = o.sigma ? 1 : 0;
+  // tunables left at 0 take the defaults: the autotuner's table is keyed by
+  // degree statistics, which a graph computes (and allocates for) on first use
   p.alpha = o.alpha > 0 ? o.alpha : 14.0f;
   p.beta = o.beta > 0 ? o.beta : 24.0f;
+  p.stage_ratio = o.stage_ratio > 0 ? o.stage_ratio : 1.0f;
   p.loop = reinterpret_cast<LoopState*>(base + kOffLoop);
 
   // launch shapes: pull levels stage the frontier bitmap in LDS when it fits
@@ -147,7 +150,9 @@ int tbfs_bfs_noalloc(const tbfs_graph* g, int32_t root, int32_t* S, uint64_t* si
   const size_t bm = (size_t)nwords * sizeof(uint64_t);
   const size_t fbl = (o.fb_lds >= 0 && ds.fbl_limit && bm <= ds.fbl_limit) ? ((bm + 15) & ~(size_t)15)
                                                                            : 0;
-  const int grid_lb = o.grid > 0 ? std::min(o.grid, 65536) : grid_for(nwords * 16, kLbBlock, 2048);
+  const int grid_lb = o.grid > 0     ? std::min(o.grid, 65536)
+                      : o.lb_grid > 0 ? std::min(o.lb_grid, 65536)
+                                      : grid_for(nwords * 16, kLbBlock, 2048);
   const int grid_stage = o.grid > 0 ? std::min(o.grid, 65536) : std::max(1, ds.cus);
 
   hipStream_t s = static_cast<hipStream_t>(stream);

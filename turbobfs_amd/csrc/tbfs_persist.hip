@@ -898,10 +898,11 @@ __device__ __forceinline__ void write_stats(const PersistParams& p,
 
 // A pull level stages the frontier bitmap (when the launch has the LDS for
 // it) if the unexplored in-list entries, one parent test each, outnumber the
-// vertices: every workgroup then reads the whole bitmap once.
+// vertices times stage_ratio (1 by default): every workgroup then reads the
+// whole bitmap once.
 __device__ __forceinline__ bool wants_stage(const PersistParams& p, bool pushing,
                                             unsigned long long mu) {
-  return !pushing && mu > (unsigned long long)p.g.n;
+  return !pushing && (double)mu > (double)p.stage_ratio * (double)p.g.n;
 }
 
 // Housekeeping for depth d+2 plus the level's work (frontier depth d).
