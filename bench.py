@@ -29,19 +29,6 @@ import time
 
 import numpy as np
 
-# The serving measurement runs 4 slots on streams of their own.  With HIP's
-# default of 4 hardware queues per process, streams made earlier in the
-# process (the single-source measurement's context) shift which queue each
-# slot's stream lands on, and two busy slots sharing a queue wait for each
-# other (saturated serving then drops from ~2200 to ~1800 queries/s).  Read
-# by the HIP runtime when it starts, i.e. before the library is loaded.
-try:
-    _hw_queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
-except ValueError:
-    _hw_queues = 4
-if _hw_queues < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
-
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -524,9 +511,13 @@ def _bench_serving(tb, cs, names, gs, seconds, rates):
             done = collect(srv, count)
             lat = np.array([d["done_us"] - d["submit_us"] for d in done]) * 1e-3
             gof = np.array([graph_of[d["ticket"]] for d in done])
+            wait = np.array([d["start_us"] - d["submit_us"] for d in done]) * 1e-3
+            serv = np.array([d["done_us"] - d["start_us"] for d in done]) * 1e-3
             per_graph = {names[gi]: {"queries": int((gof == gi).sum()),
                                      "p50_ms": float(np.percentile(lat[gof == gi], 50)),
-                                     "p99_ms": float(np.percentile(lat[gof == gi], 99))}
+                                     "p99_ms": float(np.percentile(lat[gof == gi], 99)),
+                                     "wait_p50_ms": float(np.percentile(wait[gof == gi], 50)),
+                                     "service_p50_ms": float(np.percentile(serv[gof == gi], 50))}
                          for gi in range(len(cs)) if (gof == gi).any()}
             span = (max(d["done_us"] for d in done) - min(d["submit_us"] for d in done)) * 1e-6
             out["poisson"].append({
@@ -607,7 +598,8 @@ def main():
                                        "er4096"],
                     default=None,
                     help="run only that measurement and print its JSON object (for profiling)")
-    ap.add_argument("--no-tune", action="store_true", help="batched run with default options only")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default options only: no autotuning of the single-source or batched runs")
     ap.add_argument("--grid-loops", default="chained,graph,persistent_1024x64,host",
                     help="level loops timed by --only grid (comma-separated)")
     ap.add_argument("--no-loop", action="store_true",
@@ -705,11 +697,35 @@ def bench_single(tb, graphs, args, g, sources, gr, ctx, opts):
         ctx.run_many(gr, roots[:args.warmup] if args.warmup <= len(roots) else
                      np.resize(roots, args.warmup), opts)
 
-    # HIP events on the context's stream (the stream every BFS kernel runs on)
-    ctx.mark(0)
-    stats = ctx.run_many(gr, roots, opts)
-    ctx.mark(1)
-    total_ms = ctx.elapsed(0, 1)
+    def timed(o):
+        # HIP events on the context's stream (the stream every BFS kernel runs on)
+        ctx.mark(0)
+        st = ctx.run_many(gr, roots, o)
+        ctx.mark(1)
+        return st, ctx.elapsed(0, 1)
+
+    stats, total_ms = timed(opts)
+    tune_info = None
+    if not args.no_tune and args.mode in ("auto_chain",):
+        # the autotuner on sources the timed steps do not use (the last 8 of
+        # the sample), then the same timed steps again: runs with the
+        # tunables at 0 take the tuned values for this graph's class
+        tune_roots = np.asarray(sources[-8:], np.int32)
+        best, tms, dms = ctx.autotune(gr, tune_roots, opts)
+        ctx.run_many(gr, roots[:max(1, args.warmup)], opts)
+        stats_t, total_t = timed(opts)
+        tune_info = {
+            "tuning_sources": len(tune_roots),
+            "best": {"alpha": best.alpha, "beta": best.beta, "stage_ratio": best.stage_ratio,
+                     "lb_grid": best.lb_grid},
+            "tuning_ms_per_bfs": {"best": tms, "defaults": dms},
+            "timed_ms_per_step": {"tuned": total_t / args.steps,
+                                  "defaults": total_ms / args.steps},
+            "gain": total_ms / total_t,
+            "value_from": "tuned" if total_t < total_ms else "defaults",
+        }
+        if total_t < total_ms:
+            stats, total_ms = stats_t, total_t
     edges = sum(st["edges"] for st in stats)
     kern_ms = sum(st["ms"] for st in stats)
     levels = [st["levels"] for st in stats]
@@ -750,6 +766,7 @@ def bench_single(tb, graphs, args, g, sources, gr, ctx, opts):
                                                           (st["edges"] / 2) for st in stats)),
             "in_degree": {k: v for k, v in gr.degree_stats("in").items() if k != "bins"},
         },
+        "autotune": tune_info,
         "copy_kernel_gbs": (2 * calib_bytes / (copy_ms * 1e-3) / 1e9) if copy_ms else None,
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
