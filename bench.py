@@ -524,6 +524,7 @@ def _bench_serving(tb, cs, names, gs, seconds, rates):
                 "offered_qps": rate, "queries": count, "achieved_qps": count / span,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                 "mean_batch": float(np.mean([d["batch"] for d in done])),
+                "admitted_mid_batch": int(sum(1 for d in done if d["level"] > 0)),
                 "per_graph": per_graph,
             })
         return out
@@ -535,7 +536,7 @@ def _bench_serving(tb, cs, names, gs, seconds, rates):
     for p, f in zip(batched["poisson"], rates):
         p["offered_fraction"] = f
     # the per-source loop at the lowest offered rate (and saturated)
-    loop_srv = tb.Server(gs, slots=1, max_batch=1)
+    loop_srv = tb.Server(gs, slots=1, max_batch=1, open_depth=-1)
     loop = run(loop_srv, 256, [batched["poisson"][0]["offered_qps"]])
     loop_srv.close()
     # the sampled served results against the single-source engine (tbfs_bfs:

@@ -345,12 +345,13 @@ int tbfs_batch_read(tbfs_batch *b, int32_t first, int32_t count, int32_t n,
 
 /* ---- serving -------------------------------------------------------------
  * A server keeps a set of graphs resident and answers single-source queries
- * submitted at any time.  A dispatcher thread batches the queries waiting for
- * a graph (continuous batching: queries that arrive while the device is busy
- * join the next batch formed for their graph) into batch slots whose device
- * memory comes from mem_budget; slots run concurrently on their own streams.
- * Every query's S / sigma / overflow flags are exactly those of tbfs_bfs from
- * its root, whatever batch it ran in. */
+ * submitted at any time.  Each of its slots (batch state in device memory
+ * from mem_budget, a stream and a worker thread) takes the queries waiting
+ * for one graph and runs them as single-source BFS, as one batch, or -- on
+ * deep graphs -- as a continuous batch that queries of that graph join at
+ * level boundaries while it runs.  Slots run concurrently.  Every query's
+ * S / sigma / overflow flags are exactly those of tbfs_bfs from its root,
+ * however it ran. */
 typedef struct tbfs_server tbfs_server;
 
 typedef struct tbfs_server_opts {
@@ -366,7 +367,14 @@ typedef struct tbfs_server_opts {
                          counted: they belong to the graphs */
   int32_t slots;      /* batches in flight at once (0: 4) */
   int32_t max_batch;  /* sources per batch, <= 1024 (0: 1024) */
-  int32_t reserved[4];
+  int32_t open_depth; /* graphs on which a BFS has needed more levels than
+                         this run as continuous batches: queries that arrive
+                         while one runs join it at the next level boundary
+                         (0: 64; -1: never) */
+  int32_t short_slots; /* slots kept from long jobs (deep graphs, single-
+                         source queries over 5 ms) for short queries (0: 1
+                         when slots >= 2; -1: none) */
+  int32_t reserved[2];
 } tbfs_server_opts;
 
 typedef struct tbfs_query_done {
@@ -374,10 +382,14 @@ typedef struct tbfs_query_done {
   int32_t graph;
   int32_t root;
   int32_t status;    /* TBFS_OK or an error code */
-  int32_t batch;     /* sources in the batch that answered it */
-  double submit_us;  /* times on the server's clock (microseconds since */
-  double start_us;   /* creation): submitted, its batch started, its     */
+  int32_t batch;     /* sources in the batch that answered it (a continuous
+                        batch: those running in it when it finished) */
+  double submit_us;  /* times on the server clock (microseconds since */
+  double start_us;   /* creation): submitted, its run started, its      */
   double done_us;    /* results were in its host buffers               */
+  int32_t level;     /* level of its continuous batch at which it was
+                        seeded (0 for every other run) */
+  int32_t pad;
 } tbfs_query_done;
 
 /* graphs[i] stay owned by the caller and must outlive the server. */

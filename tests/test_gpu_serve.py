@@ -142,3 +142,95 @@ def test_serve_invalid(gpu):
     srv.close()
     with pytest.raises(tb.TbfsError):
         tb.Server(gs, mem_budget=1000)   # not even one word of the largest graph
+
+
+def _stream(srv, cs, plan, gap_s, store):
+    """Submit (graph, root) pairs `gap_s` seconds apart."""
+    t0 = time.perf_counter()
+    for i, (gi, r) in enumerate(plan):
+        while time.perf_counter() - t0 < i * gap_s:
+            pass
+        _submit(srv, cs[gi], gi, r, store)
+
+
+def test_serve_continuous_batch_admits_mid_batch(gpu):
+    """A deep graph (256 x 256 grid, 10% of edges removed: ~500 levels) runs
+    as a continuous batch; queries arriving while it runs are seeded at a
+    later level boundary (level > 0) and their rows are still exactly the
+    oracle's, and equal to the same root's run alone."""
+    cs = [graphs.grid(256, 256, 0.1, seed=4, relabel=True),
+          graphs.rmat(12, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)]
+    gs = [tb.Graph(g.CP, g.IC, symmetric=True) for g in cs]
+    srv = tb.Server(gs, mem_budget=1 << 30, slots=2)
+    rng = np.random.default_rng(11)
+    roots = [int(x) for x in rng.integers(0, cs[0].n, size=40)]
+    # alone: the first query learns the graph's depth (a batch of one), the
+    # next ones run alone in continuous batches
+    alone = {}
+    for r in roots[:3]:
+        store = {}
+        t = _submit(srv, cs[0], 0, r, store)
+        srv.drain()
+        (d,) = srv.poll(10)
+        assert d["ticket"] == t and d["status"] == 0
+        alone[r] = store[t][2:]
+    print("[serve] continuous batch: 40 grid queries 0.4 ms apart, R-MAT between", flush=True)
+    plan = []
+    for i, r in enumerate(roots):
+        plan.append((0, r))
+        if i % 4 == 0:
+            plan.append((1, int(rng.integers(0, cs[1].n))))
+    store = {}
+    _stream(srv, cs, plan, 0.0004, store)
+    srv.drain()
+    done = srv.poll(1000)
+    assert len(done) == len(plan)
+    _check(cs, store, done)
+    mid = [d for d in done if d["graph"] == 0 and d["level"] > 0]
+    print(f"[serve] admitted mid-batch: {len(mid)} (levels {sorted(d['level'] for d in mid)[:8]}...)",
+          flush=True)
+    assert len(mid) >= 5
+    compared = 0
+    for d in done:
+        gi, r = store[d["ticket"]][:2]
+        if gi == 0 and r in alone:
+            for a, b in zip(alone[r], store[d["ticket"]][2:]):
+                assert np.array_equal(a, b)
+            compared += d["level"] > 0
+    # roots 0..2 ran alone and again in the stream; at least one of the
+    # repeats was seeded mid-batch
+    assert compared >= 1
+    assert all(d["level"] == 0 for d in done if d["graph"] == 1)
+    srv.close()
+
+
+def test_serve_continuous_batches_on_every_graph(gpu):
+    """open_depth=1 makes every graph run as continuous batches after its
+    first query: pull levels, the degree-ordered copy of the R-MAT graph,
+    directed graphs and counts past 2^64 (48 x 48 grid) all go through
+    admission at level boundaries; every row matches the oracle."""
+    cs, gs = _graph_set()
+    srv = tb.Server(gs, mem_budget=1 << 30, slots=3, open_depth=1)
+    rng = np.random.default_rng(21)
+    store = {}
+    # one query per graph first (learns the depths), then a stream
+    for gi in range(len(cs)):
+        _submit(srv, cs[gi], gi, 0, store)
+    srv.drain()
+    first = srv.poll(100)
+    _check(cs, store, first)
+    store = {}
+    plan = []
+    for _ in range(300):
+        gi = int(rng.integers(0, len(cs)))
+        plan.append((gi, int(rng.integers(0, cs[gi].n))))
+    print("[serve] continuous batches on every graph: 300 queries", flush=True)
+    _stream(srv, cs, plan, 0.0001, store)
+    srv.drain()
+    done = srv.poll(1000)
+    assert len(done) == 300
+    _check(cs, store, done)
+    levels = [d["level"] for d in done]
+    print(f"[serve] seeded mid-batch: {sum(1 for x in levels if x > 0)} of 300", flush=True)
+    assert any(x > 0 for x in levels)
+    srv.close()

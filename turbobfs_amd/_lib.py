@@ -88,7 +88,9 @@ class ServerOpts(C.Structure):
         ("mem_budget", C.c_int64),
         ("slots", C.c_int32),
         ("max_batch", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("open_depth", C.c_int32),
+        ("short_slots", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -102,6 +104,8 @@ class QueryDone(C.Structure):
         ("submit_us", C.c_double),
         ("start_us", C.c_double),
         ("done_us", C.c_double),
+        ("level", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
 

@@ -433,16 +433,21 @@ def server_budget_for(graphs, slots: int, words: int) -> int:
 
 
 class Server(_Handle):
-    """Resident graphs + a native dispatcher that batches queries as they
-    arrive (continuous batching over concurrent batch slots)."""
+    """Resident graphs + native worker threads, one per batch slot, that run
+    queries as they arrive: single-source runs, batches, or (deep graphs,
+    more levels than open_depth) continuous batches that later queries join
+    at level boundaries.  short_slots slots stay free of long jobs."""
 
-    def __init__(self, graphs, mem_budget: int = 0, slots: int = 0, max_batch: int = 0):
+    def __init__(self, graphs, mem_budget: int = 0, slots: int = 0, max_batch: int = 0,
+                 open_depth: int = 0, short_slots: int = 0):
         self._graphs = list(graphs)  # keep the graphs alive
         arr = (C.c_void_p * len(self._graphs))(*[g.handle for g in self._graphs])
         o = _lib.ServerOpts()
         o.mem_budget = int(mem_budget)
         o.slots = int(slots)
         o.max_batch = int(max_batch)
+        o.open_depth = int(open_depth)
+        o.short_slots = int(short_slots)
         self._h = C.c_void_p()
         check(_lib.load().tbfs_server_create(arr, len(self._graphs), C.byref(o), C.byref(self._h)),
               "tbfs_server_create")

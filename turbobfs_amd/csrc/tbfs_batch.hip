@@ -80,6 +80,7 @@ constexpr int kBatchMaxW = 16;    // words per vertex: up to 1024 sources
 constexpr float kBatchAlpha = 32.0f;       // default push / pull switch
 constexpr float kBatchAlphaShort = 4.0f;   // the same for short in-lists
 constexpr int kBatchWideLevels = 16;  // levels on the full-occupancy grid
+constexpr int kOpenRing = 4;          // chunk probes of a continuous batch in flight
 constexpr int kBatchLbDegree = 12;    // mean in-degree below which pulls share passes  // default push / pull switch
 
 // Level counters, striped over kStripes cache lines like the single-source
@@ -139,6 +140,7 @@ struct BatchParams {
   RunStats* stats;
   const int32_t* roots;
   const int32_t* perm;  // caller's vertex id -> id in the graph run (null: same)
+  int32_t open;         // continuous batch: an empty frontier does not end the loop
 };
 
 __device__ __forceinline__ uint64_t rl64(uint64_t x, int l) {
@@ -727,7 +729,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int3
   st.reached += bits;
   st.edges += edges;
   st.ovf_any |= ovf;
-  if (q == 0) {
+  if (q == 0 && !p.open) {
     st.done = 1;
     if (gtid == 0) {
       p.loop[(d + 1) & 1] = st;
@@ -849,6 +851,140 @@ __global__ __launch_bounds__(256) void k_batch_extract(
   }
 }
 
+// ---- continuous batches ------------------------------------------------------
+// A continuous batch keeps running levels while sources come and go: a new
+// source takes a free bit j at a level boundary d0 (its root enters the
+// frontier of depth d0 with depth d0 and count 1), so the level kernels write
+// d0 + its depth into S; a source whose bit is absent from a whole frontier
+// is finished, its row is read out with d0 subtracted and its column cleared
+// (seen bits, overflow bits, counts) so that the bit can take the next
+// source.  Counts are 64-bit throughout.  Up to 64 sources per launch.
+struct OpenSeeds {
+  int32_t bit[64];
+  int32_t root[64];
+};
+struct OpenRows {
+  int32_t bit[64];
+  int32_t off[64];      // level at which the source was seeded
+  uint64_t wmask[16];   // per word: the bits of this launch
+};
+// What the host reads at a chunk boundary d: the sources present in the
+// frontier of depth d (per word) and that frontier's word count.
+struct alignas(256) OpenProbe {
+  uint64_t alive[16];
+  unsigned long long q;
+  int32_t d;
+  int32_t pad;
+};
+
+// Seed count sources at boundary d (after level d - 1, before level d).
+__global__ __launch_bounds__(64) void k_batch_open_seed(BatchParams p, int32_t d, OpenSeeds sd,
+                                                        int32_t count) {
+  const int t = threadIdx.x;
+  if (t >= count) return;
+  const int j = sd.bit[t];
+  const int r = p.perm ? p.perm[sd.root[t]] : sd.root[t];
+  const int64_t iw = (int64_t)(j >> 6) * p.n + r;
+  const unsigned long long old =
+      atomicOr((unsigned long long*)&p.F[d % 3][iw], (unsigned long long)(1ull << (j & 63)));
+  p.S[iw * 64 + (j & 63)] = d;
+  ((uint64_t*)p.sigma)[iw * 64 + (j & 63)] = 1ull;
+  BStripe& c = p.cset[d % 3].s[t & (kStripes - 1)];
+  if (old == 0) {
+    atomicAdd(&c.q, 1ull);
+    atomicAdd(&c.mout, (unsigned long long)(p.RP[r + 1] - p.RP[r]));
+    p.lst[d % 3][atomicAdd(&p.cset[d % 3].s[0].tail, 1u)] = (uint32_t)iw;
+  }
+  atomicAdd(&c.bits, 1ull);
+  atomicAdd(&c.edges, (unsigned long long)(p.CP[r + 1] - p.CP[r]));
+}
+
+// The probe of boundary d into out (cleared beforehand); clears next (the
+// probe slot of the following boundary).  The frontier's words come from its
+// word list when that is complete, else from a scan of every word.
+template <int W>
+__global__ __launch_bounds__(256) void k_batch_open_probe(BatchParams p, int32_t d,
+                                                          OpenProbe* out, OpenProbe* next) {
+  __shared__ unsigned long long acc[W];
+  if (threadIdx.x < W) acc[threadIdx.x] = 0ull;
+  __syncthreads();
+  const BLoop st = p.loop[d & 1];
+  unsigned long long q = 0;
+#pragma unroll
+  for (int k = 0; k < kStripes; ++k) q += p.cset[d % 3].s[k].q;
+  const uint64_t* F = p.F[d % 3];
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  const bool listed = st.list_cur != 0;
+  const int64_t span = listed ? (int64_t)q : (int64_t)p.n * W;
+  const uint32_t* lst = p.lst[d % 3];
+  for (int64_t k = gtid; k < span; k += gstride) {
+    const int64_t i = listed ? (int64_t)lst[k] : k;
+    const uint64_t f = F[i];
+    if (f) atomicOr(&acc[i / p.n], (unsigned long long)f);
+  }
+  __syncthreads();
+  if (threadIdx.x < W && acc[threadIdx.x])
+    atomicOr((unsigned long long*)&out->alive[threadIdx.x], acc[threadIdx.x]);
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < 16) next->alive[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) {
+      out->q = q;
+      out->d = d;
+      next->q = 0ull;
+    }
+  }
+}
+
+// Rows of finished sources (bits rows.bit[0..count)) out of the [n][64W]
+// layout, depth minus the seeding level, then their columns cleared: seen
+// bits in both slots, overflow bits, counts.  `seen` holds the OR of every
+// frontier so far.  A 64-vertex tile per workgroup; perm as k_batch_extract.
+__global__ __launch_bounds__(256) void k_batch_open_extract(
+    int32_t* __restrict__ S, uint64_t* __restrict__ sigma, uint64_t* __restrict__ ovfw,
+    uint64_t* __restrict__ seen, uint64_t* __restrict__ seen2, int32_t n, int32_t W, OpenRows rows,
+    int32_t count, int32_t* So, uint64_t* sgo, uint8_t* ovo, const int32_t* __restrict__ perm) {
+  __shared__ int32_t ts[64][65];
+  __shared__ uint64_t tg[64][65];
+  __shared__ uint8_t to[64][65];
+  const int64_t v0 = (int64_t)blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {  // r: vertex row of the tile, tx: source
+    const int64_t v = v0 + r;
+    if (v < n && tx < count) {
+      const int j = rows.bit[tx];
+      const int64_t iw = (int64_t)(j >> 6) * n + (perm ? perm[v] : v);
+      const int64_t ix = iw * 64 + (j & 63);
+      const bool reached = (seen[iw] >> (j & 63)) & 1ull;
+      ts[r][tx] = reached ? S[ix] - rows.off[tx] : -1;
+      tg[r][tx] = reached ? sigma[ix] : 0ull;
+      to[r][tx] = reached ? (uint8_t)((ovfw[iw] >> (j & 63)) & 1ull) : (uint8_t)0;
+      if (reached) sigma[ix] = 0ull;
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < count; r += 4) {  // r: source, tx: vertex
+    const int64_t v = v0 + tx;
+    if (v < n) {
+      So[(int64_t)r * n + v] = ts[tx][r];
+      sgo[(int64_t)r * n + v] = tg[tx][r];
+      ovo[(int64_t)r * n + v] = to[tx][r];
+    }
+  }
+  // the columns' bits (every read of this tile is done)
+  for (int k = threadIdx.x; k < 64 * W; k += blockDim.x) {
+    const int w = k >> 6;
+    const int64_t v = v0 + (k & 63);
+    const uint64_t m = rows.wmask[w];
+    if (m && v < n) {
+      const int64_t iw = (int64_t)w * n + (perm ? perm[v] : v);
+      seen[iw] &= ~m;
+      seen2[iw] &= ~m;
+      ovfw[iw] &= ~m;
+    }
+  }
+}
+
 }  // namespace
 }  // namespace tbfs
 
@@ -895,6 +1031,14 @@ struct tbfs_batch {
   RunStats rs{};
   float ms = 0.f;
   const uint64_t* seen_final = nullptr;  // reached bits of the finished batch
+  // continuous batch: probes of the chunk boundaries (device, pinned host
+  // copies, their events), chunks enqueued and chunks whose probe was read
+  bool open = false;
+  OpenProbe* probe_d = nullptr;
+  OpenProbe* probe_h = nullptr;
+  hipEvent_t probe_ev[kOpenRing] = {};
+  int64_t chunk_head = 0;
+  int64_t chunk_tail = 0;
 };
 
 namespace tbfs {
@@ -922,6 +1066,10 @@ void batch_free(tbfs_batch* b) {
   dev_free(b->xsig);
   dev_free(b->xovf);
   dev_free(b->xperm);
+  dev_free(b->probe_d);
+  if (b->probe_h) (void)hipHostFree(b->probe_h);
+  for (int i = 0; i < kOpenRing; ++i)
+    if (b->probe_ev[i]) (void)hipEventDestroy(b->probe_ev[i]);
   if (b->stats_h) (void)hipHostFree(b->stats_h);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev_step) (void)hipEventDestroy(b->ev_step);
@@ -1030,26 +1178,31 @@ BatchParams make_bparams(const tbfs_batch* b) {
   p.stats = b->stats;
   p.roots = b->roots;
   p.perm = b->use_perm ? b->xperm : nullptr;
+  p.open = b->open ? 1 : 0;
   return p;
 }
 
 // Enqueue (init + seed, when d0 == 0) and levels d0 .. d0+levels-1 of the
 // current batch, then the stats copy and the step event.
+// The full-occupancy level grid: about two 64-item chunks per wavefront of
+// the pull (more workgroups than fit at once: those that finish early make
+// room for the rest), between one and 32 workgroups per CU, a multiple of 8
+// (XCD mapping).
+void set_grid(tbfs_batch* b, const BatchParams& p) {
+  if (b->grid) return;
+  const int64_t chunks = (int64_t)b->W * (((int64_t)p.nslices + 63) >> 6);
+  const int64_t cus = std::max(1, b->cus);
+  int64_t gw = (chunks + 2 * (kBatchBlock / 64) - 1) / (2 * (kBatchBlock / 64));
+  gw = std::min(std::max(gw, cus), cus * 32);
+  gw = (gw + 7) & ~(int64_t)7;
+  b->grid = b->o.grid > 0 ? std::min(b->o.grid, 65536) : (int)gw;
+}
+
 template <int W, typename CT>
 int enqueue_w(tbfs_batch* b, int d0, int levels) {
   const BatchParams p = make_bparams(b);
   hipStream_t s = b->stream;
-  if (b->grid == 0) {
-    // about two 64-item chunks per wavefront of the pull (more workgroups
-    // than fit at once: those that finish early make room for the rest),
-    // between one and 32 workgroups per CU, a multiple of 8 (XCD mapping)
-    const int64_t chunks = (int64_t)W * (((int64_t)p.nslices + 63) >> 6);
-    const int64_t cus = std::max(1, b->cus);
-    int64_t gw = (chunks + 2 * (kBatchBlock / 64) - 1) / (2 * (kBatchBlock / 64));
-    gw = std::min(std::max(gw, cus), cus * 32);
-    gw = (gw + 7) & ~(int64_t)7;
-    b->grid = b->o.grid > 0 ? std::min(b->o.grid, 65536) : (int)gw;
-  }
+  set_grid(b, p);
   if (d0 == 0) {
     const int igrid = grid_for((int64_t)p.n * W, 256, std::max(1, b->cus) * 8);
     hipLaunchKernelGGL((k_batch_init<W, CT>), dim3(igrid), dim3(256), 0, s, p);
@@ -1169,6 +1322,7 @@ int batch_start(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32
 // results are complete on the device (b->rs, b->ms filled).
 int batch_progress(tbfs_batch* b, bool* finished) {
   *finished = false;
+  if (b->open) return TBFS_ERR_INVALID;  // continuous batches report by probes
   if (!b->active) {
     *finished = b->done;
     return b->done ? TBFS_OK : TBFS_ERR_INVALID;
@@ -1242,26 +1396,56 @@ int batch_run(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t 
 // into per-source host destinations (row i -> S_dst[i], sig_dst[i],
 // ovf_dst[i]; any may be null), through a device staging buffer of bounded
 // size; ends with the step event recorded (nothing waits here).
+namespace {
+
+// Device staging for rows on their way to the host: up to 64 rows of n
+// entries, at most 256 MiB; *rows = the rows it holds.
+int ensure_staging(tbfs_batch* b, int32_t n, int* rows) {
+  const int64_t stage_bytes = (int64_t)256 << 20;
+  *rows = (int)std::max<int64_t>(1, std::min<int64_t>(64, stage_bytes / ((int64_t)n * 13)));
+  const int64_t need = (int64_t)*rows * n;
+  if (b->xcap >= need) return TBFS_OK;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  dev_free(b->xS);
+  dev_free(b->xsig);
+  dev_free(b->xovf);
+  b->xcap = 0;
+  int rc = dev_alloc(&b->xS, (size_t)need);
+  if (!rc) rc = dev_alloc(&b->xsig, (size_t)need);
+  if (!rc) rc = dev_alloc(&b->xovf, (size_t)need);
+  if (rc) return rc;
+  b->xcap = need;
+  return TBFS_OK;
+}
+
+// Copies of staged rows 0 .. c-1 to the destinations of rows f .. f+c-1.
+int copy_rows(tbfs_batch* b, int32_t n, int32_t f, int32_t c, int32_t* const* S_dst,
+              uint64_t* const* sig_dst, uint8_t* const* ovf_dst) {
+  hipStream_t s = b->stream;
+  for (int32_t r = 0; r < c; ++r) {
+    const size_t off = (size_t)r * n;
+    if (S_dst && S_dst[f + r])
+      HIPCHK(hipMemcpyAsync(S_dst[f + r], b->xS + off, (size_t)n * sizeof(int32_t),
+                            hipMemcpyDeviceToHost, s));
+    if (sig_dst && sig_dst[f + r])
+      HIPCHK(hipMemcpyAsync(sig_dst[f + r], b->xsig + off, (size_t)n * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, s));
+    if (ovf_dst && ovf_dst[f + r])
+      HIPCHK(hipMemcpyAsync(ovf_dst[f + r], b->xovf + off, (size_t)n, hipMemcpyDeviceToHost, s));
+  }
+  return TBFS_OK;
+}
+
+}  // namespace
+
 int batch_enqueue_rows(tbfs_batch* b, int32_t first, int32_t count, int32_t* const* S_dst,
                        uint64_t* const* sig_dst, uint8_t* const* ovf_dst) {
   if (!b->done || first < 0 || count < 0 || first + count > b->count) return TBFS_ERR_INVALID;
   const int32_t n = b->n;
-  const int64_t stage_bytes = (int64_t)256 << 20;
-  const int rows = (int)std::max<int64_t>(1, std::min<int64_t>(64, stage_bytes / ((int64_t)n * 13)));
-  const int64_t need = (int64_t)rows * n;
+  int rows = 0;
+  int rc = ensure_staging(b, n, &rows);
+  if (rc) return rc;
   hipStream_t s = b->stream;
-  if (b->xcap < need) {
-    HIPCHK(hipStreamSynchronize(s));
-    dev_free(b->xS);
-    dev_free(b->xsig);
-    dev_free(b->xovf);
-    b->xcap = 0;
-    int rc = dev_alloc(&b->xS, (size_t)need);
-    if (!rc) rc = dev_alloc(&b->xsig, (size_t)need);
-    if (!rc) rc = dev_alloc(&b->xovf, (size_t)need);
-    if (rc) return rc;
-    b->xcap = need;
-  }
   for (int32_t f = 0; f < count; f += rows) {
     const int32_t c = std::min(rows, count - f);
     if (b->ct == 4)
@@ -1273,17 +1457,8 @@ int batch_enqueue_rows(tbfs_batch* b, int32_t first, int32_t count, int32_t* con
                          (const uint64_t*)b->sigma, b->ovfw, b->seen_final, n, first + f, c,
                          b->xS, b->xsig, b->xovf, b->use_perm ? b->xperm : nullptr);
     HIPCHK(hipGetLastError());
-    for (int32_t r = 0; r < c; ++r) {
-      const size_t off = (size_t)r * n;
-      if (S_dst && S_dst[f + r])
-        HIPCHK(hipMemcpyAsync(S_dst[f + r], b->xS + off, (size_t)n * sizeof(int32_t),
-                              hipMemcpyDeviceToHost, s));
-      if (sig_dst && sig_dst[f + r])
-        HIPCHK(hipMemcpyAsync(sig_dst[f + r], b->xsig + off, (size_t)n * sizeof(uint64_t),
-                              hipMemcpyDeviceToHost, s));
-      if (ovf_dst && ovf_dst[f + r])
-        HIPCHK(hipMemcpyAsync(ovf_dst[f + r], b->xovf + off, (size_t)n, hipMemcpyDeviceToHost, s));
-    }
+    rc = copy_rows(b, n, f, c, S_dst, sig_dst, ovf_dst);
+    if (rc) return rc;
   }
   HIPCHK(hipEventRecord(b->ev_step, s));
   return TBFS_OK;
@@ -1306,6 +1481,211 @@ int batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* 
   int rc = batch_enqueue_rows(b, first, count, sd.data(), gd.data(), od.data());
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(b->stream));
+  return TBFS_OK;
+}
+
+// ---- continuous batches (host side) -------------------------------------------
+namespace {
+
+template <int W>
+int open_levels_w(tbfs_batch* b, int levels, bool wide) {
+  const BatchParams p = make_bparams(b);
+  hipStream_t s = b->stream;
+  // narrow levels (one workgroup per CU) unless the last probed frontier
+  // was large: a continuous batch lives mostly in thin tails
+  const int gd = wide ? b->grid : std::min(b->grid, std::max(1, b->cus));
+  for (int i = 0; i < levels; ++i) {
+    const int d = b->d_next + i;
+    if (p.pull_lb)
+      hipLaunchKernelGGL((k_batch_level<W, uint64_t, true>), dim3(gd), dim3(kBatchBlock), 0, s,
+                         p, d);
+    else
+      hipLaunchKernelGGL((k_batch_level<W, uint64_t, false>), dim3(gd), dim3(kBatchBlock), 0, s,
+                         p, d);
+    HIPCHK(hipGetLastError());
+  }
+  const int d = b->d_next + levels;
+  const int slot = (int)(b->chunk_head % kOpenRing);
+  const int pgrid = grid_for((int64_t)p.n * W, 256, std::max(1, b->cus) * 2);
+  hipLaunchKernelGGL((k_batch_open_probe<W>), dim3(pgrid), dim3(256), 0, s, p, d,
+                     b->probe_d + slot, b->probe_d + (slot + 1) % kOpenRing);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->probe_h + slot, b->probe_d + slot, sizeof(OpenProbe),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(b->probe_ev[slot], s));
+  ++b->chunk_head;
+  b->d_next = d;
+  return TBFS_OK;
+}
+
+}  // namespace
+
+int batch_open_begin(tbfs_batch* b, const tbfs_graph* gc, int32_t words, const tbfs_opts* o_in) {
+  if (!b || !gc || (words != 1 && words != 2 && words != 4 && words != 8 && words != 16))
+    return TBFS_ERR_INVALID;
+  if (b->active) return TBFS_ERR_INVALID;
+  tbfs_graph* g = const_cast<tbfs_graph*>(gc);
+  if (!g->CP) return TBFS_ERR_UNSUPPORTED;
+  if ((int64_t)g->n * words > b->cap_words) return TBFS_ERR_INVALID;
+  tbfs_opts o;
+  if (o_in) o = *o_in; else tbfs_opts_default(&o);
+  if (o.batch_order < -1 || o.batch_order > 1) return TBFS_ERR_INVALID;
+  int rc = TBFS_OK;
+  if (!b->probe_d) {
+    rc = dev_alloc(&b->probe_d, kOpenRing);
+    if (rc) return rc;
+    HIPCHK(hipHostMalloc((void**)&b->probe_h, kOpenRing * sizeof(OpenProbe),
+                         hipHostMallocDefault));
+    for (int i = 0; i < kOpenRing; ++i)
+      HIPCHK(hipEventCreateWithFlags(&b->probe_ev[i], hipEventDisableTiming));
+  }
+  const int32_t* pm = nullptr;
+  {
+    tbfs_graph* gv = nullptr;
+    rc = graph_batch_view(g, o.batch_order, b->stream, &gv, &pm);
+    if (rc) return rc;
+    b->use_perm = pm != nullptr;
+    if (pm && b->xperm_cap < g->n) {
+      HIPCHK(hipStreamSynchronize(b->stream));
+      dev_free(b->xperm);
+      b->xperm_cap = 0;
+      rc = dev_alloc(&b->xperm, (size_t)g->n);
+      if (rc) return rc;
+      b->xperm_cap = g->n;
+    }
+    g = gv;
+  }
+  rc = graph_ensure_batch_slices(g, b->stream);
+  if (rc) return rc;
+  b->g = g;
+  b->n = g->n;
+  b->o = o;
+  b->count = 64 * words;
+  b->W = words;
+  b->ct = 8;
+  b->grid = 0;
+  b->done = false;
+  b->seen_final = nullptr;
+  b->open = true;
+  b->d_next = 0;
+  b->chunk_head = 0;
+  b->chunk_tail = 0;
+  hipStream_t s = b->stream;
+  if (pm)
+    HIPCHK(hipMemcpyAsync(b->xperm, pm, (size_t)g->n * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                          s));
+  HIPCHK(hipMemsetAsync(b->probe_d, 0, kOpenRing * sizeof(OpenProbe), s));
+  const BatchParams p = make_bparams(b);
+  set_grid(b, p);
+  const int igrid = grid_for((int64_t)p.n * words, 256, std::max(1, b->cus) * 8);
+  switch (words) {
+    case 1: hipLaunchKernelGGL((k_batch_init<1, uint64_t>), dim3(igrid), dim3(256), 0, s, p); break;
+    case 2: hipLaunchKernelGGL((k_batch_init<2, uint64_t>), dim3(igrid), dim3(256), 0, s, p); break;
+    case 4: hipLaunchKernelGGL((k_batch_init<4, uint64_t>), dim3(igrid), dim3(256), 0, s, p); break;
+    case 8: hipLaunchKernelGGL((k_batch_init<8, uint64_t>), dim3(igrid), dim3(256), 0, s, p); break;
+    default: hipLaunchKernelGGL((k_batch_init<16, uint64_t>), dim3(igrid), dim3(256), 0, s, p);
+  }
+  HIPCHK(hipGetLastError());
+  b->active = true;
+  return TBFS_OK;
+}
+
+int batch_open_seed(tbfs_batch* b, const int32_t* bits, const int32_t* roots, int32_t count) {
+  if (!b || !b->open || !b->active || count < 0 || (count > 0 && (!bits || !roots)))
+    return TBFS_ERR_INVALID;
+  for (int32_t i = 0; i < count; ++i)
+    if (bits[i] < 0 || bits[i] >= b->count || roots[i] < 0 || roots[i] >= b->n)
+      return TBFS_ERR_INVALID;
+  const BatchParams p = make_bparams(b);
+  for (int32_t f = 0; f < count; f += 64) {
+    OpenSeeds sd{};
+    const int32_t c = std::min<int32_t>(64, count - f);
+    for (int32_t i = 0; i < c; ++i) {
+      sd.bit[i] = bits[f + i];
+      sd.root[i] = roots[f + i];
+    }
+    hipLaunchKernelGGL(k_batch_open_seed, dim3(1), dim3(64), 0, b->stream, p, b->d_next, sd, c);
+    HIPCHK(hipGetLastError());
+  }
+  return TBFS_OK;
+}
+
+int batch_open_extract(tbfs_batch* b, const int32_t* bits, const int32_t* offs, int32_t count,
+                       int32_t* const* S_dst, uint64_t* const* sig_dst,
+                       uint8_t* const* ovf_dst) {
+  if (!b || !b->open || !b->active || count < 0 || (count > 0 && (!bits || !offs)))
+    return TBFS_ERR_INVALID;
+  for (int32_t i = 0; i < count; ++i)
+    if (bits[i] < 0 || bits[i] >= b->count || offs[i] < 0 || offs[i] > b->d_next)
+      return TBFS_ERR_INVALID;
+  if (count == 0) return TBFS_OK;
+  const int32_t n = b->n;
+  int rows = 0;
+  int rc = ensure_staging(b, n, &rows);
+  if (rc) return rc;
+  rows = std::min(rows, 64);
+  for (int32_t f = 0; f < count; f += rows) {
+    OpenRows rw{};
+    const int32_t c = std::min(rows, count - f);
+    for (int32_t i = 0; i < c; ++i) {
+      rw.bit[i] = bits[f + i];
+      rw.off[i] = offs[f + i];
+      rw.wmask[bits[f + i] >> 6] |= 1ull << (bits[f + i] & 63);
+    }
+    hipLaunchKernelGGL(k_batch_open_extract, dim3((n + 63) / 64), dim3(256), 0, b->stream, b->S,
+                       b->sigma, b->ovfw, b->seen[b->d_next & 1], b->seen[(b->d_next + 1) & 1], n,
+                       b->W, rw, c, b->xS, b->xsig, b->xovf, b->use_perm ? b->xperm : nullptr);
+    HIPCHK(hipGetLastError());
+    rc = copy_rows(b, n, f, c, S_dst, sig_dst, ovf_dst);
+    if (rc) return rc;
+  }
+  return TBFS_OK;
+}
+
+int batch_open_chunk(tbfs_batch* b, int32_t levels, bool wide) {
+  if (!b || !b->open || !b->active || levels < 0) return TBFS_ERR_INVALID;
+  if (b->chunk_head - b->chunk_tail >= kOpenRing) return TBFS_ERR_INVALID;  // probes not read
+  if ((int64_t)b->d_next + levels >= ((int64_t)1 << 30)) return TBFS_ERR_INVALID;
+  switch (b->W) {
+    case 1: return open_levels_w<1>(b, levels, wide);
+    case 2: return open_levels_w<2>(b, levels, wide);
+    case 4: return open_levels_w<4>(b, levels, wide);
+    case 8: return open_levels_w<8>(b, levels, wide);
+    default: return open_levels_w<16>(b, levels, wide);
+  }
+}
+
+int batch_open_poll(tbfs_batch* b, bool* ready, int32_t* boundary, uint64_t* alive,
+                    int64_t* frontier_words) {
+  *ready = false;
+  if (!b || !b->open) return TBFS_ERR_INVALID;
+  if (b->chunk_tail == b->chunk_head) return TBFS_OK;
+  const int slot = (int)(b->chunk_tail % kOpenRing);
+  const hipError_t e = hipEventQuery(b->probe_ev[slot]);
+  if (e == hipErrorNotReady) return TBFS_OK;
+  if (e != hipSuccess) return hip_fail(e, "hipEventQuery", __FILE__, __LINE__);
+  const OpenProbe& pr = b->probe_h[slot];
+  *boundary = pr.d;
+  for (int w = 0; w < 16; ++w) alive[w] = w < b->W ? pr.alive[w] : 0ull;
+  if (frontier_words) *frontier_words = (int64_t)pr.q;
+  ++b->chunk_tail;
+  *ready = true;
+  return TBFS_OK;
+}
+
+int batch_open_outstanding(const tbfs_batch* b) {
+  return b && b->open ? (int)(b->chunk_head - b->chunk_tail) : 0;
+}
+
+int32_t batch_open_depth(const tbfs_batch* b) { return b ? b->d_next : 0; }
+
+int batch_open_end(tbfs_batch* b) {
+  if (!b || !b->open) return TBFS_ERR_INVALID;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->open = false;
+  b->active = false;
+  b->done = false;
+  b->chunk_tail = b->chunk_head;
   return TBFS_OK;
 }
 

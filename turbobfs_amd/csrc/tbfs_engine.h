@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <vector>
 
 #include "tbfs.h"
@@ -30,7 +31,9 @@ struct tbfs_graph {
   uint64_t* skip = nullptr;  // static: in-degree 0 or > kBigIn (one bit per vertex)
   uint64_t* bsl = nullptr;   // batched pull slices of the in-lists (v | slice << 32),
   int32_t nbsl = 0;          // built on first use
-  bool wide_counts = false;  // a batch on this graph needed 64-bit counts: start with them
+  // a batch on this graph needed 64-bit counts: start with them (batches on
+  // several threads read and set it)
+  std::atomic<bool> wide_counts{false};
   bool deg_in_valid = false; // deg_in computed (first use)
   tbfs_degree_stats deg_in{};
   // batches: a copy numbered by decreasing in-degree (hubs' state rows
@@ -208,6 +211,27 @@ int batch_progress(tbfs_batch* b, bool* finished);
 void batch_fill_stats(const tbfs_batch* b, tbfs_stats* st);
 int batch_run(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t count,
               const tbfs_opts* o, tbfs_stats* st);
+// Continuous batches (the query server): a batch of `words` 64-source words
+// whose level loop keeps running while sources come and go.  Sources are
+// seeded into free bits at the current boundary (the first level not yet
+// enqueued, batch_open_depth); batch_open_chunk enqueues `levels` levels
+// and a probe of the boundary after them; batch_open_poll returns the
+// oldest unread probe once the device has passed it: the boundary depth D,
+// per word the bits present in the frontier of depth D, and that
+// frontier's word count.  A source seeded at d0 < D whose bit is absent
+// from the probe of D is finished: batch_open_extract enqueues its row
+// (depths relative to d0) to host destinations and clears its bit for
+// reuse.  64-bit counts throughout.
+int batch_open_begin(tbfs_batch* b, const tbfs_graph* g, int32_t words, const tbfs_opts* o);
+int batch_open_seed(tbfs_batch* b, const int32_t* bits, const int32_t* roots, int32_t count);
+int batch_open_extract(tbfs_batch* b, const int32_t* bits, const int32_t* offs, int32_t count,
+                       int32_t* const* S_dst, uint64_t* const* sig_dst, uint8_t* const* ovf_dst);
+int batch_open_chunk(tbfs_batch* b, int32_t levels, bool wide);
+int batch_open_poll(tbfs_batch* b, bool* ready, int32_t* boundary, uint64_t* alive,
+                    int64_t* frontier_words);
+int batch_open_outstanding(const tbfs_batch* b);
+int32_t batch_open_depth(const tbfs_batch* b);
+int batch_open_end(tbfs_batch* b);
 int batch_enqueue_rows(tbfs_batch* b, int32_t first, int32_t count, int32_t* const* S_dst,
                        uint64_t* const* sig_dst, uint8_t* const* ovf_dst);
 int batch_read(tbfs_batch* b, int32_t first, int32_t count, int32_t n, int32_t* S,

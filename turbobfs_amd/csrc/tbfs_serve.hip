@@ -1,25 +1,28 @@
 // tbfs_serve.hip -- BFS query serving: resident graphs, continuous batching.
 //
 // A server holds a set of graphs resident in HBM and a fixed number of batch
-// slots (tbfs_batch objects, one HIP stream each) sized from a device-memory
-// budget.  Queries (graph, root, host destinations) are queued per graph as
-// they arrive.  One dispatcher thread owns every HIP call after creation:
-//   - it advances the slots in flight without blocking (event queries): a
-//     slot whose BFS finished enqueues the copy of each query's row (S,
-//     sigma, overflow flags) to that query's host buffers; when the copy
-//     event completes, the queries are reported done;
-//   - whenever a slot is idle it admits work: the graph whose oldest waiting
-//     query is oldest, with every query waiting for that graph up to the
-//     slot's capacity for it (the largest power-of-two number of 64-source
-//     words whose state fits the slot's memory, at most max_batch sources);
-//     those queries run as one batch, or one after another as single-source
-//     BFS on the slot's context when that costs less for so few queries (by
-//     the per-graph costs measured on earlier admissions).
-// Queries that arrive while slots are busy join the next batch formed for
-// their graph; slots run concurrently on their own streams, so batches of
-// different graphs (or of one graph) overlap on the device.  A query's
-// result does not depend on the batch it ran in: every source of a batch is
-// exactly its own single-source BFS.
+// slots (a tbfs_batch and a single-source context on one HIP stream each)
+// sized from a device-memory budget.  Queries (graph, root, host
+// destinations) are queued per graph as they arrive.  Each slot has a worker
+// thread of its own, so the host work of one slot (enqueueing thousands of
+// level kernels for a deep graph) never delays another.  An idle worker
+// takes the graph whose oldest waiting query is oldest, with every query
+// waiting for it up to the slot's capacity for that graph (the largest
+// power-of-two number of 64-source words whose state fits the slot's memory,
+// at most max_batch sources), and runs them
+//   - as single-source BFS on the slot's context when that costs less for so
+//     few queries than a batch (by the per-graph costs measured so far), or
+//   - as one batch (tbfs_batch), its rows copied to the queries' buffers, or
+//   - for a deep graph (a BFS on it has needed more than open_depth levels):
+//     as a continuous batch, whose level loop keeps running while sources
+//     come and go: queries of that graph that arrive while it runs are seeded
+//     into its free bits at the next level boundary (depths counted from
+//     there), finished sources are read out at chunk boundaries and their
+//     bits reused.
+// Long jobs (deep graphs, or single-source queries slower than kLongMs) hold
+// at most slots - short_slots slots at once, so short queries always find a
+// slot.  A query's result does not depend on how it ran: every source of a
+// batch is exactly its own single-source BFS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +31,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -70,17 +74,29 @@ struct Query {
   double t_submit = 0;
 };
 
+// Work a slot takes: single-source runs, one batch, or a continuous batch.
+enum JobKind { kJobSingle = 0, kJobBatch = 1, kJobOpen = 2 };
+
+struct Job {
+  int kind = kJobSingle;
+  int graph = -1;
+  int words = 0;        // continuous batches: 64-source words
+  bool long_job = false;
+  std::vector<Query> q;
+};
+
 struct Slot {
   tbfs_batch* b = nullptr;
   tbfs_ctx* c = nullptr;  // single-source runs (small batches)
-  int state = 0;  // 0 idle, 1 batch BFS running, 2 rows being copied, 3 single runs
-  int graph = -1;
-  int words = 0;  // 64-source words of the running batch (0: single runs)
-  double t_start = 0;
-  std::vector<Query> q;
-  std::vector<int32_t> roots;  // kept until the batch has finished
+  std::thread th;         // the slot's worker
+  std::vector<int32_t> roots;
   std::vector<int> redo;
 };
+
+constexpr int kDefaultOpenDepth = 64;  // deeper graphs: continuous batches
+constexpr double kLongMs = 5.0;        // single-source queries slower than this are long
+constexpr int kOpenChunk = 16;         // levels per chunk of a continuous batch
+constexpr int kOpenAhead = 2;          // chunks enqueued ahead of the device
 
 }  // namespace
 
@@ -96,22 +112,30 @@ struct tbfs_server {
   // level budget found by doubling) and is not taken as a cost sample
   std::vector<int> n_single;
   std::vector<std::array<int, 5>> n_batch;
-  std::vector<int> budget;  // level kernels per single-source run, per graph
+  std::vector<int> budget;   // level kernels per single-source run, per graph
+  std::vector<int> depth;    // most levels a BFS on the graph has needed so far
+  std::vector<int> open_free;  // free bits of the graph's running continuous batches
+  int open_depth = kDefaultOpenDepth;  // 0: no continuous batches
+  int long_cap = 1;      // slots that long jobs may hold at once
+  int long_running = 0;  // slots holding one
   int32_t nmax = 0;
   std::vector<std::deque<Query>> pending;
   std::deque<tbfs_query_done> completed;
   std::vector<Slot> slots;
-  std::mutex mu;
+  std::mutex mu;  // everything above (and the slots' job hand-over)
   std::condition_variable cv_work, cv_done;
-  std::thread th;
   std::atomic<bool> stop{false};
   int64_t in_flight = 0;  // submitted, not yet completed
   uint64_t next_ticket = 1;
   int device = 0;
   tbfs_opts bopts{};
   std::chrono::steady_clock::time_point t0;
-  // written by the dispatcher, read by tbfs_server_info on caller threads
+  // written by the workers, read by tbfs_server_info on caller threads
   std::atomic<uint64_t> batches{0}, batched{0};
+  // counts printed at destroy with TBFS_VERBOSE set (under mu)
+  uint64_t jobs[3] = {0, 0, 0};
+  uint64_t admitted_mid = 0;  // sources seeded into a running continuous batch
+  double enqueue_us = 0;      // host time spent enqueueing work
 
   double now_us() const {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0)
@@ -127,107 +151,47 @@ int word_index(int words) {
   return k;
 }
 
-void complete(tbfs_server* s, Slot& sl, int status, double t_done) {
-  if (status == TBFS_OK && sl.graph >= 0 && !sl.q.empty()) {
-    const double ms = (t_done - sl.t_start) * 1e-3;
-    const int wi = word_index(sl.words);
-    double& t = sl.words ? s->t_batch[sl.graph][wi] : s->t_single[sl.graph];
-    int& runs = sl.words ? s->n_batch[sl.graph][wi] : s->n_single[sl.graph];
-    const double x = sl.words ? ms : ms / (double)sl.q.size();
-    if (runs++ > 0) t = t > 0 ? 0.7 * t + 0.3 * x : x;
-  }
+bool is_deep(const tbfs_server* s, int g) {
+  return s->open_depth > 0 && s->depth[g] > s->open_depth;
+}
+
+bool is_long(const tbfs_server* s, int g) {
+  return is_deep(s, g) || s->t_single[g] > kLongMs;
+}
+
+// Report queries done (status, times on the server clock); the caller holds
+// no lock.
+void report(tbfs_server* s, const Query* q, size_t k, int status, int32_t batch, double t_start,
+            double t_done, const int32_t* level) {
   std::lock_guard<std::mutex> lk(s->mu);
-  for (const Query& q : sl.q) {
+  for (size_t i = 0; i < k; ++i) {
     tbfs_query_done d{};
-    d.ticket = q.ticket;
-    d.graph = q.graph;
-    d.root = q.root;
+    d.ticket = q[i].ticket;
+    d.graph = q[i].graph;
+    d.root = q[i].root;
     d.status = status;
-    d.batch = (int32_t)sl.q.size();
-    d.submit_us = q.t_submit;
-    d.start_us = sl.t_start;
+    d.batch = batch;
+    d.submit_us = q[i].t_submit;
+    d.start_us = t_start;
     d.done_us = t_done;
+    d.level = level ? level[i] : 0;
     s->completed.push_back(d);
   }
-  s->in_flight -= (int64_t)sl.q.size();
-  sl.q.clear();
-  sl.state = 0;
-  sl.graph = -1;
-  sl.words = 0;
+  s->in_flight -= (int64_t)k;
   s->cv_done.notify_all();
 }
 
-// Advance one slot; true if anything changed.
-bool step_slot(tbfs_server* s, Slot& sl) {
-  if (sl.state == 1) {
-    bool fin = false;
-    int rc = batch_progress(sl.b, &fin);
-    if (rc) {
-      complete(s, sl, rc, s->now_us());
-      return true;
-    }
-    if (!fin) return false;
-    {  // a single-source run on this graph needs at most the batch's depth
-       // in launches, plus the one that finds the frontier empty
-      tbfs_stats bst{};
-      batch_fill_stats(sl.b, &bst);
-      int& bud = s->budget[sl.graph];
-      bud = std::max(bud, std::min(1 << 16, bst.levels + 2));
-    }
-    const size_t k = sl.q.size();
-    std::vector<int32_t*> sd(k);
-    std::vector<uint64_t*> gd(k);
-    std::vector<uint8_t*> od(k);
-    for (size_t i = 0; i < k; ++i) {
-      sd[i] = sl.q[i].S;
-      gd[i] = sl.q[i].sigma;
-      od[i] = sl.q[i].ovf;
-    }
-    rc = batch_enqueue_rows(sl.b, 0, (int32_t)k, sd.data(), gd.data(), od.data());
-    if (rc) {
-      complete(s, sl, rc, s->now_us());
-      return true;
-    }
-    sl.state = 2;
-    return true;
-  }
-  if (sl.state == 2) {
-    const hipError_t e = hipEventQuery(batch_step_event(sl.b));
-    if (e == hipErrorNotReady) return false;
-    complete(s, sl, e == hipSuccess ? TBFS_OK : TBFS_ERR_HIP, s->now_us());
-    return true;
-  }
-  if (sl.state == 3) {
-    bool ready = false;
-    int maxl = 0;
-    int rc = ctx_async_poll(sl.c, (int)sl.q.size(), &ready, &sl.redo, &maxl);
-    if (rc) {
-      complete(s, sl, rc, s->now_us());
-      return true;
-    }
-    if (!ready) return false;
-    int& bud = s->budget[sl.graph];
-    // the most launches a BFS needed plus one spare (never lowered: sources
-    // of one graph differ in depth, and a run-out costs a whole BFS); twice
-    // the budget if one ran out
-    bud = std::max(bud, std::min(1 << 16, sl.redo.empty() ? maxl + 1 : 2 * bud));
-    if (sl.redo.empty()) {
-      complete(s, sl, TBFS_OK, s->now_us());
-      return true;
-    }
-    // sources that ran out of levels: again, with the larger budget
-    for (int i : sl.redo) {
-      const Query& q = sl.q[i];
-      rc = ctx_async_enqueue(sl.c, s->graphs[sl.graph], q.root, s->bopts, i, bud, q.S, q.sigma,
-                             q.ovf);
-      if (rc) break;
-    }
-    if (!rc) rc = ctx_async_finish(sl.c, (int)sl.q.size());
-    if (rc) complete(s, sl, rc, s->now_us());
-    return true;
-  }
-  return false;
+// A cost sample (ms) of a completed run; `words` 0 for single-source runs
+// (ms per query).
+void note_cost(tbfs_server* s, int g, int words, double ms) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int wi = word_index(words);
+  double& t = words ? s->t_batch[g][wi] : s->t_single[g];
+  int& runs = words ? s->n_batch[g][wi] : s->n_single[g];
+  if (runs++ > 0) t = t > 0 ? 0.7 * t + 0.3 * ms : ms;
 }
+
+void sleep_poll() { std::this_thread::sleep_for(std::chrono::microseconds(20)); }
 
 // Single-source runs or one batch: single runs while k queries cost less
 // that way than one batch of their word count, by the costs measured so far
@@ -250,100 +214,338 @@ bool use_single(tbfs_server* s, int g, size_t* k) {
   return true;
 }
 
-// Form a batch for an idle slot; true if one was started.
-bool admit(tbfs_server* s, Slot& sl) {
-  std::vector<Query> take;
+// Choose work for an idle slot (s->mu held): the graph whose oldest waiting
+// query is oldest, skipping graphs whose running continuous batches have
+// room for all their waiting queries (those join at the next level
+// boundary) and long graphs while long jobs hold s->long_cap slots (the
+// remaining slots stay free for short queries).
+bool pick(tbfs_server* s, Job* job) {
   int g = -1;
-  bool single = false;
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    double oldest = 0;
-    for (size_t i = 0; i < s->pending.size(); ++i) {
-      if (s->pending[i].empty()) continue;
-      const double t = s->pending[i].front().t_submit;
-      if (g < 0 || t < oldest) {
-        g = (int)i;
-        oldest = t;
-      }
+  double oldest = 0;
+  for (size_t i = 0; i < s->pending.size(); ++i) {
+    const std::deque<Query>& pq = s->pending[i];
+    if (pq.empty()) continue;
+    if (s->open_free[i] > 0 && (size_t)s->open_free[i] >= pq.size()) continue;
+    if (is_long(s, (int)i) && s->long_running >= s->long_cap) continue;
+    const double t = pq.front().t_submit;
+    if (g < 0 || t < oldest) {
+      g = (int)i;
+      oldest = t;
     }
-    if (g < 0) return false;
-    std::deque<Query>& pq = s->pending[g];
-    size_t k = std::min(pq.size(), (size_t)s->cap[g]);
-    single = use_single(s, g, &k);
-    take.assign(pq.begin(), pq.begin() + k);
-    pq.erase(pq.begin(), pq.begin() + k);
   }
-  if (single) {
-    sl.q = std::move(take);
-    sl.graph = g;
-    sl.words = 0;
-    sl.t_start = s->now_us();
-    int rc = TBFS_OK;
-    if (!sl.c) rc = tbfs_ctx_create(s->nmax, &sl.c);
-    if (!rc) rc = ctx_async_begin(sl.c, (int)sl.q.size());
-    for (size_t i = 0; i < sl.q.size() && !rc; ++i) {
-      const Query& q = sl.q[i];
-      rc = ctx_async_enqueue(sl.c, s->graphs[g], q.root, s->bopts, (int)i, s->budget[g], q.S,
-                             q.sigma, q.ovf);
-    }
-    if (!rc) rc = ctx_async_finish(sl.c, (int)sl.q.size());
-    if (rc) {
-      complete(s, sl, rc, s->now_us());
-      return true;
-    }
-    sl.state = 3;
-    ++s->batches;
-    s->batched += sl.q.size();
-    return true;
+  if (g < 0) return false;
+  std::deque<Query>& pq = s->pending[g];
+  job->graph = g;
+  job->long_job = is_long(s, g);
+  size_t k = std::min(pq.size(), (size_t)s->cap[g]);
+  if (is_deep(s, g)) {
+    // room for twice the waiting queries, at least one word
+    job->kind = kJobOpen;
+    job->words = batch_words_for((int32_t)std::min<size_t>((size_t)s->cap[g], std::max<size_t>(64, 2 * k)));
+    k = std::min(k, (size_t)(64 * job->words));
+  } else {
+    job->kind = use_single(s, g, &k) ? kJobSingle : kJobBatch;
+    job->words = job->kind == kJobBatch ? batch_words_for((int32_t)k) : 0;
   }
-  std::vector<int32_t>& roots = sl.roots;
-  roots.resize(take.size());
-  for (size_t i = 0; i < take.size(); ++i) roots[i] = take[i].root;
-  sl.q = std::move(take);
-  sl.graph = g;
-  sl.words = batch_words_for((int32_t)roots.size());
-  sl.t_start = s->now_us();
-  const int rc = batch_start(sl.b, s->graphs[g], roots.data(), (int32_t)roots.size(), &s->bopts);
-  if (rc) {
-    complete(s, sl, rc, s->now_us());
-    return true;
-  }
-  sl.state = 1;
+  job->q.assign(pq.begin(), pq.begin() + k);
+  pq.erase(pq.begin(), pq.begin() + k);
+  if (job->long_job) ++s->long_running;
+  ++s->jobs[job->kind];
   ++s->batches;
-  s->batched += roots.size();
+  s->batched += k;
   return true;
 }
 
-void dispatcher(tbfs_server* s) {
+// Single-source runs of job.q (one at a time on the slot's context).
+void run_single(tbfs_server* s, Slot& sl, Job& job) {
+  const int g = job.graph;
+  const tbfs_graph* gr = s->graphs[g];
+  const double t_start = s->now_us();
+  const int n = (int)job.q.size();
+  int bud;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    bud = s->budget[g];
+  }
+  int rc = ctx_async_begin(sl.c, n);
+  for (int i = 0; i < n && !rc; ++i) {
+    const Query& q = job.q[i];
+    rc = ctx_async_enqueue(sl.c, gr, q.root, s->bopts, i, bud, q.S, q.sigma, q.ovf);
+  }
+  if (!rc) rc = ctx_async_finish(sl.c, n);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->enqueue_us += s->now_us() - t_start;
+  }
+  while (!rc) {
+    bool ready = false;
+    int maxl = 0;
+    rc = ctx_async_poll(sl.c, n, &ready, &sl.redo, &maxl);
+    if (rc) break;
+    if (!ready) {
+      sleep_poll();
+      continue;
+    }
+    {
+      // the most launches a BFS needed plus one spare (never lowered:
+      // sources of one graph differ in depth, and a run-out costs a whole
+      // BFS); twice the budget if one ran out
+      std::lock_guard<std::mutex> lk(s->mu);
+      int& b = s->budget[g];
+      b = std::max(b, std::min(1 << 16, sl.redo.empty() ? maxl + 1 : 2 * bud));
+      bud = b;
+      if (sl.redo.empty()) s->depth[g] = std::max(s->depth[g], maxl - 1);
+    }
+    if (sl.redo.empty()) break;
+    // sources that ran out of levels: again, with the larger budget
+    for (int i : sl.redo) {
+      const Query& q = job.q[i];
+      rc = ctx_async_enqueue(sl.c, gr, q.root, s->bopts, i, bud, q.S, q.sigma, q.ovf);
+      if (rc) break;
+    }
+    if (!rc) rc = ctx_async_finish(sl.c, n);
+  }
+  const double t_done = s->now_us();
+  if (!rc) note_cost(s, g, 0, (t_done - t_start) * 1e-3 / n);
+  report(s, job.q.data(), job.q.size(), rc, n, t_start, t_done, nullptr);
+}
+
+// One batch of job.q; rows copied to the queries' destinations.
+void run_batch(tbfs_server* s, Slot& sl, Job& job) {
+  const int g = job.graph;
+  const double t_start = s->now_us();
+  const int32_t k = (int32_t)job.q.size();
+  sl.roots.resize(k);
+  for (int32_t i = 0; i < k; ++i) sl.roots[i] = job.q[i].root;
+  int rc = batch_start(sl.b, s->graphs[g], sl.roots.data(), k, &s->bopts);
+  while (!rc) {
+    bool fin = false;
+    rc = batch_progress(sl.b, &fin);
+    if (rc || fin) break;
+    sleep_poll();
+  }
+  if (!rc) {
+    // a single-source run on this graph needs at most the batch's depth in
+    // launches, plus the one that finds the frontier empty
+    tbfs_stats bst{};
+    batch_fill_stats(sl.b, &bst);
+    std::lock_guard<std::mutex> lk(s->mu);
+    int& bud = s->budget[g];
+    bud = std::max(bud, std::min(1 << 16, bst.levels + 2));
+    s->depth[g] = std::max(s->depth[g], bst.levels);
+  }
+  if (!rc) {
+    std::vector<int32_t*> sd(k);
+    std::vector<uint64_t*> gd(k);
+    std::vector<uint8_t*> od(k);
+    for (int32_t i = 0; i < k; ++i) {
+      sd[i] = job.q[i].S;
+      gd[i] = job.q[i].sigma;
+      od[i] = job.q[i].ovf;
+    }
+    rc = batch_enqueue_rows(sl.b, 0, k, sd.data(), gd.data(), od.data());
+  }
+  while (!rc) {
+    const hipError_t e = hipEventQuery(batch_step_event(sl.b));
+    if (e == hipSuccess) break;
+    if (e != hipErrorNotReady) rc = hip_fail(e, "hipEventQuery", __FILE__, __LINE__);
+    else sleep_poll();
+  }
+  const double t_done = s->now_us();
+  if (!rc) note_cost(s, g, batch_words_for(k), (t_done - t_start) * 1e-3);
+  report(s, job.q.data(), job.q.size(), rc, k, t_start, t_done, nullptr);
+}
+
+// A continuous batch: job.q seeded at depth 0, then chunks of levels; at
+// each chunk's probe the finished sources are read out (their results reach
+// the host before the next probe, when they are reported) and queries of the
+// graph that have arrived meanwhile take the free bits, seeded at the
+// current boundary.  Ends when no source is running and none is waiting.
+void run_open(tbfs_server* s, Slot& sl, Job& job) {
+  const int g = job.graph;
+  const int32_t nbits = 64 * job.words;
+  struct Src {
+    Query q;
+    int32_t d0 = -1;  // seeding level; -1: bit free
+    double t_start = 0;
+  };
+  std::vector<Src> src((size_t)nbits);
+  std::vector<int32_t> free_bits;
+  for (int32_t j = nbits - 1; j >= 0; --j) free_bits.push_back(j);
+  struct Done {
+    std::vector<Query> q;
+    std::vector<double> t_start;
+    std::vector<int32_t> level;
+    int64_t after;  // reported once the probe of chunk `after` has been read
+  };
+  std::deque<Done> awaiting;
+  int running = 0;
+  int64_t polled = 0;  // probes read
+  int64_t head = 0;    // chunks enqueued
+  bool wide = false;
+  int rc = batch_open_begin(sl.b, s->graphs[g], job.words, &s->bopts);
+  std::vector<int32_t> bits, roots, offs;
+  std::vector<int32_t*> sd;
+  std::vector<uint64_t*> gd;
+  std::vector<uint8_t*> od;
+  // seed queries (at the current boundary) into free bits
+  auto seed = [&](std::vector<Query>& qs) {
+    bits.clear();
+    roots.clear();
+    const int32_t d0 = batch_open_depth(sl.b);
+    const double t = s->now_us();
+    for (Query& q : qs) {
+      const int32_t j = free_bits.back();
+      free_bits.pop_back();
+      src[j].q = q;
+      src[j].d0 = d0;
+      src[j].t_start = t;
+      bits.push_back(j);
+      roots.push_back(q.root);
+    }
+    running += (int)qs.size();
+    return batch_open_seed(sl.b, bits.data(), roots.data(), (int32_t)bits.size());
+  };
+  if (!rc) rc = seed(job.q);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->open_free[g] += (int)free_bits.size();
+  }
+  if (!rc) {
+    rc = batch_open_chunk(sl.b, kOpenChunk, wide);
+    ++head;
+  }
+  uint64_t alive[16];
+  while (!rc) {
+    bool ready = false;
+    int32_t D = 0;
+    int64_t fw = 0;
+    rc = batch_open_poll(sl.b, &ready, &D, alive, &fw);
+    if (rc) break;
+    if (!ready) {
+      if (head - polled < kOpenAhead && running > 0) {
+        rc = batch_open_chunk(sl.b, kOpenChunk, wide);
+        ++head;
+      } else {
+        sleep_poll();
+      }
+      continue;
+    }
+    ++polled;
+    wide = fw > (int64_t)64 * 256;
+    const double t_now = s->now_us();
+    // results whose copies were enqueued before this probe's chunk
+    while (!awaiting.empty() && awaiting.front().after < polled) {
+      Done& dn = awaiting.front();
+      for (size_t i = 0; i < dn.q.size(); ++i)
+        report(s, &dn.q[i], 1, TBFS_OK, running + (int)dn.q.size(), dn.t_start[i], t_now,
+               &dn.level[i]);
+      awaiting.pop_front();
+    }
+    // sources finished by this probe: read out, bits freed
+    bits.clear();
+    offs.clear();
+    Done dn;
+    for (int32_t j = 0; j < nbits; ++j) {
+      Src& x = src[j];
+      if (x.d0 < 0 || x.d0 >= D || ((alive[j >> 6] >> (j & 63)) & 1ull)) continue;
+      bits.push_back(j);
+      offs.push_back(x.d0);
+      dn.q.push_back(x.q);
+      dn.t_start.push_back(x.t_start);
+      dn.level.push_back(x.d0);
+      x.d0 = -1;
+    }
+    if (!bits.empty()) {
+      sd.resize(bits.size());
+      gd.resize(bits.size());
+      od.resize(bits.size());
+      for (size_t i = 0; i < bits.size(); ++i) {
+        sd[i] = dn.q[i].S;
+        gd[i] = dn.q[i].sigma;
+        od[i] = dn.q[i].ovf;
+      }
+      rc = batch_open_extract(sl.b, bits.data(), offs.data(), (int32_t)bits.size(), sd.data(),
+                              gd.data(), od.data());
+      if (rc) break;
+      running -= (int)bits.size();
+      for (int32_t j : bits) free_bits.push_back(j);
+      dn.after = head;  // the next chunk's probe follows the copies
+      awaiting.push_back(std::move(dn));
+    }
+    // arrivals: into the free bits, at this boundary
+    std::vector<Query> add;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      std::deque<Query>& pq = s->pending[g];
+      // at most the graph's batch capacity running at once (max_batch)
+      const bool room = batch_open_depth(sl.b) < (1 << 29) - 4 * kOpenChunk &&
+                        !s->stop.load(std::memory_order_relaxed);
+      const size_t lim = (size_t)std::max(0, s->cap[g] - running);
+      const size_t k = room ? std::min({pq.size(), free_bits.size(), lim}) : 0;
+      add.assign(pq.begin(), pq.begin() + k);
+      pq.erase(pq.begin(), pq.begin() + k);
+      s->open_free[g] += (int)bits.size() - (int)k;
+      s->admitted_mid += k;
+      s->batched += k;
+    }
+    if (!add.empty()) {
+      rc = seed(add);
+      if (rc) break;
+    }
+    if (running == 0 && awaiting.empty() && head == polled) break;
+    if (head - polled < kOpenAhead && (running > 0 || !awaiting.empty())) {
+      rc = batch_open_chunk(sl.b, running > 0 ? kOpenChunk : 0, wide);
+      ++head;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->open_free[g] -= (int)free_bits.size();
+  }
+  if (rc) {  // every query still held by the batch fails
+    const double t = s->now_us();
+    for (Done& dn : awaiting) report(s, dn.q.data(), dn.q.size(), rc, 0, t, t, nullptr);
+    for (Src& x : src)
+      if (x.d0 >= 0) report(s, &x.q, 1, rc, 0, x.t_start, t, nullptr);
+  }
+  (void)batch_open_end(sl.b);
+}
+
+void worker(tbfs_server* s, Slot* sl) {
   (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
   while (!s->stop.load(std::memory_order_relaxed)) {
-    bool moved = false;
-    bool busy = false;
-    for (Slot& sl : s->slots) {
-      moved |= step_slot(s, sl);
-      if (sl.state == 0) moved |= admit(s, sl);
-      busy |= sl.state != 0;
+    Job job;
+    if (!pick(s, &job)) {
+      s->cv_work.wait_for(lk, std::chrono::milliseconds(2));
+      continue;
     }
-    if (moved) continue;
-    if (busy) {
-      std::this_thread::sleep_for(std::chrono::microseconds(20));
-    } else {
-      std::unique_lock<std::mutex> lk(s->mu);
-      bool any = false;
-      for (const auto& pq : s->pending) any |= !pq.empty();
-      if (!any && !s->stop.load())
-        s->cv_work.wait_for(lk, std::chrono::milliseconds(2));
-    }
+    lk.unlock();
+    if (job.kind == kJobSingle) run_single(s, *sl, job);
+    else if (job.kind == kJobBatch) run_batch(s, *sl, job);
+    else run_open(s, *sl, job);
+    lk.lock();
+    if (job.long_job) --s->long_running;
+    // a slot came free: another worker may now take a long job
+    s->cv_work.notify_all();
   }
 }
 
 void server_free(tbfs_server* s) {
   if (!s) return;
-  if (s->th.joinable()) {
-    s->stop.store(true);
-    s->cv_work.notify_all();
-    s->th.join();
-  }
+  s->stop.store(true);
+  s->cv_work.notify_all();
+  for (Slot& sl : s->slots)
+    if (sl.th.joinable()) sl.th.join();
+  const char* v = std::getenv("TBFS_VERBOSE");
+  if (v && v[0] && v[0] != '0')
+    std::fprintf(stderr,
+                 "turbobfs: server: %llu single-source jobs, %llu batches, %llu continuous "
+                 "batches (%llu sources admitted into running ones); %.1f ms enqueueing\n",
+                 (unsigned long long)s->jobs[kJobSingle], (unsigned long long)s->jobs[kJobBatch],
+                 (unsigned long long)s->jobs[kJobOpen], (unsigned long long)s->admitted_mid,
+                 s->enqueue_us * 1e-3);
   for (Slot& sl : s->slots) {
     if (sl.c) tbfs_ctx_destroy(sl.c);  // first: it runs on the batch's stream
     batch_free(sl.b);
@@ -361,7 +563,8 @@ int tbfs_server_create(const tbfs_graph* const* graphs, int32_t ngraphs,
   *out = nullptr;
   tbfs_server_opts so{};
   if (o) so = *o;
-  if (so.slots < 0 || so.max_batch < 0 || so.max_batch > 1024 || so.mem_budget < 0)
+  if (so.slots < 0 || so.max_batch < 0 || so.max_batch > 1024 || so.mem_budget < 0 ||
+      so.open_depth < -1 || so.short_slots < -1)
     return TBFS_ERR_INVALID;
   const int nslots = so.slots ? so.slots : 4;
   const int max_batch = so.max_batch ? so.max_batch : 1024;
@@ -392,6 +595,13 @@ int tbfs_server_create(const tbfs_graph* const* graphs, int32_t ngraphs,
   s->n_single.assign(ngraphs, 0);
   s->n_batch.assign(ngraphs, std::array<int, 5>{});
   s->budget.assign(ngraphs, 8);
+  s->depth.assign(ngraphs, 0);
+  s->open_free.assign(ngraphs, 0);
+  s->open_depth = so.open_depth == 0 ? kDefaultOpenDepth : std::max(0, so.open_depth);
+  {
+    const int keep = so.short_slots == 0 ? (nslots >= 2 ? 1 : 0) : std::max(0, so.short_slots);
+    s->long_cap = std::max(1, nslots - keep);
+  }
   s->nmax = nmax;
   for (int i = 0; i < ngraphs; ++i) {
     int w = 1;
@@ -419,7 +629,7 @@ int tbfs_server_create(const tbfs_graph* const* graphs, int32_t ngraphs,
       return rc;
     }
   }
-  s->th = std::thread(dispatcher, s);
+  for (Slot& sl : s->slots) sl.th = std::thread(worker, s, &sl);
   *out = s;
   return TBFS_OK;
 }
