@@ -56,3 +56,32 @@ def test_status_strings_and_defaults():
     assert o.mode == 0 and o.sigma == 1
     # the tunables are left at 0: the autotuner's value, else the default
     assert o.alpha == 0 and o.beta == 0 and o.stage_ratio == 0 and o.lb_grid == 0
+
+
+def test_python_structs_match_the_header(tmp_path):
+    """Every struct the binding mirrors has the header's size and field
+    offsets (a C program built from include/tbfs.h prints them)."""
+    import subprocess
+
+    from turbobfs_amd import _lib
+
+    pairs = [("tbfs_opts", _lib.Opts), ("tbfs_stats", _lib.Stats),
+             ("tbfs_server_opts", _lib.ServerOpts), ("tbfs_query_done", _lib.QueryDone),
+             ("tbfs_degree_stats", _lib.DegreeStats)]
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "tbfs.h"', "int main(void) {"]
+    for cname, py in pairs:
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", str(exe),
+                    str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(x.split()[:2]): int(x.split()[2]) for x in out if x.strip()}
+    for cname, py in pairs:
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
