@@ -359,8 +359,9 @@ typedef struct tbfs_server_opts {
                          split evenly over them; each slot's share covers its
                          batch state (828 bytes per vertex per 64-source word),
                          its staging of finished rows for the host copies (up
-                         to 64 rows of 13 bytes per vertex, at most 256 MiB)
-                         and its single-source context (~14 bytes per vertex
+                         to 64 rows of 13 bytes per vertex, at most 256 MiB),
+                         an id map (4 bytes per vertex) and its single-source
+                         context (~14 bytes per vertex
                          plus 16 bytes per 64 list entries), all sized for the
                          largest graph.  Resident graphs and their per-graph
                          tables (slices, degree-ordered copies) are not

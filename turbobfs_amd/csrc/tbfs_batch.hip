@@ -80,7 +80,7 @@ constexpr int kBatchMaxW = 16;    // words per vertex: up to 1024 sources
 constexpr float kBatchAlpha = 32.0f;       // default push / pull switch
 constexpr float kBatchAlphaShort = 4.0f;   // the same for short in-lists
 constexpr int kBatchWideLevels = 16;  // levels on the full-occupancy grid
-constexpr int kOpenRing = 4;          // chunk probes of a continuous batch in flight
+constexpr int kOpenRing = 8;          // chunk probes of a continuous batch in flight
 constexpr int kBatchLbDegree = 12;    // mean in-degree below which pulls share passes  // default push / pull switch
 
 // Level counters, striped over kStripes cache lines like the single-source
@@ -899,12 +899,12 @@ __global__ __launch_bounds__(64) void k_batch_open_seed(BatchParams p, int32_t d
   atomicAdd(&c.edges, (unsigned long long)(p.CP[r + 1] - p.CP[r]));
 }
 
-// The probe of boundary d into out (cleared beforehand); clears next (the
-// probe slot of the following boundary).  The frontier's words come from its
-// word list when that is complete, else from a scan of every word.
+// The probe of boundary d into out (cleared beforehand).  The frontier's
+// words come from its word list when that is complete, else from a scan of
+// every word.
 template <int W>
 __global__ __launch_bounds__(256) void k_batch_open_probe(BatchParams p, int32_t d,
-                                                          OpenProbe* out, OpenProbe* next) {
+                                                          OpenProbe* out) {
   __shared__ unsigned long long acc[W];
   if (threadIdx.x < W) acc[threadIdx.x] = 0ull;
   __syncthreads();
@@ -926,13 +926,27 @@ __global__ __launch_bounds__(256) void k_batch_open_probe(BatchParams p, int32_t
   __syncthreads();
   if (threadIdx.x < W && acc[threadIdx.x])
     atomicOr((unsigned long long*)&out->alive[threadIdx.x], acc[threadIdx.x]);
-  if (blockIdx.x == 0) {
-    if (threadIdx.x < 16) next->alive[threadIdx.x] = 0ull;
-    if (threadIdx.x == 0) {
-      out->q = q;
-      out->d = d;
-      next->q = 0ull;
-    }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out->q = q;
+    out->d = d;
+  }
+}
+
+// The finished probe `in` to host memory (`host`: pinned, coherent, mapped;
+// the host reads it once an event after this kernel has passed, with no
+// copy engine in between), and the next probe slot cleared.
+__global__ __launch_bounds__(64) void k_batch_open_publish(const OpenProbe* in, OpenProbe* host,
+                                                           OpenProbe* next) {
+  const int t = threadIdx.x;
+  if (t < 16) {
+    host->alive[t] = in->alive[t];
+    next->alive[t] = 0ull;
+  }
+  if (t == 0) {
+    host->q = in->q;
+    host->d = in->d;
+    host->pad = 0;
+    next->q = 0ull;
   }
 }
 
@@ -1035,7 +1049,8 @@ struct tbfs_batch {
   // copies, their events), chunks enqueued and chunks whose probe was read
   bool open = false;
   OpenProbe* probe_d = nullptr;
-  OpenProbe* probe_h = nullptr;
+  OpenProbe* probe_h = nullptr;   // pinned, coherent host memory
+  OpenProbe* probe_hd = nullptr;  // its device address
   hipEvent_t probe_ev[kOpenRing] = {};
   int64_t chunk_head = 0;
   int64_t chunk_tail = 0;
@@ -1247,6 +1262,13 @@ int enqueue(tbfs_batch* b, int d0, int levels) {
 
 }  // namespace
 
+namespace {
+int ensure_perm(tbfs_batch* b, int32_t n);
+int ensure_probes(tbfs_batch* b);
+int stage_rows(int32_t n);
+int alloc_staging(tbfs_batch* b, int64_t need);
+}  // namespace
+
 // Start a batch without waiting: roots are copied, the first chain of levels
 // is enqueued on the batch's stream.  32-bit counts first (half the bytes of
 // every count gather); a batch in which some count reaches 2^32 is run again
@@ -1285,13 +1307,9 @@ int batch_start(tbfs_batch* b, const tbfs_graph* gc, const int32_t* roots, int32
     rc = graph_batch_view(g, o.batch_order, b->stream, &gv, &pm);
     if (rc) return rc;
     b->use_perm = pm != nullptr;
-    if (pm && b->xperm_cap < g->n) {
-      HIPCHK(hipStreamSynchronize(b->stream));
-      dev_free(b->xperm);
-      b->xperm_cap = 0;
-      rc = dev_alloc(&b->xperm, (size_t)g->n);
+    if (pm) {
+      rc = ensure_perm(b, g->n);
       if (rc) return rc;
-      b->xperm_cap = g->n;
     }
     g = gv;
   }
@@ -1399,11 +1417,13 @@ int batch_run(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t 
 namespace {
 
 // Device staging for rows on their way to the host: up to 64 rows of n
-// entries, at most 256 MiB; *rows = the rows it holds.
-int ensure_staging(tbfs_batch* b, int32_t n, int* rows) {
+// entries, at most 256 MiB (at least one row).
+int stage_rows(int32_t n) {
   const int64_t stage_bytes = (int64_t)256 << 20;
-  *rows = (int)std::max<int64_t>(1, std::min<int64_t>(64, stage_bytes / ((int64_t)n * 13)));
-  const int64_t need = (int64_t)*rows * n;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(64, stage_bytes / ((int64_t)n * 13)));
+}
+
+int alloc_staging(tbfs_batch* b, int64_t need) {
   if (b->xcap >= need) return TBFS_OK;
   HIPCHK(hipStreamSynchronize(b->stream));
   dev_free(b->xS);
@@ -1417,6 +1437,37 @@ int ensure_staging(tbfs_batch* b, int32_t n, int* rows) {
   b->xcap = need;
   return TBFS_OK;
 }
+
+// *rows = the rows of n entries the staging holds (grown when needed).
+int ensure_staging(tbfs_batch* b, int32_t n, int* rows) {
+  *rows = stage_rows(n);
+  return alloc_staging(b, (int64_t)*rows * n);
+}
+
+int ensure_probes(tbfs_batch* b) {
+  if (b->probe_d) return TBFS_OK;
+  int rc = dev_alloc(&b->probe_d, kOpenRing);
+  if (rc) return rc;
+  HIPCHK(hipHostMalloc((void**)&b->probe_h, kOpenRing * sizeof(OpenProbe),
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  std::memset(b->probe_h, 0, kOpenRing * sizeof(OpenProbe));
+  HIPCHK(hipHostGetDevicePointer((void**)&b->probe_hd, b->probe_h, 0));
+  for (int i = 0; i < kOpenRing; ++i)
+    HIPCHK(hipEventCreateWithFlags(&b->probe_ev[i], hipEventDisableTiming));
+  return TBFS_OK;
+}
+
+int ensure_perm(tbfs_batch* b, int32_t n) {
+  if (b->xperm_cap >= n) return TBFS_OK;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  dev_free(b->xperm);
+  b->xperm_cap = 0;
+  const int rc = dev_alloc(&b->xperm, (size_t)n);
+  if (rc) return rc;
+  b->xperm_cap = n;
+  return TBFS_OK;
+}
+
 
 // Copies of staged rows 0 .. c-1 to the destinations of rows f .. f+c-1.
 int copy_rows(tbfs_batch* b, int32_t n, int32_t f, int32_t c, int32_t* const* S_dst,
@@ -1437,6 +1488,25 @@ int copy_rows(tbfs_batch* b, int32_t n, int32_t f, int32_t c, int32_t* const* S_
 }
 
 }  // namespace
+
+// Every buffer a batch on graphs of up to nmax vertices may grow into (row
+// staging for the largest of graphs' n, the id map, the continuous batch's
+// probes), allocated now: growing one later frees device memory, which
+// waits for every stream on the device.
+int batch_reserve(tbfs_batch* b, const int32_t* ns, int32_t count) {
+  if (!b || !ns || count < 1) return TBFS_ERR_INVALID;
+  int64_t stage = 0;
+  int32_t nmax = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    if (ns[i] < 1) return TBFS_ERR_INVALID;
+    stage = std::max(stage, (int64_t)stage_rows(ns[i]) * ns[i]);
+    nmax = std::max(nmax, ns[i]);
+  }
+  int rc = alloc_staging(b, stage);
+  if (!rc) rc = ensure_perm(b, nmax);
+  if (!rc) rc = ensure_probes(b);
+  return rc;
+}
 
 int batch_enqueue_rows(tbfs_batch* b, int32_t first, int32_t count, int32_t* const* S_dst,
                        uint64_t* const* sig_dst, uint8_t* const* ovf_dst) {
@@ -1508,10 +1578,11 @@ int open_levels_w(tbfs_batch* b, int levels, bool wide) {
   const int slot = (int)(b->chunk_head % kOpenRing);
   const int pgrid = grid_for((int64_t)p.n * W, 256, std::max(1, b->cus) * 2);
   hipLaunchKernelGGL((k_batch_open_probe<W>), dim3(pgrid), dim3(256), 0, s, p, d,
-                     b->probe_d + slot, b->probe_d + (slot + 1) % kOpenRing);
+                     b->probe_d + slot);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(b->probe_h + slot, b->probe_d + slot, sizeof(OpenProbe),
-                        hipMemcpyDeviceToHost, s));
+  hipLaunchKernelGGL(k_batch_open_publish, dim3(1), dim3(64), 0, s, b->probe_d + slot,
+                     b->probe_hd + slot, b->probe_d + (slot + 1) % kOpenRing);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(b->probe_ev[slot], s));
   ++b->chunk_head;
   b->d_next = d;
@@ -1531,27 +1602,17 @@ int batch_open_begin(tbfs_batch* b, const tbfs_graph* gc, int32_t words, const t
   if (o_in) o = *o_in; else tbfs_opts_default(&o);
   if (o.batch_order < -1 || o.batch_order > 1) return TBFS_ERR_INVALID;
   int rc = TBFS_OK;
-  if (!b->probe_d) {
-    rc = dev_alloc(&b->probe_d, kOpenRing);
-    if (rc) return rc;
-    HIPCHK(hipHostMalloc((void**)&b->probe_h, kOpenRing * sizeof(OpenProbe),
-                         hipHostMallocDefault));
-    for (int i = 0; i < kOpenRing; ++i)
-      HIPCHK(hipEventCreateWithFlags(&b->probe_ev[i], hipEventDisableTiming));
-  }
+  rc = ensure_probes(b);
+  if (rc) return rc;
   const int32_t* pm = nullptr;
   {
     tbfs_graph* gv = nullptr;
     rc = graph_batch_view(g, o.batch_order, b->stream, &gv, &pm);
     if (rc) return rc;
     b->use_perm = pm != nullptr;
-    if (pm && b->xperm_cap < g->n) {
-      HIPCHK(hipStreamSynchronize(b->stream));
-      dev_free(b->xperm);
-      b->xperm_cap = 0;
-      rc = dev_alloc(&b->xperm, (size_t)g->n);
+    if (pm) {
+      rc = ensure_perm(b, g->n);
       if (rc) return rc;
-      b->xperm_cap = g->n;
     }
     g = gv;
   }

@@ -222,6 +222,7 @@ int batch_run(tbfs_batch* b, const tbfs_graph* g, const int32_t* roots, int32_t 
 // from the probe of D is finished: batch_open_extract enqueues its row
 // (depths relative to d0) to host destinations and clears its bit for
 // reuse.  64-bit counts throughout.
+int batch_reserve(tbfs_batch* b, const int32_t* ns, int32_t count);
 int batch_open_begin(tbfs_batch* b, const tbfs_graph* g, int32_t words, const tbfs_opts* o);
 int batch_open_seed(tbfs_batch* b, const int32_t* bits, const int32_t* roots, int32_t count);
 int batch_open_extract(tbfs_batch* b, const int32_t* bits, const int32_t* offs, int32_t count,

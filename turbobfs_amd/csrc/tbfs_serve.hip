@@ -54,14 +54,14 @@ constexpr int64_t kSlotBytesPerWord = 6 * 8 + 3 * 4 + 64 * (4 + 8);
 // Device bytes a slot holds besides its per-word batch state, for graphs of
 // up to nmax vertices and nzmax entries: the rows of finished queries staged
 // for their copies to the host (up to 64 rows of 13 bytes per vertex, at
-// most 256 MiB, at least one row) and the slot's single-source context
-// (depths 4, counts 8, flags 1 per vertex; five bitmaps; two work queues of
-// n + nz/64 entries of 8 bytes).
+// most 256 MiB, at least one row), the id map of a degree-ordered copy (4
+// per vertex) and the slot's single-source context (depths 4, counts 8,
+// flags 1 per vertex; five bitmaps; two work queues of n + nz/64 entries of
+// 8 bytes).
 int64_t slot_fixed_bytes(int64_t nmax, int64_t nzmax) {
-  const int64_t row = 13 * nmax;
-  const int64_t stage = row * std::max<int64_t>(1, std::min<int64_t>(64, ((int64_t)256 << 20) / row));
+  const int64_t stage = std::min<int64_t>(64 * 13 * nmax, std::max<int64_t>((int64_t)256 << 20, 13 * nmax));
   const int64_t ctx = 13 * nmax + 5 * 8 * ((nmax + 63) / 64) + 2 * 8 * (nmax + nzmax / 64 + 64);
-  return stage + ctx + ((int64_t)1 << 20);  // counters, loop state, alignment
+  return stage + 4 * nmax + ctx + ((int64_t)1 << 20);  // counters, loop state, alignment
 }
 
 struct Query {
@@ -96,7 +96,7 @@ struct Slot {
 constexpr int kDefaultOpenDepth = 64;  // deeper graphs: continuous batches
 constexpr double kLongMs = 5.0;        // single-source queries slower than this are long
 constexpr int kOpenChunk = 16;         // levels per chunk of a continuous batch
-constexpr int kOpenAhead = 2;          // chunks enqueued ahead of the device
+constexpr int kOpenAhead = 4;          // chunks enqueued ahead of the device
 
 }  // namespace
 
@@ -239,10 +239,11 @@ bool pick(tbfs_server* s, Job* job) {
   job->long_job = is_long(s, g);
   size_t k = std::min(pq.size(), (size_t)s->cap[g]);
   if (is_deep(s, g)) {
-    // room for twice the waiting queries, at least one word
+    // the graph's full capacity: every query that arrives while it runs can
+    // join (a deep BFS keeps a slot for thousands of levels whatever its
+    // width, and the width costs only at the batch's start)
     job->kind = kJobOpen;
-    job->words = batch_words_for((int32_t)std::min<size_t>((size_t)s->cap[g], std::max<size_t>(64, 2 * k)));
-    k = std::min(k, (size_t)(64 * job->words));
+    job->words = batch_words_for(s->cap[g]);
   } else {
     job->kind = use_single(s, g, &k) ? kJobSingle : kJobBatch;
     job->words = job->kind == kJobBatch ? batch_words_for((int32_t)k) : 0;
@@ -432,7 +433,7 @@ void run_open(tbfs_server* s, Slot& sl, Job& job) {
       continue;
     }
     ++polled;
-    wide = fw > (int64_t)64 * 256;
+    wide = fw > ((int64_t)1 << 22);
     const double t_now = s->now_us();
     // results whose copies were enqueued before this probe's chunk
     while (!awaiting.empty() && awaiting.front().after < polled) {
@@ -621,9 +622,15 @@ int tbfs_server_create(const tbfs_graph* const* graphs, int32_t ngraphs,
   // hardware queues (a busy stream sharing a queue with another busy one
   // waits for it)
   s->slots.resize(nslots);
+  std::vector<int32_t> ns(ngraphs);
+  for (int i = 0; i < ngraphs; ++i) ns[i] = graphs[i]->n;
   for (Slot& sl : s->slots) {
+    // everything a slot's work allocates is allocated here: growing a
+    // buffer later would free device memory, which waits for every stream
     rc = batch_create_words(cap_words, &sl.b);
+    if (!rc) rc = batch_reserve(sl.b, ns.data(), ngraphs);
     if (!rc) rc = ctx_create_on(nmax, batch_stream(sl.b), &sl.c);
+    if (!rc) rc = ctx_async_begin(sl.c, (int)kSingleMax);
     if (rc) {
       server_free(s);
       return rc;
