@@ -101,7 +101,10 @@ typedef struct tbfs_opts {
   int32_t lb_grid;     /* chained loop: workgroups of the launches that do
                           not stage the bitmap (0: about one wavefront per
                           four frontier words) */
-  int32_t reserved[1];
+  int32_t finish;      /* chained loop, AUTO mode: once at most this many
+                          in-list entries are unexplored, one workgroup runs
+                          the remaining levels (0 or -1: never; at most
+                          65536 on graphs of more than 65536 vertices) */
   /* alpha, beta, stage_ratio and lb_grid left at 0 take the autotuner's
    * values for the graph's class (tbfs_autotune) when the table holds them,
    * else alpha 14, beta 24, stage_ratio 1 and the default grid. */

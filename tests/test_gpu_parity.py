@@ -132,7 +132,15 @@ MODES = [
     ("chain_push", dict(mode=L.MODE_PUSH, pull=L.PULL_LB, loop=L.LOOP_CHAINED)),
     ("chain_pull_depth_only", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED,
                                    alpha=1e9, sigma=0)),
+    ("chain_auto_no_finisher", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED,
+                                    finish=-1)),
+    ("chain_auto_finisher_early", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED,
+                                       finish=65536)),
+    ("chain_depth_only_finisher", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_CHAINED,
+                                       sigma=0, finish=65536)),
     ("graph_auto", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_GRAPH)),
+    ("graph_auto_finisher_early", dict(mode=L.MODE_AUTO, pull=L.PULL_LB, loop=L.LOOP_GRAPH,
+                                       finish=65536)),
     ("graph_push", dict(mode=L.MODE_PUSH, pull=L.PULL_LB, loop=L.LOOP_GRAPH)),
 ]
 
@@ -317,9 +325,10 @@ def test_launch_budget_follows_the_graph(gpu):
     gd = tb.Graph(deep.CP, deep.IC, symmetric=True)
     gs = tb.Graph(shallow.CP, shallow.IC, symmetric=True)
     ctx = tb.Context(max(deep.n, shallow.n))
+    every_level = tb.default_opts(finish=-1)  # one launch per level throughout
     for g, gr, roots in ((shallow, gs, [0, 1, 2]), (deep, gd, [0, 5]), (shallow, gs, [3, 4]),
                          (deep, gd, [7, 0])):
-        st = ctx.run_many(gr, roots)
+        st = ctx.run_many(gr, roots, every_level)
         for r, x in zip(roots, st):
             S, sig, ovf, lv = oracle.textbook(g.CP, g.IC, int(r))
             assert x["levels"] == lv and x["launches"] == lv + 1, (r, x)
@@ -374,3 +383,28 @@ def _check_golden(res, fx, i, m):
             np.testing.assert_array_equal(sig, fx["sigma_tdbu"][i], err_msg=name)
         else:
             np.testing.assert_array_equal(sig, fx["sigma_cooc"][i], err_msg=name)
+
+
+def test_finisher_takes_the_tail(gpu):
+    """Once at most `finish` in-list entries are unexplored, one workgroup
+    runs the remaining levels: with finish = 65536 a BFS on R-MAT 14 (nz
+    ~ 460k) ends within fewer launches than one per level (the default, off);
+    results and level / vertex / edge counts are the same."""
+    g = graphs.rmat(14, 16, seed=2)
+    gr = tb.Graph(g.CP, g.IC, symmetric=True)
+    ctx = tb.Context(g.n)
+    roots = graphs.sample_sources(graphs.nonzero_degree_vertices(g), 6, seed=5)
+    for r in roots:
+        S, sig, ovf, lv = oracle.textbook(g.CP, g.IC, int(r))
+        runs = {}
+        for name, fin in (("off", -1), ("default", 0), ("late", 1024), ("early", 65536)):
+            res = ctx.bfs(gr, int(r), tb.default_opts(finish=fin))
+            np.testing.assert_array_equal(res.S, S, err_msg=f"{name} r={r}")
+            np.testing.assert_array_equal(res.sigma, sig, err_msg=f"{name} r={r}")
+            np.testing.assert_array_equal(res.ovf, ovf, err_msg=f"{name} r={r}")
+            assert res.stats["levels"] == lv and res.stats["reached"] == int((S >= 0).sum())
+            assert res.stats["edges"] == int(np.diff(g.CP)[S >= 0].sum())
+            runs[name] = res.stats["launches"]
+        assert runs["off"] == lv + 1 and runs["default"] == lv + 1, runs
+        assert runs["early"] < runs["off"], runs
+    ctx.close()

@@ -64,7 +64,7 @@ class Opts(C.Structure):
         ("batch_order", C.c_int32),
         ("stage_ratio", C.c_float),
         ("lb_grid", C.c_int32),
-        ("reserved", C.c_int32 * 1),
+        ("finish", C.c_int32),
     ]
 
 

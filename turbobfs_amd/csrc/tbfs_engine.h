@@ -72,6 +72,8 @@ struct tbfs_ctx {
   int persist_grid_max = 0;         // 0: no cooperative launch available
   int persist_grid_max1024 = 0;     // the same for 1024-thread workgroups
   tbfs::LoopState* loop = nullptr;  // [2] chained-loop state slots
+  uint32_t* fin_list[2] = {nullptr, nullptr};  // the finisher's column lists
+  unsigned* fin_ctr = nullptr;      // its list length and workgroup count
   tbfs::LoopState* loop_h = nullptr;
   int chain_k = 8;                  // levels enqueued per host check
   // shapes of the chained level launches, learned from the last call on
@@ -82,6 +84,9 @@ struct tbfs_ctx {
   const tbfs_graph* pred_graph = nullptr;
   uint64_t pred_stage = ~0ull;
   int pred_levels = 0;
+  float stage_votes[64] = {};       // decayed count of sources whose level d staged
+  float level_votes[64] = {};       // ... whose BFS ran level d at all
+  int pred_fin = -1;                // first depth one workgroup finished at (-1: none)
   size_t fbl_limit = 0;             // bytes of LDS for a staged frontier bitmap
   int cus = 0;
   int32_t last_n = 0;

@@ -4,6 +4,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <iterator>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -615,6 +616,9 @@ void ctx_free(tbfs_ctx* c) {
   dev_free(c->stats);
   dev_free(c->loop);
   dev_free(c->cset);
+  dev_free(c->fin_list[0]);
+  dev_free(c->fin_list[1]);
+  dev_free(c->fin_ctr);
   dev_free(c->roots_d);
   dev_free(c->gstats);
   if (c->groots_h) (void)hipHostFree(c->groots_h);
@@ -664,6 +668,9 @@ int ctx_create_on(int32_t n_cap, hipStream_t stream, tbfs_ctx** out) {
   if (!rc) rc = dev_alloc(&c->cset, 3);
   if (!rc) rc = dev_alloc(&c->stats, 1);
   if (!rc) rc = dev_alloc(&c->loop, 2);
+  if (!rc) rc = dev_alloc(&c->fin_list[0], (size_t)std::min<uint32_t>(kFinCap, (uint32_t)n_cap));
+  if (!rc) rc = dev_alloc(&c->fin_list[1], (size_t)std::min<uint32_t>(kFinCap, (uint32_t)n_cap));
+  if (!rc) rc = dev_alloc(&c->fin_ctr, 16);
   if (rc) goto fail;
   e = hipHostMalloc((void**)&c->cset_h, sizeof(CtrSet), hipHostMallocDefault);
   if (e != hipSuccess) { rc = hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); goto fail; }
@@ -825,6 +832,15 @@ static PersistParams make_params(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   p.beta = o.beta > 0 ? o.beta : kDefaultBeta;
   p.stage_ratio = o.stage_ratio > 0 ? o.stage_ratio : kDefaultStageRatio;
   p.loop = c->loop;
+  // the finisher's lists hold min(n_cap, kFinCap) columns: the unreached
+  // columns with in-lists number at most min(n, unexplored entries), so
+  // fin_mu is capped at kFinCap only when n_cap exceeds it
+  p.fin_list[0] = c->fin_list[0];
+  p.fin_list[1] = c->fin_list[1];
+  p.fin_ctr = c->fin_ctr;
+  p.fin_mu = o.finish <= 0 ? 0u
+             : (uint32_t)c->n_cap <= kFinCap ? (uint32_t)o.finish
+                                             : std::min<uint32_t>((uint32_t)o.finish, kFinCap);
   return p;
 }
 
@@ -1273,21 +1289,35 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
     // spare, so a source needing one launch more than any seen here still
     // finishes without a re-run (a spare launch costs ~4 us, a re-run a BFS)
     c->chain_k = std::min(4096, std::max(2, maxl + 1));
-    // the next call's launch shapes: the levels that staged for any source,
-    // kept across calls on this graph (on R-MAT scale 20 a staging pull on
-    // the non-staging shape costs ~40 us, a hub push on the staging shape
-    // ~10 us: a level that stages for some sources stages for all), and the
-    // depth
-    uint64_t stage = 0;
-    int depth = 0;
+    // the next call's launch shapes.  Per level d, a decayed vote over the
+    // sources so far on this graph (each source weighs 0.95 of the one
+    // after it): the launch stages when more than a fifth of the sources
+    // that ran level d staged there (on R-MAT scale 20 a staging pull on the
+    // non-staging shape costs ~40 us, a push on the staging shape ~10 us, so
+    // staging pays from a probability of 10 / (40 + 10) on).  And the
+    // deepest BFS so far.
+    if (c->pred_graph != g) {
+      std::fill(std::begin(c->stage_votes), std::end(c->stage_votes), 0.f);
+      std::fill(std::begin(c->level_votes), std::end(c->level_votes), 0.f);
+      c->pred_levels = 0;
+    }
+    int depth = c->pred_levels;
     for (int i = 0; i < count; ++i) {
-      stage |= c->stats_h[i].stage_mask;
-      depth = std::max(depth, (int)c->stats_h[i].levels);
+      const RunStats& r = c->stats_h[i];
+      const int lv = std::min(64, (int)r.levels);
+      for (int d = 0; d < 64; ++d) {
+        c->stage_votes[d] *= 0.95f;
+        c->level_votes[d] *= 0.95f;
+      }
+      for (int d = 0; d < lv; ++d) {
+        c->level_votes[d] += 1.f;
+        if ((r.stage_mask >> d) & 1ull) c->stage_votes[d] += 1.f;
+      }
+      depth = std::max(depth, (int)r.levels);
     }
-    if (c->pred_graph == g && c->pred_stage != ~0ull) {
-      stage |= c->pred_stage;
-      depth = std::max(depth, c->pred_levels);
-    }
+    uint64_t stage = 0;
+    for (int d = 0; d < 64; ++d)
+      if (c->stage_votes[d] > 0.2f * c->level_votes[d]) stage |= 1ull << d;
     c->pred_graph = g;
     c->pred_stage = stage;
     c->pred_levels = depth;

@@ -80,7 +80,8 @@ struct alignas(64) RunStats {
   unsigned long long reached;
   unsigned long long edges;
   int32_t launches;  // chained loop: level kernels used, the last one included
-  int32_t pad0;
+  int32_t fin_level; // chained loop: depth at which one workgroup finished the
+                     // BFS (-1: every level ran on the whole grid)
   unsigned long long stage_mask;  // chained loop: bit min(d, 63) set when level d
                                   // was a pull that stages the frontier bitmap
   unsigned long long pad[2];
@@ -152,7 +153,16 @@ struct PersistParams {
   float beta;
   float stage_ratio;  // pull levels stage the bitmap when mu > stage_ratio * n
   LoopState* loop;  // [2] (chained loop only)
+  // chained loop: once at most fin_mu in-list entries are unexplored, one
+  // workgroup runs the remaining levels over a list of the unreached columns
+  // (fin_list[0/1]: kFinCap entries each; fin_ctr: list length, workgroups
+  // done; zeroed by the init kernel).  fin_mu 0: never.
+  uint32_t* fin_list[2];
+  unsigned* fin_ctr;
+  uint32_t fin_mu;
 };
+
+constexpr uint32_t kFinCap = 1u << 16;  // entries of each finisher list (at least)
 
 // Degree statistics of the lists delimited by ptr[0..n]: acc (zeroed by the
 // caller) receives the maximum key (degree << 32 | ~vertex), the degree sum,

@@ -992,6 +992,11 @@ __global__ __launch_bounds__(256) void k_chain_init(PersistParams p) {
   if (gtid == 0) {
     p.loop[0] = first_state(p);
     p.stats->levels = -1;  // set by the level that finds the frontier empty
+    p.stats->fin_level = -1;
+    if (p.fin_ctr) {
+      p.fin_ctr[0] = 0;
+      p.fin_ctr[1] = 0;
+    }
   }
 }
 
@@ -1012,6 +1017,268 @@ __global__ __launch_bounds__(256) void k_chain_seed(PersistParams p) {
   atomicAdd(&c->min, (unsigned long long)indeg(p.g, r));
   const unsigned long long hb = hub_bits(p.g, r);
   if (hb) atomicAdd(&c->hubs, hb);
+}
+
+// ---- the finisher ------------------------------------------------------------
+// Once few in-list entries are unexplored (mu <= fin_mu), the rest of the BFS
+// costs more in level launches than in work: every workgroup of the level
+// lists the unreached columns that have in-lists (grid-stride over the
+// bitmap words, appended with one atomic per wavefront), and the last
+// workgroup to finish its share runs every remaining level itself, as a pull
+// over that list with workgroup barriers between levels.  A column's parents
+// at depth k are the in-list entries u with S[u] == k; a column found at
+// depth k + 1 gets S, sigma (the parents' counts added with carry into the
+// overflow flag) and its flag, the others go on to the next level's list.
+// The last workgroup is the only one left, so it needs no grid barrier, and
+// it reads and writes S / sigma / ovf with plain stores (one CU: coherent at
+// workgroup scope).
+constexpr int kFinU = 8;       // columns per thread in flight
+constexpr int kFinLight = 16;  // columns with more in-list entries: a wavefront each
+constexpr int kFinHeavyCap = 1024;
+
+template <int BLOCK>
+struct FinLds {
+  int32_t heavy[kFinHeavyCap];  // this pass's columns with long in-lists
+  unsigned int nheavy;
+  unsigned int nkeep;           // columns kept for the next level
+  unsigned int found;           // columns found this level
+  unsigned int ovf;
+  unsigned long long edges;     // their in-list entries
+  unsigned long long hin;       // how many of them have in-degree > kBigIn
+  unsigned int last;            // this workgroup finishes the BFS
+};
+
+// List the unreached columns (not in vis or the current frontier) that have
+// in-lists; skip bits mark in-degree 0 or > kBigIn, so they are checked
+// only while some column with in-degree > kBigIn is unreached.
+__device__ __forceinline__ void fin_list_columns(const PersistParams& p, int d, bool heavy_open,
+                                                 int64_t gtid, int64_t gstride) {
+  const GraphView& g = p.g;
+  const int64_t nwords = ((int64_t)g.n + 63) >> 6;
+  const uint64_t* fcur = p.fb[d % 3];
+  uint32_t* out = p.fin_list[0];
+  const int ln = lane();
+  for (int64_t wb = gtid - ln; wb < nwords; wb += gstride) {
+    const int64_t w = wb + ln;
+    uint64_t live = 0;
+    if (w < nwords) {
+      uint64_t m = ~(p.vis[w] | fcur[w]);
+      if (w == nwords - 1 && (g.n & 63)) m &= (1ull << (g.n & 63)) - 1ull;
+      const uint64_t sk = p.skip[w];
+      live = m & ~sk;
+      if (heavy_open) {
+        uint64_t c = m & sk;
+        while (c) {
+          const int b = __builtin_ctzll(c);
+          c &= c - 1;
+          if (indeg(g, (int)(w * 64 + b)) > 0) live |= 1ull << b;
+        }
+      }
+    }
+    const uint32_t cnt = (uint32_t)__popcll(live);
+    const uint32_t incl = wave_incl_scan(cnt);
+    const uint32_t total = __shfl(incl, 63);
+    if (total == 0) continue;
+    unsigned int base = 0;
+    if (ln == 0) base = atomicAdd(&p.fin_ctr[0], total);
+    base = __shfl(base, 0);
+    uint32_t o = base + incl - cnt;
+    while (live) {
+      const int b = __builtin_ctzll(live);
+      live &= live - 1;
+      if (o < kFinCap) out[o] = (uint32_t)(w * 64 + b);
+      ++o;
+    }
+  }
+}
+
+// Add x (with its overflow flag xo) into the 64-bit sum acc / flag ao.
+__device__ __forceinline__ void fin_add(uint64_t& acc, uint32_t& ao, uint64_t x, uint32_t xo) {
+  const uint64_t t = acc + x;
+  ao |= xo | (t < acc ? 1u : 0u);
+  acc = t;
+}
+
+template <int BLOCK>
+__device__ void fin_run(const PersistParams& p, LoopState& s, int32_t step, FinLds<BLOCK>& L) {
+  const GraphView& g = p.g;
+  const int t = threadIdx.x;
+  const int ln = lane();
+  const int wave = t >> 6;
+  constexpr int kWaves = BLOCK / 64;
+  uint32_t* A = p.fin_list[0];
+  uint32_t* B = p.fin_list[1];
+  uint32_t LA = min(p.fin_ctr[0], kFinCap);
+  const int d0 = s.d;
+  // level d0 expands as a pull here, whatever step_state chose for it
+  if (s.pushing) {
+    --s.push_levels;
+    ++s.pull_levels;
+  }
+  for (int k = d0;; ++k) {
+    if (t == 0) {
+      L.nkeep = 0;
+      L.found = 0;
+      L.ovf = 0;
+      L.edges = 0;
+      L.hin = 0;
+    }
+    __syncthreads();
+    unsigned int found = 0, ovfl = 0;
+    unsigned long long edges = 0, hin = 0;
+    for (uint32_t base = 0; base < LA; base += (uint32_t)BLOCK * kFinU) {
+      if (t == 0) L.nheavy = 0;
+      __syncthreads();
+      int v[kFinU], b[kFinU], e[kFinU];
+#pragma unroll
+      for (int i = 0; i < kFinU; ++i) {
+        const uint32_t j = base + (uint32_t)(i * BLOCK + t);
+        v[i] = j < LA ? (int)A[j] : -1;
+      }
+#pragma unroll
+      for (int i = 0; i < kFinU; ++i) {
+        b[i] = 0;
+        e[i] = 0;
+        if (v[i] >= 0) {
+          b[i] = g.CP[v[i]];
+          e[i] = g.CP[v[i] + 1];
+        }
+      }
+      // long in-lists: a wavefront each, after this pass
+#pragma unroll
+      for (int i = 0; i < kFinU; ++i) {
+        if (v[i] >= 0 && e[i] - b[i] > kFinLight) {
+          const unsigned int h = atomicAdd(&L.nheavy, 1u);
+          if (h < (unsigned)kFinHeavyCap) {
+            L.heavy[h] = v[i];
+            v[i] = -2;  // settled by its wavefront
+          }
+        }
+      }
+      uint64_t acc[kFinU];
+      uint32_t ao[kFinU], hit[kFinU];
+#pragma unroll
+      for (int i = 0; i < kFinU; ++i) {
+        acc[i] = 0;
+        ao[i] = 0;
+        hit[i] = 0;
+      }
+      // entry r of every light column at once
+      for (int r = 0;; ++r) {
+        int u[kFinU];
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < kFinU; ++i) {
+          u[i] = -1;
+          if (v[i] >= 0 && b[i] + r < e[i]) {
+            u[i] = g.IC[b[i] + r];
+            any = true;
+          }
+        }
+        if (!__any(any)) break;  // wave-uniform exit
+        int32_t su[kFinU];
+#pragma unroll
+        for (int i = 0; i < kFinU; ++i) su[i] = u[i] >= 0 ? p.S[u[i]] : -1;
+#pragma unroll
+        for (int i = 0; i < kFinU; ++i) {
+          if (su[i] == k) {
+            hit[i] = 1;
+            if (p.sigma_on) fin_add(acc[i], ao[i], p.sigma[u[i]], p.ovf[u[i]]);
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kFinU; ++i) {
+        if (v[i] < 0) continue;
+        if (hit[i]) {
+          p.S[v[i]] = k + 1;
+          if (p.sigma_on) {
+            p.sigma[v[i]] = acc[i];
+            p.ovf[v[i]] = (uint8_t)ao[i];
+          }
+          ++found;
+          ovfl |= ao[i];
+          edges += (unsigned long long)(e[i] - b[i]);
+        } else {
+          B[atomicAdd(&L.nkeep, 1u)] = (uint32_t)v[i];
+        }
+      }
+      __syncthreads();
+      // the long in-lists: a wavefront per column, lanes over its entries
+      const unsigned int nh = min(L.nheavy, (unsigned)kFinHeavyCap);
+      for (unsigned int h = wave; h < nh; h += kWaves) {
+        const int vv = L.heavy[h];
+        const int hb = g.CP[vv], he = g.CP[vv + 1];
+        uint64_t lo = 0, hi = 0;
+        uint32_t o = 0, any = 0;
+        for (int q = hb + ln; q < he; q += 64) {
+          const int uu = g.IC[q];
+          if (p.S[uu] == k) {
+            any = 1;
+            if (p.sigma_on) {
+              const uint64_t x = p.sigma[uu];
+              lo += x & 0xffffffffull;
+              hi += x >> 32;
+              o |= p.ovf[uu];
+            }
+          }
+        }
+        const bool got = __any(any != 0);
+        if (got) {
+          // sum = hi * 2^32 + lo: exact below 2^64 unless hi >= 2^32 or the
+          // final addition carries
+          lo = wave_sum_u64(lo);
+          hi = wave_sum_u64(hi);
+          o = __any(o != 0) ? 1u : 0u;
+          const uint64_t hs = hi << 32;
+          const uint64_t c = hs + lo;
+          o |= (hi >> 32) != 0 || c < hs ? 1u : 0u;
+          if (ln == 0) {
+            p.S[vv] = k + 1;
+            if (p.sigma_on) {
+              p.sigma[vv] = c;
+              p.ovf[vv] = (uint8_t)o;
+            }
+            ++found;
+            ovfl |= o;
+            edges += (unsigned long long)(he - hb);
+            if (he - hb > kBigIn) ++hin;
+          }
+        } else if (ln == 0) {
+          B[atomicAdd(&L.nkeep, 1u)] = (uint32_t)vv;
+        }
+      }
+      __syncthreads();
+    }
+    // this level's totals
+    if (found) atomicAdd(&L.found, found);
+    if (ovfl) atomicOr(&L.ovf, 1u);
+    if (edges) atomicAdd(&L.edges, edges);
+    if (hin) atomicAdd(&L.hin, hin);
+    __syncthreads();
+    const unsigned int q = L.found;
+    if (q == 0) {  // the frontier of depth k + 1 is empty
+      s.d = k + 1;
+      break;
+    }
+    s.reached += q;
+    s.edges += L.edges;
+    s.heavy_reached += L.hin;
+    s.ovf_any |= L.ovf;
+    s.mu = s.mu > L.edges ? s.mu - L.edges : 0ull;
+    ++s.pull_levels;
+    LA = L.nkeep;
+    uint32_t* tmp = A;
+    A = B;
+    B = tmp;
+    __syncthreads();  // L.nkeep read by every thread before the reset
+  }
+  if (t == 0) {
+    s.done = 1;
+    p.loop[(step + 1) & 1] = s;
+    write_stats(p, s, step);
+    p.stats->fin_level = d0;
+  }
 }
 
 // Chained loop: level d.  Reads slot d & 1 and the counters of depth d (both
@@ -1043,6 +1310,24 @@ __global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t 
     }
     return;
   }
+  const bool heavy_open = s.heavy_reached < (unsigned long long)p.nheavy;
+  if (p.fin_mu && s.mu <= (unsigned long long)p.fin_mu && p.mode == TBFS_MODE_AUTO) {
+    // the rest of the BFS in the last workgroup to list its columns
+    static_assert(sizeof(FinLds<BLOCK>) <= sizeof(lds), "the finisher's LDS fits the level's");
+    FinLds<BLOCK>& F = *reinterpret_cast<FinLds<BLOCK>*>(lds);
+    fin_list_columns(p, d, heavy_open, gtid, gstride);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      const unsigned prev = atomicAdd(&p.fin_ctr[1], 1u);
+      F.last = prev == gridDim.x - 1 ? 1u : 0u;
+      if (F.last) __threadfence();
+    }
+    __syncthreads();
+    if (!F.last) return;
+    fin_run<BLOCK>(p, s, step, F);
+    return;
+  }
   LoopState nx = s;
   nx.d = d + 1;
   // the host picks each launch's shape (staging workgroups or not) from the
@@ -1050,8 +1335,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t 
   if (wants_stage(p, s.pushing != 0, s.mu)) nx.stage_mask |= 1ull << (d < 63 ? d : 63);
   if (gtid == 0) p.loop[(step + 1) & 1] = nx;
   LevelAcc acc;
-  run_level(p, d, s.pushing != 0, s.ovf_any, s.mu, c.hout != 0,
-            s.heavy_reached < (unsigned long long)p.nheavy, gtid, gstride,
+  run_level(p, d, s.pushing != 0, s.ovf_any, s.mu, c.hout != 0, heavy_open, gtid, gstride,
             &lds[threadIdx.x >> 6], acc, FBL ? fb_dyn : nullptr);
 }
 
