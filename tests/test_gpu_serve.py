@@ -154,10 +154,10 @@ def _stream(srv, cs, plan, gap_s, store):
 
 
 def test_serve_continuous_batch_admits_mid_batch(gpu):
-    """A deep graph (256 x 256 grid, 10% of edges removed: ~500 levels) runs
-    as a continuous batch; queries arriving while it runs are seeded at a
-    later level boundary (level > 0) and their rows are still exactly the
-    oracle's, and equal to the same root's run alone."""
+    """A deep graph (256 x 256 grid, 10% of edges removed: ~500 levels):
+    overlapping queries share a continuous batch, and queries arriving while
+    it runs are seeded at a later level boundary (level > 0).  Their rows are
+    exactly the oracle's, and equal to the same root's run alone."""
     cs = [graphs.grid(256, 256, 0.1, seed=4, relabel=True),
           graphs.rmat(12, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)]
     gs = [tb.Graph(g.CP, g.IC, symmetric=True) for g in cs]
@@ -190,16 +190,21 @@ def test_serve_continuous_batch_admits_mid_batch(gpu):
     print(f"[serve] admitted mid-batch: {len(mid)} (levels {sorted(d['level'] for d in mid)[:8]}...)",
           flush=True)
     assert len(mid) >= 5
-    compared = 0
-    for d in done:
+    for d in done:  # roots 0..2 ran alone first and again in the stream
         gi, r = store[d["ticket"]][:2]
         if gi == 0 and r in alone:
             for a, b in zip(alone[r], store[d["ticket"]][2:]):
                 assert np.array_equal(a, b)
-            compared += d["level"] > 0
-    # roots 0..2 ran alone and again in the stream; at least one of the
-    # repeats was seeded mid-batch
-    assert compared >= 1
+    # three queries seeded mid-batch, each run again alone: same arrays
+    for d in mid[:3]:
+        r = store[d["ticket"]][1]
+        one = {}
+        t = _submit(srv, cs[0], 0, r, one)
+        srv.drain()
+        (d1,) = srv.poll(10)
+        assert d1["ticket"] == t and d1["status"] == 0 and d1["level"] == 0
+        for a, b in zip(one[t][2:], store[d["ticket"]][2:]):
+            assert np.array_equal(a, b)
     assert all(d["level"] == 0 for d in done if d["graph"] == 1)
     srv.close()
 
