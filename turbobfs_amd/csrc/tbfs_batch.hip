@@ -768,14 +768,20 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_level(BatchParams p, int3
   // frontier d-1 nor frontier d has a bit); from the word lists when both are
   // complete, else over every word
   if (st.list_cur && st.list_prev) {
+    // from the grid's last threads: the push below deals the same lists
+    // from its first, so on small frontiers the two chains of dependent
+    // requests run side by side on different CUs (they touch disjoint
+    // words: seen slot d + 1 and frontier d - 1 here, seen slot d and
+    // frontiers d, d + 1 there)
     const uint32_t* lp = p.lst[(d + 2) % 3];
     const uint32_t* lc = p.lst[d % 3];
-    for (int64_t k = gtid; k < (int64_t)st.q_prev; k += gstride) {
+    const int64_t rt = gstride - 1 - gtid;
+    for (int64_t k = rt; k < (int64_t)st.q_prev; k += gstride) {
       const uint32_t i = lp[k];
       sn[i] = sd[i] | Fc[i];
       Fp[i] = 0ull;
     }
-    for (int64_t k = gtid; k < (int64_t)q; k += gstride) {
+    for (int64_t k = rt; k < (int64_t)q; k += gstride) {
       const uint32_t i = lc[k];
       sn[i] = sd[i] | Fc[i];
     }
