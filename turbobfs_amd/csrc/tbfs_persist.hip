@@ -382,17 +382,20 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
         int v[kUnroll];
         uint64_t xs[kUnroll];
         bool os[kUnroll];
+        // lanes past the end take the last edge (total > 0) and are masked
+        // out; the owner searches advance together, no load behind a branch
+        int e[kUnroll], o[kUnroll], k[kUnroll];
+#pragma unroll
+        for (int t = 0; t < kUnroll; ++t) e[t] = min(e0 + 64 * t, total - 1);
+        owners_of<kUnroll>(L->excl, e, o);
+#pragma unroll
+        for (int t = 0; t < kUnroll; ++t) k[t] = L->beg[o[t]] + (e[t] - L->excl[o[t]]);
 #pragma unroll
         for (int t = 0; t < kUnroll; ++t) {
-          const int e = e0 + 64 * t;
-          const bool ok = e < total;
-          const int o = owner_of(L->excl, ok ? e : 0);
-          // lanes past the end read entry 0 (total > 0, so nz > 0)
-          const int k = ok ? L->beg[o] + (e - L->excl[o]) : 0;
-          const int vk = g.JR[k];
-          v[t] = ok ? vk : -1;
-          xs[t] = L->x[o];
-          os[t] = (oumask >> o) & 1ull;
+          const int vk = g.JR[k[t]];
+          v[t] = e0 + 64 * t < total ? vk : -1;
+          xs[t] = L->x[o[t]];
+          os[t] = (oumask >> o[t]) & 1ull;
         }
         push_edges<kUnroll>(g, st, dn, v, xs, os, a);
       }
@@ -400,62 +403,120 @@ __device__ __forceinline__ void push_lb_level(const GraphView& g,
   }
 }
 
-// One pass over in-list entries e0, e0 + 64, ... (R per lane) of the
-// columns laid out in L (exclusive prefix excl, first entry beg), every load
-// of the pass issued before any is used: owners, then in-list entries, then
-// parent bits, then parents' counts.  Lanes past the end read entry 0 and
-// are masked out; a count is read for every entry (non-parents read vertex
-// 0's, one shared line), so no load sits behind a branch.  Column sums go
-// into LDS as 32-bit halves, each added into a 64-bit word: exact for any
-// in-degree below 2^32, so no atomic's result is needed to find a carry.
-template <int R, bool SIGMA>
-__device__ __forceinline__ void pull_light_pass(const GraphView& g, const StateView& st,
-                                                WaveLds* L, const uint64_t* fpar,
-                                                uint32_t ovf_any, int e0, int total,
-                                                uint64_t& fmask, uint64_t& omask) {
-  int o[R], u[R];
-  bool par[R];
+// Parent test on a frontier bitmap read as 32-bit words (one bit-field
+// extract instead of a 64-bit shift).
+__device__ __forceinline__ bool fbit32(const uint64_t* fb, int u) {
+  return __builtin_amdgcn_ubfe(reinterpret_cast<const uint32_t*>(fb)[u >> 5], (uint32_t)u, 1u) != 0;
+}
+
+// A pull group's columns with in-list entries are numbered by rank (lane
+// order); the group's in-lists are concatenated in that order, column j's
+// entries starting at xj (strictly increasing over the ranks).  The owner of
+// group entry e is the last column whose first entry is at most e.
+//
+// Owner ranks of the entries B + 64 t + lane (t < R, B a multiple of 64):
+// for each window of 64 entries, every column starting inside it (at window
+// offset 1..63) sends a flag to the lane of its start (ds_permute moves
+// register values between lanes without touching LDS memory; lanes nobody
+// sends to receive 0).  A lane's owner is then the number of columns starting
+// at or before the window's first entry, plus the flags up to its own lane,
+// minus one -- no search.
+template <int R>
+__device__ __forceinline__ void group_owners(int B, int xj, bool hasd, int (&r)[R]) {
+  int recv[R];
 #pragma unroll
   for (int t = 0; t < R; ++t) {
-    const int e = e0 + 64 * t;
-    const bool ok = e < total;
-    o[t] = owner_of(L->excl, ok ? e : 0);
-    const int k = ok ? L->beg[o[t]] + (e - L->excl[o[t]]) : 0;
-    u[t] = g.IC[k];
-    par[t] = ok;
+    const int p = xj - (B + 64 * t);
+    const bool in = hasd && (unsigned)(p - 1) < 63u;
+    recv[t] = __builtin_amdgcn_ds_permute(in ? p << 2 : 0, in ? 1 : 0);
   }
 #pragma unroll
-  for (int t = 0; t < R; ++t) par[t] = par[t] && fbit(fpar, u[t]);
-  if (SIGMA) {
+  for (int t = 0; t < R; ++t) {
+    const int p = xj - (B + 64 * t);
+    const uint64_t m = __ballot(recv[t] != 0) >> 1;  // bit i - 1: a column starts at offset i
+    const uint32_t c0 = (uint32_t)__popcll(__ballot(hasd && p <= 0));
+    r[t] = (int)(__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, c0 - 1u)) &
+                 63u);
+  }
+}
+
+// First half of a pass: owner ranks and in-list entries of the group entries
+// B + 64 t + lane.  Entries past the group's end get an in-list index clamped
+// into [0, kmax] (kmax: the group's last in-list entry) and are masked out by
+// the second half; nothing here sits behind a branch.
+template <int R>
+__device__ __forceinline__ void group_fetch(const GraphView& g, const WaveLds* L, int B, int xj,
+                                            bool hasd, int kmax, int (&r)[R], int (&u)[R]) {
+  group_owners<R>(B, xj, hasd, r);
+  int k[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t)
+    k[t] = min(max(L->beg[r[t]] + (B + 64 * t + lane() - L->excl[r[t]]), 0), kmax);
+#pragma unroll
+  for (int t = 0; t < R; ++t) u[t] = g.IC[k[t]];
+}
+
+// Passes of R group entries per lane: parent bits, then parents' counts
+// (non-parents read vertex 0's, one shared line, so no load sits behind a
+// branch), added per column into LDS word x[rank] as (low 32 bits + 2^40):
+// a light column has at most kBigIn <= 256 entries, so bits 0..39 hold the
+// exact sum of the low halves and bits 40.. the number of parents (found iff
+// nonzero, whatever the count).  High halves go to xh[rank] (bits 0..39),
+// an overflow flag of a parent to its bit 63.  PIPE: two passes in flight --
+// the in-list entries of pass k + 1 are loaded (unconditionally, clamped)
+// while the counts of pass k are on their way.  Without PIPE the group must
+// fit one pass (total <= 64 R).
+template <int R, bool SIGMA, bool PIPE>
+__device__ __forceinline__ void pull_passes(const GraphView& g, const StateView& st,
+                                            WaveLds* L, const uint64_t* fpar, uint32_t ovf_any,
+                                            int total, int xj, bool hasd, int kmax) {
+  static_assert(kBigIn <= 256, "x[rank] keeps the parent count above bit 40");
+  int r[R], u[R];
+  group_fetch<R>(g, L, 0, xj, hasd, kmax, r, u);
+  for (int B = 0;; B += 64 * R) {
+    bool par[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) par[t] = (int)(B + 64 * t + lane() < total) & (int)fbit32(fpar, u[t]);
     uint64_t xs[R];
+    if (SIGMA) {
 #pragma unroll
-    for (int t = 0; t < R; ++t) xs[t] = st.sigma[par[t] ? u[t] : 0];
-    if (ovf_any) {
+      for (int t = 0; t < R; ++t) xs[t] = st.sigma[par[t] ? u[t] : 0];
+      if (ovf_any) {
 #pragma unroll
-      for (int t = 0; t < R; ++t)
-        if (par[t] && st.ovf[u[t]]) omask |= 1ull << o[t];
+        for (int t = 0; t < R; ++t)
+          if (par[t] && st.ovf[u[t]])
+            atomicOr((unsigned long long*)&L->xh[r[t]], 1ull << 63);
+      }
     }
+    const bool more = PIPE && B + 64 * R < total;  // wave-uniform
+    int rn[R], un[R];
+    if (PIPE) group_fetch<R>(g, L, B + 64 * R, xj, hasd, kmax, rn, un);
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       if (par[t]) {
-        fmask |= 1ull << o[t];
-        atomicAdd((unsigned long long*)&L->x[o[t]], (unsigned long long)(xs[t] & 0xffffffffull));
-        const unsigned long long h = xs[t] >> 32;
-        if (h) atomicAdd((unsigned long long*)&L->xh[o[t]], h);
+        const uint64_t lo = SIGMA ? (xs[t] & 0xffffffffull) : 0ull;
+        atomicAdd((unsigned long long*)&L->x[r[t]], (unsigned long long)(lo + (1ull << 40)));
+        if (SIGMA) {
+          const unsigned long long h = xs[t] >> 32;
+          if (h) atomicAdd((unsigned long long*)&L->xh[r[t]], h);
+        }
       }
     }
-  } else {
+    if (!more) break;
 #pragma unroll
-    for (int t = 0; t < R; ++t)
-      if (par[t]) fmask |= 1ull << o[t];
+    for (int t = 0; t < R; ++t) {
+      r[t] = rn[t];
+      u[t] = un[t];
+    }
   }
 }
 
 // One group of up to 64 columns of a pull level, column v with in-list
-// [beg, beg + deg) on its lane (deg 0: none): the in-lists are walked in
-// passes of 4 or 8 entries per lane (parents' counts summed per column in
-// LDS), and the columns that found a parent get their depth, count and
-// overflow flag and are tallied.  Returns the wavefront's mask of those
+// [beg, beg + deg) on its lane (deg 0: none; deg <= kBigIn): the in-lists are
+// walked in passes of 4 or 8 entries per lane (parents' counts summed per
+// column in LDS), and the columns that found a parent get their depth, count
+// and overflow flag and are tallied.  Returns the wavefront's mask of those
 // columns (lane bits).  All lanes must take part.
 template <bool SIGMA>
 __device__ __forceinline__ uint64_t pull_group(const GraphView& g, const StateView& st,
@@ -465,36 +526,43 @@ __device__ __forceinline__ uint64_t pull_group(const GraphView& g, const StateVi
   const uint32_t incl = wave_incl_scan((uint32_t)deg);
   const int total = (int)__shfl(incl, 63);
   if (total == 0) return 0;
+  const bool hasd = deg > 0;
+  const int xj = (int)incl - deg;
+  const uint64_t hm = __ballot(hasd);
+  const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
+  const int kmax = __shfl(beg + deg - 1, 63 - __clzll((long long)hm));
   wave_lds_sync();
-  L->excl[lane()] = (int)incl - deg;
-  L->beg[lane()] = beg;
-  if (SIGMA) {
-    L->x[lane()] = 0;
-    L->xh[lane()] = 0;
+  if (hasd) {
+    L->excl[rank] = xj;
+    L->beg[rank] = beg;
   }
+  L->x[lane()] = 0;
+  if (SIGMA) L->xh[lane()] = 0;
   wave_lds_sync();
-  uint64_t fmask = 0, omask = 0;
   // one pass of 4 entries per lane for short groups, else passes of 8
   if (total <= 64 * 4) {
-    pull_light_pass<4, SIGMA>(g, st, L, fpar, ovf_any, lane(), total, fmask, omask);
+    pull_passes<4, SIGMA, false>(g, st, L, fpar, ovf_any, total, xj, hasd, kmax);
   } else {
-    for (int e0 = lane(); e0 < total; e0 += 64 * kPullRegs)
-      pull_light_pass<kPullRegs, SIGMA>(g, st, L, fpar, ovf_any, e0, total, fmask, omask);
+    pull_passes<kPullRegs, SIGMA, true>(g, st, L, fpar, ovf_any, total, xj, hasd, kmax);
   }
-  fmask = wave_or_u64(fmask);
-  if (fmask == 0) return 0;
-  if (SIGMA) omask = wave_or_u64(omask);
   wave_lds_sync();
-  if ((fmask >> lane()) & 1ull) {
+  const int rk = hasd ? rank : 0;
+  const uint64_t xr = L->x[rk];
+  const bool f = hasd && (xr >> 40) != 0;
+  const uint64_t fmask = __ballot(f);
+  if (fmask == 0) return 0;
+  if (f) {
     st.S[v] = dn;
     if (SIGMA) {
-      // count = xh * 2^32 + x; at least 2^64 iff xh >= 2^32 or the sum
-      // of the low 64 bits carries
-      const uint64_t xh = L->xh[lane()];
+      // count = xh * 2^32 + (low sums); at least 2^64 iff xh >= 2^32 or
+      // the final addition carries
+      const uint64_t xhr = L->xh[rk];
+      const uint64_t xh = xhr & ~(1ull << 63);
       const uint64_t hs = xh << 32;
-      const uint64_t sum = hs + L->x[lane()];
+      const uint64_t sum = hs + (xr & ((1ull << 40) - 1ull));
       st.sigma[v] = sum;
-      if ((xh >> 32) != 0 || sum < hs || ((omask >> lane()) & 1ull)) {
+      if ((xh >> 32) != 0 || sum < hs || (xhr >> 63) != 0) {
         st.ovf[v] = 1;
         a.ovf = 1;
       }
@@ -592,7 +660,7 @@ __device__ __forceinline__ void pull_lb_level(const GraphView& g,
         for (int t = 0; t < kSliceRegs; ++t)
           par[h * kSliceRegs + t] = scb[h] + lane() + 64 * t < sce[h];
 #pragma unroll
-      for (int t = 0; t < R; ++t) par[t] = par[t] && fbit(fpar, u[t]);
+      for (int t = 0; t < R; ++t) par[t] = (int)par[t] & (int)fbit32(fpar, u[t]);  // u[t] valid: no branch
       uint64_t xs[R];
       if (SIGMA) {
 #pragma unroll
@@ -939,23 +1007,32 @@ __device__ __forceinline__ void run_level(const PersistParams& p, int d,
     push_lb_level(g, st, d + 1, ovf_any, p.bigo, frontier_hubs ? p.nbigo : 0, wave, nwaves,
                   L, acc, fclr);
   } else {
-    const uint64_t* fpar = st.fcur;
-    if (fb_lds && wants_stage(p, false, mu)) {
+    // A staging launch (fb_lds set) stages every pull level it runs: the
+    // bitmap pointer is then the LDS array on every path, so the parent tests
+    // compile to LDS reads (a pointer that may be either needs flat loads,
+    // which wait for every outstanding memory operation).  The launch plan
+    // stages only levels predicted heavy; a light one pays the copy (~3 us).
+    const int32_t nbigc = heavy_open ? p.nbigc : 0;
+    if (fb_lds) {
       const uint4* src = reinterpret_cast<const uint4*>(st.fcur);
       uint4* dst = reinterpret_cast<uint4*>(fb_lds);
       const int64_t n16 = nwords >> 1;
       for (int64_t i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
       if ((nwords & 1) && threadIdx.x == 0) fb_lds[nwords - 1] = st.fcur[nwords - 1];
       __syncthreads();
-      fpar = fb_lds;
-    }
-    const int32_t nbigc = heavy_open ? p.nbigc : 0;
-    if (p.sigma_on)
-      pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, nbigc, wave, nwaves, L, acc, fpar,
+      if (p.sigma_on)
+        pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, nbigc, wave, nwaves, L, acc, fb_lds,
+                            fclr);
+      else
+        pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, nbigc, wave, nwaves, L, acc, fb_lds,
+                             fclr);
+    } else if (p.sigma_on) {
+      pull_lb_level<true>(g, st, d + 1, ovf_any, p.bigc, nbigc, wave, nwaves, L, acc, st.fcur,
                           fclr);
-    else
-      pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, nbigc, wave, nwaves, L, acc, fpar,
+    } else {
+      pull_lb_level<false>(g, st, d + 1, ovf_any, p.bigc, nbigc, wave, nwaves, L, acc, st.fcur,
                            fclr);
+    }
   }
   flush_level(st.cset, acc);
 }
@@ -1100,7 +1177,7 @@ __device__ __forceinline__ void fin_add(uint64_t& acc, uint32_t& ao, uint64_t x,
 }
 
 template <int BLOCK>
-__device__ void fin_run(const PersistParams& p, LoopState& s, int32_t step, FinLds<BLOCK>& L) {
+__device__ __forceinline__ void fin_run(const PersistParams& p, LoopState& s, int32_t step, FinLds<BLOCK>& L) {
   const GraphView& g = p.g;
   const int t = threadIdx.x;
   const int ln = lane();
