@@ -86,6 +86,9 @@ struct tbfs_ctx {
   int pred_levels = 0;
   float stage_votes[64] = {};       // decayed count of sources whose level d staged
   float level_votes[64] = {};       // ... whose BFS ran level d at all
+  float work_votes[64] = {};        // decayed sum of their level d's work (log2 entries)
+  uint64_t pred_dual = 0;           // levels launched in both shapes (staging uncertain)
+  uint8_t pred_work[64] = {};       // expected log2 work of level d (0: unknown)
   int pred_fin = -1;                // first depth one workgroup finished at (-1: none)
   size_t fbl_limit = 0;             // bytes of LDS for a staged frontier bitmap
   int cus = 0;
@@ -98,6 +101,8 @@ struct tbfs_ctx {
   int ggrid = 0;
   size_t gfbl = 0;
   uint64_t gstage = 0;
+  uint64_t gdual = 0;
+  uint8_t gwork[64] = {};
   int glevels = 0;
   tbfs::PersistParams gparams{};
   tbfs::RunStats* gstats = nullptr;  // the chunk's stats slot (device)

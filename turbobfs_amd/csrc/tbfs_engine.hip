@@ -879,34 +879,86 @@ static size_t chain_fbl(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts&
 // level costs ~10 us instead of ~16 us on the full grid (any grid is correct).
 constexpr int kChainWideLevels = 16;
 
-// Shape of one chained level launch: workgroups and dynamic LDS (0: the
-// kLbBlock-thread kernel without a staged bitmap).
+// Shape of one chained level launch: workgroups, dynamic LDS (0: the
+// kLbBlock-thread kernel without a staged bitmap) and which levels it runs.
 struct LevelShape {
   int grid;
   size_t fbl;
+  int32_t want = kLevelAlways;
 };
 
 // The launch shapes of a chained BFS on g: every shape gives the same
-// results; they differ in speed.  Staging workgroups (1024 threads, one per
-// CU, the bitmap in LDS) only for the levels that staged in the context's
-// last call on g; 256-thread workgroups, several per CU, for the others;
-// launches past the levels that call needed (the one that finds the frontier
-// empty, spare launches) on a small grid.  o.grid > 0 fixes the grid.
+// results; they differ in speed.  Learned from the context's earlier calls on
+// g (decayed votes per level):
+//   - staging workgroups (1024 threads, one per CU, the bitmap in LDS) for
+//     the levels that staged for most sources; both shapes, each running the
+//     level only if the device's staging decision matches, for the levels
+//     that staged for some (an unstaged heavy pull costs ~40 us more on
+//     R-MAT scale 20, a staged push ~10 us, a launch that returns ~4 us);
+//   - 256-thread workgroups for the others, as many as the level's expected
+//     work calls for: one per CU up to 2^18 entries, two up to 2^22, else
+//     the default grid (a launch's dispatch and drain grow with its grid: a
+//     level of a few thousand entries takes ~9 us on 256 workgroups and
+//     ~15 us on 1024);
+//   - launches past the levels those calls needed (the one that finds the
+//     frontier empty, spare launches) on a small grid.
+// No history: every launch stages.  o.grid > 0 fixes the grid, o.lb_grid > 0
+// the unstaged launches' grid.
 struct LevelPlan {
-  int grid_stage = 0, grid_lb = 0, grid_spare = 0;
+  int grid_stage = 0, grid_lb = 0, grid_spare = 0, cus = 1;
   size_t fbl = 0;
   uint64_t stage = ~0ull;
+  uint64_t dual = 0;
+  uint8_t work[64] = {};
   int levels = 0;
   bool fixed = false;
+  bool lb_fixed = false;
 
-  LevelShape at(const tbfs_ctx* c, int d) const {
-    const bool st = fbl && ((stage >> (d < 63 ? d : 63)) & 1ull);
-    if (fixed) return {st ? grid_stage : grid_lb, st ? fbl : 0};
-    if (st) return {grid_stage, fbl};
-    if (levels > 0 && d >= levels) return {grid_spare, 0};
-    return {d < kChainWideLevels ? grid_lb : std::min(grid_lb, std::max(1, c->cus)), 0};
+  int lb_grid(int d) const {
+    const int w = d < 64 ? work[d] : 0;
+    if (!lb_fixed && w > 0) {
+      if (w <= 18) return std::min(grid_lb, cus);
+      if (w <= 22) return std::min(grid_lb, 2 * cus);
+      return grid_lb;
+    }
+    return d < kChainWideLevels ? grid_lb : std::min(grid_lb, cus);
+  }
+  // the launches of step d (one or two); returns how many
+  int at(int d, LevelShape (&out)[2]) const {
+    const uint64_t bit = 1ull << (d < 63 ? d : 63);
+    const bool st = fbl && (stage & bit);
+    if (fixed) {
+      out[0] = {st ? grid_stage : grid_lb, st ? fbl : 0};
+      return 1;
+    }
+    if (fbl && (dual & bit) && !(levels > 0 && d >= levels)) {
+      out[0] = {grid_stage, fbl, kLevelIfStaged};
+      out[1] = {lb_grid(d), 0, kLevelIfNotStaged};
+      return 2;
+    }
+    if (st) {
+      out[0] = {grid_stage, fbl};
+      return 1;
+    }
+    if (levels > 0 && d >= levels) {
+      out[0] = {grid_spare, 0};
+      return 1;
+    }
+    out[0] = {lb_grid(d), 0};
+    return 1;
   }
 };
+
+// Enqueue the launches of chained step d.
+static hipError_t chain_step(hipStream_t s, const LevelPlan& pl, const PersistParams& p, int d) {
+  LevelShape sh[2];
+  const int k = pl.at(d, sh);
+  for (int i = 0; i < k; ++i) {
+    const hipError_t e = kern::chain_level(s, sh[i].grid, p, d, sh[i].fbl, sh[i].want);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 
 // about four frontier words per wavefront: fewer workgroups to dispatch per
 // level and fewer counter atomics, still >> 256 CUs' worth of waves
@@ -918,6 +970,8 @@ static LevelPlan level_plan(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_o
   LevelPlan pl;
   pl.fbl = chain_fbl(c, g, o);
   pl.grid_lb = o.lb_grid > 0 ? std::min(o.lb_grid, 65536) : default_lb_grid(g);
+  pl.lb_fixed = o.lb_grid > 0;
+  pl.cus = std::max(1, c->cus);
   pl.grid_stage = std::max(1, c->cus);
   pl.grid_spare = std::max(1, c->cus / 8);
   if (o.grid > 0) {
@@ -926,6 +980,8 @@ static LevelPlan level_plan(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_o
   }
   if (c->pred_graph == g) {
     pl.stage = c->pred_stage;
+    pl.dual = c->pred_dual;
+    std::memcpy(pl.work, c->pred_work, sizeof(pl.work));
     pl.levels = c->pred_levels;
   }
   return pl;
@@ -939,10 +995,7 @@ static int enqueue_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   const LevelPlan pl = level_plan(c, g, o);
   hipStream_t s = c->stream;
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
-  for (int d = 0; d < levels; ++d) {
-    const LevelShape sh = pl.at(c, d);
-    HIPCHK(kern::chain_level(s, sh.grid, p, d, sh.fbl));
-  }
+  for (int d = 0; d < levels; ++d) HIPCHK(chain_step(s, pl, p, d));
   return TBFS_OK;
 }
 
@@ -962,10 +1015,7 @@ static int run_chained_p(tbfs_ctx* c, const tbfs_graph* g, const PersistParams& 
   int32_t d = 0;
   int chunk = std::max(c->chain_k, 8);
   for (;;) {
-    for (int i = 0; i < chunk; ++i) {
-      const LevelShape sh = pl.at(c, d + i);
-      HIPCHK(kern::chain_level(s, sh.grid, p, d + i, sh.fbl));
-    }
+    for (int i = 0; i < chunk; ++i) HIPCHK(chain_step(s, pl, p, d + i));
     d += chunk;
     chunk = std::min(chunk + chunk / 2, 256);
     HIPCHK(hipMemcpyAsync(c->loop_h, c->loop + (d & 1), sizeof(LoopState),
@@ -1145,10 +1195,13 @@ static int graph_chunk(tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o, int
   p.nroots = 1;
   p.roots = c->roots_d;
   const LevelPlan pl = level_plan(c, g, o);
-  const int grid = pl.at(c, 0).grid;
+  LevelShape sh0[2];
+  pl.at(0, sh0);
+  const int grid = sh0[0].grid;
   const size_t fbl = pl.fbl;
   if (c->gexec && c->gK == K && c->ggrid == grid && c->gfbl == fbl &&
-      c->gstage == pl.stage && c->glevels == pl.levels &&
+      c->gstage == pl.stage && c->gdual == pl.dual && c->glevels == pl.levels &&
+      std::memcmp(c->gwork, pl.work, sizeof(pl.work)) == 0 &&
       std::memcmp(&p, &c->gparams, sizeof(p)) == 0)
     return TBFS_OK;
   if (c->gexec) {
@@ -1159,10 +1212,7 @@ static int graph_chunk(tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o, int
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   hipError_t e = kern::chain_init(s, grid_for(g->n, 256, 8192), p);
   if (e == hipSuccess) e = kern::chain_seed(s, p);
-  for (int i = 0; i < K && e == hipSuccess; ++i) {
-    const LevelShape sh = pl.at(c, i);
-    e = kern::chain_level(s, sh.grid, p, i, sh.fbl);
-  }
+  for (int i = 0; i < K && e == hipSuccess; ++i) e = chain_step(s, pl, p, i);
   hipGraph_t gr = nullptr;
   const hipError_t e2 = hipStreamEndCapture(s, &gr);
   if (e != hipSuccess || e2 != hipSuccess) {
@@ -1179,6 +1229,8 @@ static int graph_chunk(tbfs_ctx* c, const tbfs_graph* g, const tbfs_opts& o, int
   c->ggrid = grid;
   c->gfbl = fbl;
   c->gstage = pl.stage;
+  c->gdual = pl.dual;
+  std::memcpy(c->gwork, pl.work, sizeof(pl.work));
   c->glevels = pl.levels;
   c->gparams = p;
   return TBFS_OK;
@@ -1299,6 +1351,7 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
     if (c->pred_graph != g) {
       std::fill(std::begin(c->stage_votes), std::end(c->stage_votes), 0.f);
       std::fill(std::begin(c->level_votes), std::end(c->level_votes), 0.f);
+      std::fill(std::begin(c->work_votes), std::end(c->work_votes), 0.f);
       c->pred_levels = 0;
     }
     int depth = c->pred_levels;
@@ -1308,18 +1361,30 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
       for (int d = 0; d < 64; ++d) {
         c->stage_votes[d] *= 0.95f;
         c->level_votes[d] *= 0.95f;
+        c->work_votes[d] *= 0.95f;
       }
       for (int d = 0; d < lv; ++d) {
         c->level_votes[d] += 1.f;
+        c->work_votes[d] += (float)r.work_lg[d];
         if ((r.stage_mask >> d) & 1ull) c->stage_votes[d] += 1.f;
       }
       depth = std::max(depth, (int)r.levels);
     }
-    uint64_t stage = 0;
-    for (int d = 0; d < 64; ++d)
-      if (c->stage_votes[d] > 0.2f * c->level_votes[d]) stage |= 1ull << d;
+    // stage where more than 0.6 of the sources staged, both shapes where
+    // more than 0.1 did (from the costs above: a lone staging launch loses
+    // ~10 us per push it runs, a lone unstaged one ~40 us per heavy pull,
+    // a second launch costs ~4 us)
+    uint64_t stage = 0, dual = 0;
+    for (int d = 0; d < 64; ++d) {
+      const float lvv = c->level_votes[d];
+      if (c->stage_votes[d] > 0.6f * lvv) stage |= 1ull << d;
+      else if (c->stage_votes[d] > 0.1f * lvv) dual |= 1ull << d;
+      const float w = lvv > 0.f ? c->work_votes[d] / lvv : 0.f;
+      c->pred_work[d] = (uint8_t)std::min(63.f, std::ceil(w));
+    }
     c->pred_graph = g;
     c->pred_stage = stage;
+    c->pred_dual = dual;
     c->pred_levels = depth;
   }
   c->last_n = g->n;

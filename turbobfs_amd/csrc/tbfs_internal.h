@@ -85,7 +85,12 @@ struct alignas(64) RunStats {
   unsigned long long stage_mask;  // chained loop: bit min(d, 63) set when level d
                                   // was a pull that stages the frontier bitmap
   unsigned long long pad[2];
+  // chained loop: work of level d (d < 64): ceil(log2(1 + entries)) of the
+  // frontier's out-list entries (push) or the unexplored in-list entries
+  // (pull); 0 for levels not run
+  uint8_t work_lg[64];
 };
+constexpr int kWorkLevels = 64;
 
 struct GraphView {
   int32_t n;
@@ -175,6 +180,15 @@ struct DegAcc {
   unsigned long long bins[33];
 };
 
+// Chained level launches: kLevelAlways runs the level whatever it is; a pair
+// of launches for one step, kLevelIfStaged (staging shape) then
+// kLevelIfNotStaged, lets the level pick its shape on the device: each runs
+// it only when the level's staging decision matches, the other returns at
+// once.
+constexpr int32_t kLevelAlways = 0;
+constexpr int32_t kLevelIfStaged = 1;
+constexpr int32_t kLevelIfNotStaged = 2;
+
 // Kernel launch wrappers.  Every launch is made in the translation unit that
 // defines the kernel.
 namespace kern {
@@ -247,8 +261,9 @@ hipError_t chain_init(hipStream_t s, int grid, const PersistParams& p);
 hipError_t chain_seed(hipStream_t s, const PersistParams& p);
 // fbl_bytes > 0: 1024-thread workgroups staging the frontier bitmap
 // (fbl_bytes of dynamic LDS) for pull levels; 0: kLbBlock-thread workgroups.
+// want: which levels the launch runs (kLevelAlways, kLevelIfStaged, kLevelIfNotStaged).
 hipError_t chain_level(hipStream_t s, int grid, const PersistParams& p, int32_t d,
-                       size_t fbl_bytes);
+                       size_t fbl_bytes, int32_t want = kLevelAlways);
 // Largest bitmap (bytes) the LDS-staging variant can take; 0 if unavailable.
 hipError_t chain_fbl_limit(size_t* bytes);
 }  // namespace kern

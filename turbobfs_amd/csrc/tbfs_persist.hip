@@ -881,7 +881,10 @@ __device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned k) {
     if (last) {
       __hip_atomic_store(gen, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < k)
+      // relaxed polls: an acquire load invalidates the caches on every
+      // iteration, under the workgroups still running the level; the fence
+      // below is the one acquire
+      while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k)
         __builtin_amdgcn_s_sleep(1);
     }
     __threadfence();
@@ -1070,6 +1073,7 @@ __global__ __launch_bounds__(256) void k_chain_init(PersistParams p) {
     p.loop[0] = first_state(p);
     p.stats->levels = -1;  // set by the level that finds the frontier empty
     p.stats->fin_level = -1;
+    for (int i = 0; i < kWorkLevels; ++i) p.stats->work_lg[i] = 0;
     if (p.fin_ctr) {
       p.fin_ctr[0] = 0;
       p.fin_ctr[1] = 0;
@@ -1365,7 +1369,8 @@ __device__ __forceinline__ void fin_run(const PersistParams& p, LoopState& s, in
 // depth itself comes from the state, so one captured chunk of level kernels
 // serves every stretch of levels starting at an even depth.
 template <int BLOCK, bool FBL>
-__global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t step) {
+__global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t step,
+                                                       int32_t want) {
   __shared__ WaveLds lds[BLOCK / 64];
   extern __shared__ uint64_t fb_dyn[];
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1388,7 +1393,16 @@ __global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t 
     return;
   }
   const bool heavy_open = s.heavy_reached < (unsigned long long)p.nheavy;
-  if (p.fin_mu && s.mu <= (unsigned long long)p.fin_mu && p.mode == TBFS_MODE_AUTO) {
+  const bool fin = p.fin_mu && s.mu <= (unsigned long long)p.fin_mu && p.mode == TBFS_MODE_AUTO;
+  const bool stage = !fin && wants_stage(p, s.pushing != 0, s.mu);
+  // one of a pair of launches for this step: the other one runs the level
+  if (want == kLevelIfStaged && !stage) return;
+  if (want == kLevelIfNotStaged && stage) return;
+  if (gtid == 0 && d < kWorkLevels) {
+    const unsigned long long w = s.pushing ? c.mout : s.mu;
+    p.stats->work_lg[d] = (uint8_t)(64 - __clzll((long long)w));
+  }
+  if (fin) {
     // the rest of the BFS in the last workgroup to list its columns
     static_assert(sizeof(FinLds<BLOCK>) <= sizeof(lds), "the finisher's LDS fits the level's");
     FinLds<BLOCK>& F = *reinterpret_cast<FinLds<BLOCK>*>(lds);
@@ -1409,7 +1423,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t 
   nx.d = d + 1;
   // the host picks each launch's shape (staging workgroups or not) from the
   // levels that staged in its last BFS on this graph
-  if (wants_stage(p, s.pushing != 0, s.mu)) nx.stage_mask |= 1ull << (d < 63 ? d : 63);
+  if (stage) nx.stage_mask |= 1ull << (d < 63 ? d : 63);
   if (gtid == 0) p.loop[(step + 1) & 1] = nx;
   LevelAcc acc;
   run_level(p, d, s.pushing != 0, s.ovf_any, s.mu, c.hout != 0, heavy_open, gtid, gstride,
@@ -1475,11 +1489,13 @@ hipError_t chain_seed(hipStream_t s, const PersistParams& p) {
 }
 
 hipError_t chain_level(hipStream_t s, int grid, const PersistParams& p, int32_t d,
-                       size_t fbl_bytes) {
+                       size_t fbl_bytes, int32_t want) {
   if (fbl_bytes)
-    hipLaunchKernelGGL((k_chain_level<1024, true>), dim3(grid), dim3(1024), fbl_bytes, s, p, d);
+    hipLaunchKernelGGL((k_chain_level<1024, true>), dim3(grid), dim3(1024), fbl_bytes, s, p, d,
+                       want);
   else
-    hipLaunchKernelGGL((k_chain_level<kLbBlock, false>), dim3(grid), dim3(kLbBlock), 0, s, p, d);
+    hipLaunchKernelGGL((k_chain_level<kLbBlock, false>), dim3(grid), dim3(kLbBlock), 0, s, p, d,
+                       want);
   return hipGetLastError();
 }
 
