@@ -99,15 +99,18 @@ typedef struct tbfs_opts {
                           in LDS when the unexplored in-list entries exceed
                           stage_ratio * n */
   int32_t lb_grid;     /* chained loop: workgroups of the launches that do
-                          not stage the bitmap (0: about one wavefront per
-                          four frontier words) */
+                          not stage the bitmap; -1: one grid per level, from
+                          that level's work in the context's earlier runs on
+                          the graph (one workgroup per CU up to 2^18 list
+                          entries, two up to 2^22, else about one wavefront
+                          per four frontier words) */
   int32_t finish;      /* chained loop, AUTO mode: once at most this many
                           in-list entries are unexplored, one workgroup runs
                           the remaining levels (0 or -1: never; at most
                           65536 on graphs of more than 65536 vertices) */
   /* alpha, beta, stage_ratio and lb_grid left at 0 take the autotuner's
    * values for the graph's class (tbfs_autotune) when the table holds them,
-   * else alpha 14, beta 24, stage_ratio 1 and the default grid. */
+   * else alpha 14, beta 24, stage_ratio 1 and lb_grid -1. */
 } tbfs_opts;
 
 typedef struct tbfs_stats {
@@ -127,7 +130,7 @@ typedef struct tbfs_stats {
 
 /* Fill *o with the defaults: AUTO, load-balanced kernels, chained loop,
  * sigma on; alpha, beta, stage_ratio, lb_grid 0 (tuned, else 14, 24, 1 and
- * the default grid). */
+ * -1: a grid per level). */
 void tbfs_opts_default(tbfs_opts *o);
 
 /* Human-readable name of a status code. */
@@ -305,8 +308,8 @@ int tbfs_batch_autotune(tbfs_batch *b, const tbfs_graph *g, const int32_t *roots
 /* ---- autotuner -----------------------------------------------------------
  * Single-source runs: times `count` BFS from roots on g with the default
  * thresholds and launch shapes, then searches alpha {6,10,14,20,30}, beta
- * {8,16,24,48,96}, stage_ratio {0.25,0.5,1,2,4} and lb_grid {1,2,4,8
- * workgroups per CU, default} one at a time, twice round (best of two runs
+ * {8,16,24,48,96}, stage_ratio {0.25,0.5,1,2,4} and lb_grid {-1 (per
+ * level), 1, 2, 4 workgroups per CU, default} one at a time, twice round (best of two runs
  * per candidate).  Every candidate gives the same results, which is checked
  * on each source's level, vertex and edge counts.  *best gets the winner
  * (other fields from base, or the defaults), *best_ms / *default_ms the time

@@ -756,7 +756,7 @@ constexpr float kDefaultStageRatio = 1.0f;
 // table entry for g's class, when there is one (the others keep 0: defaults).
 static tbfs_opts resolve_opts(const tbfs_graph* g, const tbfs_opts& o, hipStream_t s) {
   tbfs_opts r = o;
-  if (r.alpha > 0 && r.beta > 0 && r.stage_ratio > 0 && r.lb_grid > 0) return r;
+  if (r.alpha > 0 && r.beta > 0 && r.stage_ratio > 0 && r.lb_grid != 0) return r;
   if (tune_empty() || !g->CP) return r;
   TuneKey k;
   TuneEntry e;
@@ -768,7 +768,7 @@ static tbfs_opts resolve_opts(const tbfs_graph* g, const tbfs_opts& o, hipStream
   if (r.alpha <= 0) r.alpha = e.alpha;
   if (r.beta <= 0) r.beta = e.beta;
   if (r.stage_ratio <= 0) r.stage_ratio = e.stage_ratio;
-  if (r.lb_grid <= 0) r.lb_grid = e.lb_grid;
+  if (r.lb_grid == 0) r.lb_grid = e.lb_grid;
   return r;
 }
 
@@ -841,6 +841,8 @@ static PersistParams make_params(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
   p.fin_mu = o.finish <= 0 ? 0u
              : (uint32_t)c->n_cap <= kFinCap ? (uint32_t)o.finish
                                              : std::min<uint32_t>((uint32_t)o.finish, kFinCap);
+  // one root, AUTO / PUSH: the init kernel may run depth 0 (chained loop)
+  p.step0 = (root >= 0 && (o.mode == TBFS_MODE_AUTO || o.mode == TBFS_MODE_PUSH)) ? 1 : 0;
   return p;
 }
 
@@ -893,8 +895,9 @@ struct LevelShape {
 //   - staging workgroups (1024 threads, one per CU, the bitmap in LDS) for
 //     the levels that staged for most sources; both shapes, each running the
 //     level only if the device's staging decision matches, for the levels
-//     that staged for some (an unstaged heavy pull costs ~40 us more on
-//     R-MAT scale 20, a staged push ~10 us, a launch that returns ~4 us);
+//     that staged for a good part of them (an unstaged heavy pull costs ~17
+//     us more on R-MAT scale 20, a staged push ~10 us, a launch that
+//     returns ~5 us);
 //   - 256-thread workgroups for the others, as many as the level's expected
 //     work calls for: one per CU up to 2^18 entries, two up to 2^22, else
 //     the default grid (a launch's dispatch and drain grow with its grid: a
@@ -987,15 +990,16 @@ static LevelPlan level_plan(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_o
   return pl;
 }
 
-// Chained loop, enqueue only: init plus `levels` level kernels.  The BFS has
-// finished within them iff stats_slot->levels >= 0 afterwards.
+// Chained loop, enqueue only: init plus the level kernels of steps
+// p.step0 .. levels - 1.  The BFS has finished within them iff
+// stats_slot->levels >= 0 afterwards.
 static int enqueue_chained(tbfs_ctx* c, const tbfs_graph* g, int32_t root,
                            const tbfs_opts& o, RunStats* stats_slot, int levels) {
   const PersistParams p = make_params(c, g, root, o, stats_slot);
   const LevelPlan pl = level_plan(c, g, o);
   hipStream_t s = c->stream;
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
-  for (int d = 0; d < levels; ++d) HIPCHK(chain_step(s, pl, p, d));
+  for (int d = p.step0; d < levels; ++d) HIPCHK(chain_step(s, pl, p, d));
   return TBFS_OK;
 }
 
@@ -1012,7 +1016,7 @@ static int run_chained_p(tbfs_ctx* c, const tbfs_graph* g, const PersistParams& 
   // chunks of level kernels: chain_k, then growing by half up to 256, so a
   // BFS far deeper than the budget costs few host checks and at most one
   // chunk of spare launches
-  int32_t d = 0;
+  int32_t d = p.step0;
   int chunk = std::max(c->chain_k, 8);
   for (;;) {
     for (int i = 0; i < chunk; ++i) HIPCHK(chain_step(s, pl, p, d + i));
@@ -1370,15 +1374,15 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
       }
       depth = std::max(depth, (int)r.levels);
     }
-    // stage where more than 0.6 of the sources staged, both shapes where
-    // more than 0.1 did (from the costs above: a lone staging launch loses
-    // ~10 us per push it runs, a lone unstaged one ~40 us per heavy pull,
-    // a second launch costs ~4 us)
+    // stage where more than 0.45 of the sources staged, both shapes where
+    // more than 0.27 did: on R-MAT scale 20 a lone staging launch loses
+    // ~10 us per push it runs, a lone unstaged one ~17 us per heavy pull,
+    // and the second launch of a pair costs ~5.5 us
     uint64_t stage = 0, dual = 0;
     for (int d = 0; d < 64; ++d) {
       const float lvv = c->level_votes[d];
-      if (c->stage_votes[d] > 0.6f * lvv) stage |= 1ull << d;
-      else if (c->stage_votes[d] > 0.1f * lvv) dual |= 1ull << d;
+      if (c->stage_votes[d] > 0.45f * lvv) stage |= 1ull << d;
+      else if (c->stage_votes[d] > 0.27f * lvv) dual |= 1ull << d;
       const float w = lvv > 0.f ? c->work_votes[d] / lvv : 0.f;
       c->pred_work[d] = (uint8_t)std::min(63.f, std::ceil(w));
     }
@@ -1517,7 +1521,7 @@ int autotune_single(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int 
   o.alpha = kDefaultAlpha;
   o.beta = kDefaultBeta;
   o.stage_ratio = kDefaultStageRatio;
-  o.lb_grid = default_lb_grid(g);
+  o.lb_grid = -1;  // one grid per level, from each level's work
   std::vector<tbfs_stats> st((size_t)count), ref((size_t)count);
   auto eval = [&](const tbfs_opts& t, double* ms, bool is_ref) -> int {
     double best_t = -1;
@@ -1550,7 +1554,7 @@ int autotune_single(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int 
   const float alphas[] = {6.f, 10.f, 14.f, 20.f, 30.f};
   const float betas[] = {8.f, 16.f, 24.f, 48.f, 96.f};
   const float ratios[] = {0.25f, 0.5f, 1.f, 2.f, 4.f};
-  const int grids[] = {cus, 2 * cus, 4 * cus, 8 * cus, default_lb_grid(g)};
+  const int grids[] = {-1, cus, 2 * cus, 4 * cus, default_lb_grid(g)};
   for (int round = 0; round < 2; ++round) {
     for (int dim = 0; dim < 4; ++dim) {
       for (int k = 0; k < 5; ++k) {

@@ -165,7 +165,15 @@ struct PersistParams {
   uint32_t* fin_list[2];
   unsigned* fin_ctr;
   uint32_t fin_mu;
+  // chained loop: 1 when the init kernel may run depth 0 itself (one root,
+  // AUTO / PUSH): the level launches then start at step 1 and the init
+  // kernel writes loop slot 1.  0: depth 0 is the first level launch's.
+  int32_t step0;
 };
+
+// The init kernel runs depth 0 itself (a push from the root) when the root
+// has at most this many out-list entries: one per thread of a workgroup.
+constexpr int kInitPush = 256;
 
 constexpr uint32_t kFinCap = 1u << 16;  // entries of each finisher list (at least)
 

@@ -144,6 +144,7 @@ int tbfs_bfs_noalloc(const tbfs_graph* g, int32_t root, int32_t* S, uint64_t* si
   p.beta = o.beta > 0 ? o.beta : 24.0f;
   p.stage_ratio = o.stage_ratio > 0 ? o.stage_ratio : 1.0f;
   p.loop = reinterpret_cast<LoopState*>(base + kOffLoop);
+  p.step0 = (o.mode == TBFS_MODE_AUTO || o.mode == TBFS_MODE_PUSH) ? 1 : 0;
 
   // launch shapes: pull levels stage the frontier bitmap in LDS when it fits
   const int64_t nwords = ((int64_t)g->n + 63) / 64;
@@ -158,7 +159,7 @@ int tbfs_bfs_noalloc(const tbfs_graph* g, int32_t root, int32_t* S, uint64_t* si
   hipStream_t s = static_cast<hipStream_t>(stream);
   const auto t0 = std::chrono::steady_clock::now();
   HIPCHK(kern::chain_init(s, grid_for(g->n, 256, 8192), p));
-  int32_t d = 0;
+  int32_t d = p.step0;
   int chunk = std::max(ds.chain_k, 2);
   for (;;) {
     for (int i = 0; i < chunk; ++i) {

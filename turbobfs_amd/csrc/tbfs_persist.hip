@@ -1064,16 +1064,143 @@ __global__ __launch_bounds__(BLOCK) void k_persist(PersistParams p) {
   if (gtid == 0) write_stats(p, s, -1);
 }
 
-// Chained loop: state of depth 0 and the first loop-state slot.
+// Whether out-list entry t (of the root's, in nb[0 .. kInitPush), -1 past its
+// end) is a vertex other than the root that no earlier entry repeats.
+__device__ __forceinline__ bool v_first(const int32_t* nb, int t, int r) {
+  const int v = nb[t];
+  if (v < 0 || v == r) return false;
+  for (int j = 0; j < t; ++j)
+    if (nb[j] == v) return false;
+  return true;
+}
+
+// The state after depth 0 of a single-root BFS (the root's own counters
+// folded in); reports whether the init kernel runs depth 0 itself: a push
+// (as the first level launch would choose), from a root with at most
+// kInitPush out-list entries, with no finisher taking over.  Every thread
+// computes it from the same inputs and gets the same answer.
+__device__ __forceinline__ bool init_runs_depth0(const PersistParams& p, LoopState& s) {
+  s = first_state(p);
+  if (p.step0 != 1 || p.root < 0 || p.nroots > 0) return false;
+  const GraphView& g = p.g;
+  const int r = p.root;
+  LevelCtr c{};
+  c.q = 1;
+  c.mout = outdeg(g, r);
+  c.min = indeg(g, r);
+  const unsigned long long hb = hub_bits(g, r);
+  c.hin = (uint32_t)(hb & 0xffffffffull);
+  c.hout = (uint32_t)(hb >> 32);
+  c.ovf = 0;
+  step_state(p, &c, s);
+  const bool fin = p.fin_mu && s.mu <= (unsigned long long)p.fin_mu && p.mode == TBFS_MODE_AUTO;
+  return s.pushing && !fin && c.mout <= (unsigned long long)kInitPush;
+}
+
+// Chained loop: state of depth 0 and the first loop-state slot (p.step0 & 1).
+// When init_runs_depth0 holds, the kernel also runs depth 0: every
+// workgroup loads the root's out-list (one entry per thread) and, for the
+// vertices it initialises, counts their edges from the root in LDS, so each
+// vertex is written once, already holding its depth-1 values (S = 1, count =
+// number of edges from the root); the depth-1 frontier words come from the
+// waves' ballots, and the root joins the reached bitmap.  Workgroup 0 tallies
+// the distinct depth-1 vertices into depth 1's counters and writes loop slot
+// 1 with depth 1 next.
 __global__ __launch_bounds__(256) void k_chain_init(PersistParams p) {
+  __shared__ int32_t nb[kInitPush];
+  __shared__ uint32_t cnt[256];
+  __shared__ unsigned long long red[4][4];
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-  init_state(p, gtid, gstride);
+  LoopState s1;
+  const bool fuse = init_runs_depth0(p, s1);
+  if (!fuse) {
+    init_state(p, gtid, gstride);
+  } else {
+    const GraphView& g = p.g;
+    const int32_t n = g.n;
+    const int r = p.root;
+    const int rb = g.RP[r];
+    const int deg = (int)outdeg(g, r);
+    nb[threadIdx.x] = threadIdx.x < deg ? g.JR[rb + threadIdx.x] : -1;
+    __syncthreads();
+    // vertices: the 256-aligned runs blockIdx * 256 + k * gstride
+    for (int64_t base = (int64_t)blockIdx.x * 256; base < n; base += gstride) {
+      cnt[threadIdx.x] = 0;
+      __syncthreads();
+      const int v = nb[threadIdx.x];
+      if (v >= 0 && v != r && v >= base && v < base + 256) atomicAdd(&cnt[v - base], 1u);
+      __syncthreads();
+      const int64_t i = base + threadIdx.x;
+      const uint32_t k = cnt[threadIdx.x];
+      if (i < n) {
+        p.S[i] = i == r ? 0 : (k ? 1 : -1);
+        p.sigma[i] = i == r ? 1ull : (unsigned long long)k;
+        p.ovf[i] = 0;
+      }
+      const uint64_t word = __ballot(i < n && k != 0);
+      if (lane() == 0 && i < n) p.fb[1][i >> 6] = word;
+      __syncthreads();
+    }
+    const int64_t nwords = ((int64_t)n + 63) >> 6;
+    for (int64_t w = gtid; w < nwords; w += gstride) {
+      const uint64_t rw = (w == (r >> 6)) ? (1ull << (r & 63)) : 0ull;
+      p.fb[0][w] = rw;
+      p.fb[2][w] = 0ull;
+      p.vis[w] = rw;  // depth 0 is reached
+    }
+    if (blockIdx.x == 0) {
+      // counters: depth 1's in set 1, sets 0 and 2 cleared
+      if (threadIdx.x < 3 * kStripes) {
+        CtrStripe* c = &p.cset[threadIdx.x / kStripes].s[threadIdx.x % kStripes];
+        c->qm = 0;
+        c->min = 0;
+        c->hubs = 0;
+        c->ovf = 0;
+      }
+      // the distinct depth-1 vertices: first occurrence in the out-list
+      bool first = v_first(nb, threadIdx.x, r);
+      const int v = first ? nb[threadIdx.x] : 0;
+      const uint32_t di = first ? indeg(g, v) : 0u;
+      const uint32_t dout = first ? outdeg(g, v) : 0u;
+      const uint64_t q = wave_sum_u64(first ? 1u : 0u);
+      const uint64_t mo = wave_sum_u64(dout);
+      const uint64_t mi = wave_sum_u64(di);
+      const uint64_t hb = wave_sum_u64(
+          ((uint64_t)(dout > (uint32_t)kBigIn ? 1u : 0u) << 32) | (di > (uint32_t)kBigIn ? 1u : 0u));
+      if (lane() == 0) {
+        red[threadIdx.x >> 6][0] = q;
+        red[threadIdx.x >> 6][1] = mo;
+        red[threadIdx.x >> 6][2] = mi;
+        red[threadIdx.x >> 6][3] = hb;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        unsigned long long t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+        for (int j = 0; j < 4; ++j) {
+          t0 += red[j][0];
+          t1 += red[j][1];
+          t2 += red[j][2];
+          t3 += red[j][3];
+        }
+        CtrStripe* c = &p.cset[1].s[0];
+        c->qm = (t0 << 32) + t1;
+        c->min = t2;
+        c->hubs = t3;
+      }
+    }
+  }
   if (gtid == 0) {
-    p.loop[0] = first_state(p);
+    LoopState s = first_state(p);
+    if (fuse) {
+      s = s1;
+      s.d = 1;
+    }
+    p.loop[p.step0 & 1] = s;
     p.stats->levels = -1;  // set by the level that finds the frontier empty
     p.stats->fin_level = -1;
     for (int i = 0; i < kWorkLevels; ++i) p.stats->work_lg[i] = 0;
+    if (fuse) p.stats->work_lg[0] = (uint8_t)(64 - __clzll((long long)outdeg(p.g, p.root)));
     if (p.fin_ctr) {
       p.fin_ctr[0] = 0;
       p.fin_ctr[1] = 0;

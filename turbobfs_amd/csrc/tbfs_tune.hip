@@ -105,7 +105,7 @@ int from_json(const std::string& s, std::vector<TuneEntry>* out) {
     if (field(b, e, "ms", &v)) x.ms = v;
     if (field(b, e, "default_ms", &v)) x.default_ms = v;
     if (x.grid_per_cu < 0 || x.batch_alpha < 0 || x.alpha < 0 || x.beta < 0 ||
-        x.stage_ratio < 0 || x.lb_grid < 0)
+        x.stage_ratio < 0 || x.lb_grid < -1)
       return TBFS_ERR_INVALID;
     t.push_back(x);
     p = e + 1;
