@@ -420,6 +420,12 @@ int tbfs_server_poll(tbfs_server *s, tbfs_query_done *done, int32_t max,
                      int32_t *ndone, int64_t wait_us);
 /* Block until every submitted query has completed. */
 int tbfs_server_drain(tbfs_server *s);
+/* Hold (hold != 0) or release (0) the dispatch of queued queries: while
+ * held, queries submitted wait in their queues and running continuous
+ * batches admit none, so a group submitted together is taken together (up
+ * to each graph's batch cap) once released; jobs already running go on.
+ * tbfs_server_drain releases a hold. */
+int tbfs_server_hold(tbfs_server *s, int32_t hold);
 /* Sources per batch for graphs[graph] (graph >= 0), batches started and
  * queries they held so far, and the server clock. */
 int tbfs_server_info(tbfs_server *s, int32_t graph, int32_t *batch_cap,

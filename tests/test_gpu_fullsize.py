@@ -266,6 +266,9 @@ def test_serving_benchmark_graph_set(gpu):
     rng = np.random.default_rng(5)
     store = {}
     capped = {}
+    # the whole set is queued before any of it runs: the graphs with more
+    # queries than their cap then run batches at the cap
+    srv.hold(True)
     for gi, g in enumerate(cs):
         n = g.n
         extra = 2 * caps[gi] if caps[gi] < 1024 else 0  # unchecked: fill the cap
@@ -280,6 +283,7 @@ def test_serving_benchmark_graph_set(gpu):
             bufs = (tb.host_array(n, np.int32), tb.host_array(n, np.uint64),
                     tb.host_array(n, np.uint8))
             store[srv.submit(gi, r, *bufs)] = (gi, r) + bufs
+    srv.hold(False)
     total = len(store) + len(capped)
     done = []
     while len(done) < total:

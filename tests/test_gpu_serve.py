@@ -239,3 +239,36 @@ def test_serve_continuous_batches_on_every_graph(gpu):
     print(f"[serve] seeded mid-batch: {sum(1 for x in levels if x > 0)} of 300", flush=True)
     assert any(x > 0 for x in levels)
     srv.close()
+
+
+def test_serve_hold_batches_a_group(gpu):
+    """While dispatch is held nothing runs; a group submitted meanwhile is
+    taken together once released (one batch per graph, up to its cap), every
+    result exact; tbfs_server_drain releases a hold."""
+    cs, gs = _graph_set()
+    srv = tb.Server(gs[:2])
+    rng = np.random.default_rng(9)
+    store = {}
+    srv.hold(True)
+    for gi in (0, 1):
+        for r in rng.integers(0, cs[gi].n, 40):
+            _submit(srv, cs[gi], gi, int(r), store)
+    assert srv.poll(64, wait_us=50000) == []  # held: nothing ran
+    srv.hold(False)
+    done = []
+    while len(done) < len(store):
+        done += srv.poll(256, wait_us=100000)
+    _check(cs, store, done)
+    for gi in (0, 1):
+        sizes = {d["batch"] for d in done if d["graph"] == gi}
+        assert sizes == {40}, (gi, sizes)  # the whole group in one batch
+    # a drain while held releases the hold and returns with everything done
+    srv.hold(True)
+    more = {}
+    for r in rng.integers(0, cs[0].n, 5):
+        _submit(srv, cs[0], 0, int(r), more)
+    srv.drain()
+    got = srv.poll(64)
+    assert len(got) == 5
+    _check(cs, more, got)
+    srv.close()

@@ -476,6 +476,11 @@ class Server(_Handle):
             self._bufs[int(t.value)] = (S, sigma, ovf)
         return int(t.value)
 
+    def hold(self, on: bool = True) -> None:
+        """Hold (or release) dispatch: queries submitted while held are taken
+        together once released (up to each graph's batch cap)."""
+        check(_lib.load().tbfs_server_hold(self._h, 1 if on else 0), "tbfs_server_hold")
+
     def poll(self, max_items: int = 1024, wait_us: int = 0) -> list[dict]:
         buf = (_lib.QueryDone * max_items)()
         k = C.c_int32()

@@ -13,6 +13,8 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
+#include <vector>
 
 #include "bfsbucsc.h"
 #include "bfstdbucsc.h"
@@ -51,6 +53,50 @@ tbfs_opts fast_opts(bool sigma) {
   return o;
 }
 
+// Contexts of finished calls, kept for the next ones (a context's stream,
+// pinned host words and level-loop state cost ~1 ms to create and free).  A
+// call takes the smallest idle context of its device that holds its graph,
+// or creates one; at most kPoolMax stay idle.
+constexpr size_t kPoolMax = 4;
+std::mutex g_pool_mu;
+std::vector<tbfs_ctx*> g_pool;
+
+int pool_take(int32_t n, tbfs_ctx** out) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    size_t best = g_pool.size();
+    for (size_t i = 0; i < g_pool.size(); ++i)
+      if (g_pool[i]->device == dev && g_pool[i]->n_cap >= n &&
+          (best == g_pool.size() || g_pool[i]->n_cap < g_pool[best]->n_cap))
+        best = i;
+    if (best < g_pool.size()) {
+      tbfs_ctx* c = g_pool[best];
+      g_pool.erase(g_pool.begin() + (std::ptrdiff_t)best);
+      // launch plan history belongs to graphs of earlier calls
+      c->pred_graph = nullptr;
+      c->chain_k = 8;
+      *out = c;
+      return TBFS_OK;
+    }
+  }
+  return ctx_create(n, out);
+}
+
+void pool_give(tbfs_ctx* c) {
+  tbfs_ctx* drop = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool.push_back(c);
+    if (g_pool.size() > kPoolMax) {
+      drop = g_pool.front();
+      g_pool.erase(g_pool.begin());
+    }
+  }
+  if (drop) ctx_free(drop);
+}
+
 // Conversion of the exact counts at the boundary (k_dropin_convert, on the
 // device): float for the TD CSC and COOC entry points (counts above 2^24, or
 // >= 2^64, are not exact in float), int32 modulo 2^32 for TDBU (the
@@ -74,6 +120,9 @@ int run_dropin(const char* name, const int* A, const int* B, int nz, int n, int 
   }
   using clk = std::chrono::steady_clock;
   const auto t0 = clk::now();
+  // every device buffer of the call comes from (and goes back to) the reuse
+  // cache: nothing below frees a buffer with work pending on it
+  DevReuse reuse;
   tbfs_graph* g = nullptr;
   tbfs_ctx* c = nullptr;
   int rc = cooc ? graph_create_cooc(A, B, n, nz, 0u, &g)
@@ -84,7 +133,7 @@ int run_dropin(const char* name, const int* A, const int* B, int nz, int n, int 
                  tbfs_status_str(rc));
     return rc;
   }
-  rc = ctx_create(n, &c);
+  rc = pool_take(n, &c);
   if (rc) {
     graph_free(g);
     return rc;
@@ -107,7 +156,11 @@ int run_dropin(const char* name, const int* A, const int* B, int nz, int n, int 
   int64_t inexact = 0;
   if (!rc) rc = ctx_read_converted(c, n, s_unreached, kind, S_h, val_h, &inexact);
   const auto t4 = clk::now();
-  ctx_free(c);
+  if (rc) {
+    ctx_free(c);  // its state is unknown after a failure
+  } else {
+    pool_give(c);
+  }
   graph_free(g);
   const auto t5 = clk::now();
   auto ms = [](clk::time_point a, clk::time_point b) {

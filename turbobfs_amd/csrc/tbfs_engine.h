@@ -119,21 +119,38 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
     if (e_ != hipSuccess) return hip_fail(e_, #call, __FILE__, __LINE__); \
   } while (0)
 
+// Device memory.  While a DevReuse object lives on the calling thread,
+// allocations are taken from a per-device cache of freed blocks when one of
+// a fitting size is there, and blocks handed out under it go back to the
+// cache when freed (they must have no work pending on the device by then);
+// otherwise both go straight to the runtime.  The reference's entry points
+// upload, run and free everything on every call: with the cache, calls of
+// the same sizes pay no hipMalloc / hipFree (each of which waits for the
+// device).  The cache holds at most kDevReuseCap bytes; it is kept until
+// the process ends.
+int dev_alloc_bytes(void** p, size_t bytes);
+void dev_free_bytes(void* p);
+constexpr size_t kDevReuseCap = size_t(4) << 30;
+struct DevReuse {
+  DevReuse();
+  ~DevReuse();
+  DevReuse(const DevReuse&) = delete;
+  DevReuse& operator=(const DevReuse&) = delete;
+};
+
 template <typename T>
 inline int dev_alloc(T** p, size_t count) {
-  *p = nullptr;
   if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
-  }
-  return TBFS_OK;
+  void* q = nullptr;
+  const int rc = dev_alloc_bytes(&q, count * (sizeof(T) > 0 ? sizeof(T) : 1));
+  *p = static_cast<T*>(q);
+  return rc;
 }
+inline int dev_alloc(void** p, size_t bytes) { return dev_alloc_bytes(p, bytes ? bytes : 1); }
 
 template <typename T>
 inline void dev_free(T*& p) {
-  if (p) (void)hipFree((void*)p);
+  if (p) dev_free_bytes((void*)p);
   p = nullptr;
 }
 

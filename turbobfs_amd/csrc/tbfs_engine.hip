@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <unordered_map>
 #include <new>
 #include <thread>
 #include <vector>
@@ -29,6 +30,113 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
   std::fprintf(stderr, "turbobfs: %s failed: %s (%s:%d)\n", what,
                hipGetErrorString(e), file, line);
   return e == hipErrorOutOfMemory ? TBFS_ERR_NOMEM : TBFS_ERR_HIP;
+}
+
+// ---- device allocations with a reuse cache (see DevReuse) -------------------
+namespace {
+thread_local int t_reuse = 0;
+struct CachedBlock {
+  int dev;
+  size_t bytes;
+  void* p;
+};
+std::mutex g_dc_mu;
+std::unordered_map<void*, CachedBlock> g_dc_out;  // handed out under DevReuse
+std::vector<CachedBlock> g_dc_free;               // freed, kept for reuse (oldest first)
+size_t g_dc_free_bytes = 0;
+
+// Release cached blocks (oldest first) until at most `keep` bytes stay
+// cached; dev >= 0: only blocks of that device.  Caller does not hold the lock.
+void dc_trim(size_t keep, int dev) {
+  std::vector<CachedBlock> victims;
+  {
+    std::lock_guard<std::mutex> lk(g_dc_mu);
+    for (size_t i = 0; i < g_dc_free.size() && g_dc_free_bytes > keep;) {
+      if (dev >= 0 && g_dc_free[i].dev != dev) {
+        ++i;
+        continue;
+      }
+      victims.push_back(g_dc_free[i]);
+      g_dc_free_bytes -= g_dc_free[i].bytes;
+      g_dc_free.erase(g_dc_free.begin() + (std::ptrdiff_t)i);
+    }
+  }
+  int cur = -1;
+  if (!victims.empty()) (void)hipGetDevice(&cur);
+  for (const CachedBlock& b : victims) {
+    if (b.dev != cur) (void)hipSetDevice(b.dev);
+    (void)hipFree(b.p);
+    if (b.dev != cur) (void)hipSetDevice(cur);
+  }
+}
+}  // namespace
+
+DevReuse::DevReuse() { ++t_reuse; }
+DevReuse::~DevReuse() { --t_reuse; }
+
+int dev_alloc_bytes(void** p, size_t bytes) {
+  *p = nullptr;
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (t_reuse > 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    {
+      // the smallest cached block of this device that fits, at most 1/8 over
+      std::lock_guard<std::mutex> lk(g_dc_mu);
+      size_t best = g_dc_free.size();
+      for (size_t i = 0; i < g_dc_free.size(); ++i) {
+        const CachedBlock& b = g_dc_free[i];
+        if (b.dev == dev && b.bytes >= bytes && b.bytes <= bytes + bytes / 8 &&
+            (best == g_dc_free.size() || b.bytes < g_dc_free[best].bytes))
+          best = i;
+      }
+      if (best < g_dc_free.size()) {
+        const CachedBlock b = g_dc_free[best];
+        g_dc_free.erase(g_dc_free.begin() + (std::ptrdiff_t)best);
+        g_dc_free_bytes -= b.bytes;
+        g_dc_out[b.p] = b;
+        *p = b.p;
+        return TBFS_OK;
+      }
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {  // cached blocks hold the memory: give them back
+      (void)hipGetLastError();
+      dc_trim(0, dev);
+      e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) {
+      *p = nullptr;
+      return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+    }
+    std::lock_guard<std::mutex> lk(g_dc_mu);
+    g_dc_out[*p] = CachedBlock{dev, bytes, *p};
+    return TBFS_OK;
+  }
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+  }
+  return TBFS_OK;
+}
+
+void dev_free_bytes(void* p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_dc_mu);
+    auto it = g_dc_out.find(p);
+    if (it != g_dc_out.end()) {
+      g_dc_free.push_back(it->second);
+      g_dc_free_bytes += it->second.bytes;
+      g_dc_out.erase(it);
+      if (g_dc_free_bytes <= kDevReuseCap) return;
+    } else {
+      (void)hipFree(p);
+      return;
+    }
+  }
+  dc_trim(kDevReuseCap, -1);
 }
 
 }  // namespace tbfs
@@ -92,7 +200,7 @@ int graph_build_transpose(tbfs_graph* g, hipStream_t s) {
 out:
   dev_free(col);
   dev_free(keys_out);
-  if (tmp) (void)hipFree(tmp);
+  dev_free(tmp);
   return rc;
 }
 
@@ -440,7 +548,7 @@ int build_slices(const int32_t* ptr, int32_t n, int32_t min_deg, hipStream_t s, 
   }
   dev_free(cnt);
   dev_free(off);
-  if (tmp) (void)hipFree(tmp);
+  dev_free(tmp);
   return rc;
 }
 
@@ -561,7 +669,7 @@ static int graph_build_view(tbfs_graph* g, hipStream_t s) {
   g->bperm = perm;
   perm = nullptr;
 out:
-  if (tmp) (void)hipFree(tmp);
+  dev_free(tmp);
   dev_free(keys);
   dev_free(keys_out);
   dev_free(ids);

@@ -125,6 +125,7 @@ struct tbfs_server {
   std::mutex mu;  // everything above (and the slots' job hand-over)
   std::condition_variable cv_work, cv_done;
   std::atomic<bool> stop{false};
+  bool held = false;      // tbfs_server_hold: queued queries wait (guarded by mu)
   int64_t in_flight = 0;  // submitted, not yet completed
   uint64_t next_ticket = 1;
   int device = 0;
@@ -481,7 +482,7 @@ void run_open(tbfs_server* s, Slot& sl, Job& job) {
       std::deque<Query>& pq = s->pending[g];
       // at most the graph's batch capacity running at once (max_batch)
       const bool room = batch_open_depth(sl.b) < (1 << 29) - 4 * kOpenChunk &&
-                        !s->stop.load(std::memory_order_relaxed);
+                        !s->stop.load(std::memory_order_relaxed) && !s->held;
       const size_t lim = (size_t)std::max(0, s->cap[g] - running);
       const size_t k = room ? std::min({pq.size(), free_bits.size(), lim}) : 0;
       add.assign(pq.begin(), pq.begin() + k);
@@ -518,7 +519,7 @@ void worker(tbfs_server* s, Slot* sl) {
   std::unique_lock<std::mutex> lk(s->mu);
   while (!s->stop.load(std::memory_order_relaxed)) {
     Job job;
-    if (!pick(s, &job)) {
+    if (s->held || !pick(s, &job)) {
       s->cv_work.wait_for(lk, std::chrono::milliseconds(2));
       continue;
     }
@@ -700,7 +701,21 @@ int tbfs_server_poll(tbfs_server* s, tbfs_query_done* done, int32_t max, int32_t
 int tbfs_server_drain(tbfs_server* s) {
   if (!s) return TBFS_ERR_INVALID;
   std::unique_lock<std::mutex> lk(s->mu);
+  if (s->held) {  // held queries would never finish
+    s->held = false;
+    s->cv_work.notify_all();
+  }
   s->cv_done.wait(lk, [s] { return s->in_flight == 0; });
+  return TBFS_OK;
+}
+
+int tbfs_server_hold(tbfs_server* s, int32_t hold) {
+  if (!s) return TBFS_ERR_INVALID;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->held = hold != 0;
+  }
+  if (!hold) s->cv_work.notify_all();
   return TBFS_OK;
 }
 
