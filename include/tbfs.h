@@ -448,6 +448,12 @@ int tbfs_server_info(tbfs_server *s, int32_t graph, int32_t *batch_cap,
 #define TBFS_DROPIN_REFERENCE 1
 int tbfs_dropin_mapping(int mapping);
 
+/* The seven entry points keep what one call allocates for the next: idle BFS
+ * contexts (up to 4) and the device buffers of their graph uploads (up to
+ * 4 GiB), so repeated calls pay no device allocation or free.  This releases
+ * all of it (calls in progress keep theirs); later calls allocate again. */
+int tbfs_dropin_release(void);
+
 /* Pinned host memory (hipHostMalloc) for result buffers. */
 void *tbfs_host_alloc(int64_t bytes);
 int tbfs_host_free(void *p);

@@ -408,3 +408,16 @@ def test_finisher_takes_the_tail(gpu):
         assert runs["off"] == lv + 1 and runs["default"] == lv + 1, runs
         assert runs["early"] < runs["off"], runs
     ctx.close()
+
+
+def test_dropin_calls_reuse_and_release(gpu):
+    """Back-to-back calls of an entry point on graphs of different sizes (the
+    reused contexts and buffers must not carry anything from one call to the
+    next), then a release of what the calls keep, then calls again."""
+    small = _er4096()
+    big = graphs.rmat(14, 16, 0.57, 0.19, 0.19, seed=2, relabel=True)
+    for rep in range(2):
+        for g, r in ((big, 3), (small, 0), (big, 11), (small, 7)):
+            print(f"[dropin reuse] rep={rep} n={g.n} root={r}", flush=True)
+            _check_dropin(g, r, mappings=(tb._lib.DROPIN_FAST,))
+        tb.dropin_release()

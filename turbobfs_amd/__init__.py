@@ -24,6 +24,7 @@ from .bfs import (  # noqa: F401
     batch_tuned,
     default_opts,
     dropin_mapping,
+    dropin_release,
     host_array,
     server_budget_for,
     tune_clear,
@@ -41,5 +42,5 @@ __all__ = [
     "TbfsError",
     "bfs_gpu_td_csc_sc", "bfs_gpu_td_csc_wa", "bfs_gpu_bu_csc_sc",
     "bfs_gpu_bu_csc_wa", "bfs_gpu_tdbu_csc_sc", "bfs_gpu_tdbu_csc_wa",
-    "bfs_gpu_td_cooc_sc", "dropin_mapping",
+    "bfs_gpu_td_cooc_sc", "dropin_mapping", "dropin_release",
 ]

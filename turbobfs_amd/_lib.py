@@ -174,6 +174,7 @@ SIGNATURES = {
     "tbfs_server_submit": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _i32p, _u64p, _u8p, C.POINTER(C.c_uint64)]),
     "tbfs_server_poll": (C.c_int, [C.c_void_p, C.POINTER(QueryDone), C.c_int32, C.POINTER(C.c_int32), C.c_int64]),
     "tbfs_server_drain": (C.c_int, [C.c_void_p]),
+    "tbfs_dropin_release": (C.c_int, []),
     "tbfs_server_hold": (C.c_int, [C.c_void_p, C.c_int32]),
     "tbfs_server_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_double)]),

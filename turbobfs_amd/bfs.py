@@ -117,6 +117,12 @@ def bfs_gpu_td_cooc_sc(I_h, J_h, sigma_h, r, nz, n, repetition) -> int:
         _ptr(_req(sigma_h, np.float32, n, "sigma_h"), _f32p), r, nz, n, repetition)
 
 
+def dropin_release() -> None:
+    """Free the contexts and device buffers the seven entry points keep
+    between calls (later calls allocate again)."""
+    check(_lib.load().tbfs_dropin_release(), "tbfs_dropin_release")
+
+
 def dropin_mapping(mapping: int = -1) -> int:
     """Select the kernels behind the seven entry points above (process-wide):
     _lib.DROPIN_FAST (default) or _lib.DROPIN_REFERENCE (the reference's

@@ -233,6 +233,17 @@ int tdbu_csc(const char* name, int* I_h, int* CP_h, int* S_h, int* sigma_h,
 
 extern "C" {
 
+int tbfs_dropin_release(void) {
+  std::vector<tbfs_ctx*> idle;
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    idle.swap(g_pool);
+  }
+  for (tbfs_ctx* c : idle) ctx_free(c);
+  dev_reuse_release();
+  return TBFS_OK;
+}
+
 int tbfs_dropin_mapping(int mapping) {
   if (mapping != TBFS_DROPIN_FAST && mapping != TBFS_DROPIN_REFERENCE && mapping != -1)
     return TBFS_ERR_INVALID;

@@ -130,6 +130,7 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 // the process ends.
 int dev_alloc_bytes(void** p, size_t bytes);
 void dev_free_bytes(void* p);
+void dev_reuse_release();  // frees every cached block (blocks in use stay)
 constexpr size_t kDevReuseCap = size_t(4) << 30;
 struct DevReuse {
   DevReuse();

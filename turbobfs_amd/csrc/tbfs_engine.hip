@@ -74,6 +74,8 @@ void dc_trim(size_t keep, int dev) {
 DevReuse::DevReuse() { ++t_reuse; }
 DevReuse::~DevReuse() { --t_reuse; }
 
+void dev_reuse_release() { dc_trim(0, -1); }
+
 int dev_alloc_bytes(void** p, size_t bytes) {
   *p = nullptr;
   bytes = (bytes + 255) & ~(size_t)255;
