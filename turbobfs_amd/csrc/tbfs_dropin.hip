@@ -74,8 +74,7 @@ int pool_take(int32_t n, tbfs_ctx** out) {
     if (best < g_pool.size()) {
       tbfs_ctx* c = g_pool[best];
       g_pool.erase(g_pool.begin() + (std::ptrdiff_t)best);
-      // launch plan history belongs to graphs of earlier calls
-      c->pred_graph = nullptr;
+      // the launch budget was learned on graphs of earlier calls
       c->chain_k = 8;
       *out = c;
       return TBFS_OK;

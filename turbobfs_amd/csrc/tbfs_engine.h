@@ -4,12 +4,28 @@
 #include <stdint.h>
 
 #include <atomic>
+#include <mutex>
 #include <vector>
 
 #include "tbfs.h"
 #include "tbfs_internal.h"
 
 // A graph resident in HBM.
+// What the chained loop's launch plan learns about a graph from the BFS run
+// on it (by any context): decayed per-level votes of the sources so far
+// (each weighs 0.95 of the one after it) and what they imply.
+struct PlanHistory {
+  std::mutex mu;
+  float stage_votes[64] = {};  // sources whose level d staged the bitmap
+  float level_votes[64] = {};  // sources whose BFS ran level d at all
+  float work_votes[64] = {};   // sum of their level d's work (log2 entries)
+  uint64_t stage = ~0ull;      // levels launched on the staging shape
+  uint64_t dual = 0;           // levels launched in both shapes
+  uint8_t work[64] = {};       // expected log2 work of level d (0: unknown)
+  int levels = 0;              // deepest BFS so far
+  bool valid = false;          // no BFS learned from yet: every launch stages
+};
+
 struct tbfs_graph {
   int32_t n = 0;
   int32_t nz = 0;
@@ -43,6 +59,7 @@ struct tbfs_graph {
   tbfs_graph* bview = nullptr;
   int32_t* bperm = nullptr;
   int bview_default = -1;
+  mutable PlanHistory plan;  // chained launch plan, shared by every context
 };
 
 // Per-BFS device state bound to one stream.
@@ -76,20 +93,7 @@ struct tbfs_ctx {
   unsigned* fin_ctr = nullptr;      // its list length and workgroup count
   tbfs::LoopState* loop_h = nullptr;
   int chain_k = 8;                  // levels enqueued per host check
-  // shapes of the chained level launches, learned from the last call on
-  // pred_graph: bit min(d, 63) of pred_stage set when level d staged the
-  // frontier bitmap for some source (those launches get the staging
-  // workgroups); launches past pred_levels (the one finding the frontier
-  // empty, spares) get a small grid.  No history: every launch stages
-  const tbfs_graph* pred_graph = nullptr;
-  uint64_t pred_stage = ~0ull;
-  int pred_levels = 0;
-  float stage_votes[64] = {};       // decayed count of sources whose level d staged
-  float level_votes[64] = {};       // ... whose BFS ran level d at all
-  float work_votes[64] = {};        // decayed sum of their level d's work (log2 entries)
-  uint64_t pred_dual = 0;           // levels launched in both shapes (staging uncertain)
-  uint8_t pred_work[64] = {};       // expected log2 work of level d (0: unknown)
-  int pred_fin = -1;                // first depth one workgroup finished at (-1: none)
+  const tbfs_graph* async_g = nullptr;  // graph of the queued async runs
   size_t fbl_limit = 0;             // bytes of LDS for a staged frontier bitmap
   int cus = 0;
   int32_t last_n = 0;
@@ -218,6 +222,7 @@ int run_multi(tbfs_ctx* c, const tbfs_graph* g, const int32_t* roots, int nroots
               const tbfs_opts& o, tbfs_stats* st);
 
 // asynchronous single-source runs (query server)
+void plan_learn(const tbfs_graph* g, const RunStats* st, int count);
 int ctx_async_begin(tbfs_ctx* c, int count);
 int ctx_async_enqueue(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs_opts& o,
                       int idx, int levels, int32_t* S, uint64_t* sigma, uint8_t* ovf);

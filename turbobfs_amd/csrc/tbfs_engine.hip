@@ -1073,6 +1073,48 @@ static hipError_t chain_step(hipStream_t s, const LevelPlan& pl, const PersistPa
   return hipSuccess;
 }
 
+// Learn the launch plan of g from finished chained BFS (stats with levels <
+// 0 did not finish and are skipped): per level d, decayed votes (each source
+// weighs 0.95 of the one after it).  A level stages where more than 0.45 of
+// the sources that ran it staged, runs in both shapes where more than 0.27
+// did (on R-MAT scale 20 a lone staging launch loses ~10 us per push it
+// runs, a lone unstaged one ~17 us per heavy pull, and the second launch of
+// a pair costs ~5.5 us); its grid follows the sources' mean work there.
+void plan_learn(const tbfs_graph* g, const RunStats* st, int count) {
+  PlanHistory& h = g->plan;
+  std::lock_guard<std::mutex> lk(h.mu);
+  bool any = false;
+  for (int i = 0; i < count; ++i) {
+    const RunStats& r = st[i];
+    if (r.levels < 0) continue;
+    any = true;
+    const int lv = std::min(64, (int)r.levels);
+    for (int d = 0; d < 64; ++d) {
+      h.stage_votes[d] *= 0.95f;
+      h.level_votes[d] *= 0.95f;
+      h.work_votes[d] *= 0.95f;
+    }
+    for (int d = 0; d < lv; ++d) {
+      h.level_votes[d] += 1.f;
+      h.work_votes[d] += (float)r.work_lg[d];
+      if ((r.stage_mask >> d) & 1ull) h.stage_votes[d] += 1.f;
+    }
+    h.levels = std::max(h.levels, (int)r.levels);
+  }
+  if (!any) return;
+  uint64_t stage = 0, dual = 0;
+  for (int d = 0; d < 64; ++d) {
+    const float lvv = h.level_votes[d];
+    if (h.stage_votes[d] > 0.45f * lvv) stage |= 1ull << d;
+    else if (h.stage_votes[d] > 0.27f * lvv) dual |= 1ull << d;
+    const float w = lvv > 0.f ? h.work_votes[d] / lvv : 0.f;
+    h.work[d] = (uint8_t)std::min(63.f, std::ceil(w));
+  }
+  h.stage = stage;
+  h.dual = dual;
+  h.valid = true;
+}
+
 // about four frontier words per wavefront: fewer workgroups to dispatch per
 // level and fewer counter atomics, still >> 256 CUs' worth of waves
 static int default_lb_grid(const tbfs_graph* g) {
@@ -1091,11 +1133,15 @@ static LevelPlan level_plan(const tbfs_ctx* c, const tbfs_graph* g, const tbfs_o
     pl.fixed = true;
     pl.grid_lb = pl.grid_stage = std::min(o.grid, 65536);
   }
-  if (c->pred_graph == g) {
-    pl.stage = c->pred_stage;
-    pl.dual = c->pred_dual;
-    std::memcpy(pl.work, c->pred_work, sizeof(pl.work));
-    pl.levels = c->pred_levels;
+  {
+    PlanHistory& h = g->plan;
+    std::lock_guard<std::mutex> lk(h.mu);
+    if (h.valid) {
+      pl.stage = h.stage;
+      pl.dual = h.dual;
+      std::memcpy(pl.work, h.work, sizeof(pl.work));
+      pl.levels = h.levels;
+    }
   }
   return pl;
 }
@@ -1455,51 +1501,7 @@ int run_bfs_many(tbfs_ctx* c, const tbfs_graph* gc, const int32_t* roots,
     // spare, so a source needing one launch more than any seen here still
     // finishes without a re-run (a spare launch costs ~4 us, a re-run a BFS)
     c->chain_k = std::min(4096, std::max(2, maxl + 1));
-    // the next call's launch shapes.  Per level d, a decayed vote over the
-    // sources so far on this graph (each source weighs 0.95 of the one
-    // after it): the launch stages when more than a fifth of the sources
-    // that ran level d staged there (on R-MAT scale 20 a staging pull on the
-    // non-staging shape costs ~40 us, a push on the staging shape ~10 us, so
-    // staging pays from a probability of 10 / (40 + 10) on).  And the
-    // deepest BFS so far.
-    if (c->pred_graph != g) {
-      std::fill(std::begin(c->stage_votes), std::end(c->stage_votes), 0.f);
-      std::fill(std::begin(c->level_votes), std::end(c->level_votes), 0.f);
-      std::fill(std::begin(c->work_votes), std::end(c->work_votes), 0.f);
-      c->pred_levels = 0;
-    }
-    int depth = c->pred_levels;
-    for (int i = 0; i < count; ++i) {
-      const RunStats& r = c->stats_h[i];
-      const int lv = std::min(64, (int)r.levels);
-      for (int d = 0; d < 64; ++d) {
-        c->stage_votes[d] *= 0.95f;
-        c->level_votes[d] *= 0.95f;
-        c->work_votes[d] *= 0.95f;
-      }
-      for (int d = 0; d < lv; ++d) {
-        c->level_votes[d] += 1.f;
-        c->work_votes[d] += (float)r.work_lg[d];
-        if ((r.stage_mask >> d) & 1ull) c->stage_votes[d] += 1.f;
-      }
-      depth = std::max(depth, (int)r.levels);
-    }
-    // stage where more than 0.45 of the sources staged, both shapes where
-    // more than 0.27 did: on R-MAT scale 20 a lone staging launch loses
-    // ~10 us per push it runs, a lone unstaged one ~17 us per heavy pull,
-    // and the second launch of a pair costs ~5.5 us
-    uint64_t stage = 0, dual = 0;
-    for (int d = 0; d < 64; ++d) {
-      const float lvv = c->level_votes[d];
-      if (c->stage_votes[d] > 0.45f * lvv) stage |= 1ull << d;
-      else if (c->stage_votes[d] > 0.27f * lvv) dual |= 1ull << d;
-      const float w = lvv > 0.f ? c->work_votes[d] / lvv : 0.f;
-      c->pred_work[d] = (uint8_t)std::min(63.f, std::ceil(w));
-    }
-    c->pred_graph = g;
-    c->pred_stage = stage;
-    c->pred_dual = dual;
-    c->pred_levels = depth;
+    plan_learn(g, c->stats_h, count);
   }
   c->last_n = g->n;
   if (st) {
@@ -1572,6 +1574,7 @@ int ctx_async_enqueue(tbfs_ctx* c, const tbfs_graph* g, int32_t root, const tbfs
   if (rc) return rc;
   rc = enqueue_chained(c, g, root, o, c->stats + idx, levels);
   if (rc) return rc;
+  c->async_g = g;
   hipStream_t s = c->stream;
   const size_t n = (size_t)g->n;
   if (S) HIPCHK(hipMemcpyAsync(S, c->S, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1602,6 +1605,7 @@ int ctx_async_poll(tbfs_ctx* c, int count, bool* ready, std::vector<int>* unfini
     if (c->stats_h[i].levels < 0) unfinished->push_back(i);
     else *maxl = std::max(*maxl, c->stats_h[i].launches);
   }
+  if (c->async_g) plan_learn(c->async_g, c->stats_h, count);
   return TBFS_OK;
 }
 
