@@ -425,7 +425,8 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
             og[g] = __builtin_amdgcn_readfirstlane(L.po[g0 + g]);
             const int ue = __builtin_amdgcn_readfirstlane(L.pu[g0 + g]);
             const uint64_t me = rfl64(L.pm[g0 + g]);
-            if ((me >> ln) & 1ull) {
+            // the edge's mask is the lane mask of its load (no per-lane bit test)
+            if (__builtin_amdgcn_inverse_ballot_w64(me)) {
               x[g] = sgw[(int64_t)ue * 64];
               hm |= 1u << g;
               if (pov && ((ovw[ue] >> ln) & 1ull)) pm |= 1u << g;
@@ -457,6 +458,14 @@ __device__ __forceinline__ void batch_pull(const BatchParams& p, int32_t d, cons
                         __builtin_amdgcn_readfirstlane(L.deg[cur]), acc, hit, ovb, a);
   }
 }
+
+struct PushLds {
+  int32_t excl[64];
+  int32_t beg[64];
+  int32_t u[64];
+  int32_t w[64];
+  uint64_t f[64];
+};
 
 // Pull, one item at a time (graphs with long in-lists): a wavefront tests 64
 // consecutive items of one word at once (one per lane), then expands the
@@ -521,11 +530,14 @@ __device__ __forceinline__ void batch_pull_items(const BatchParams& p, int32_t d
       const int kb = __builtin_amdgcn_readlane(k1, src);
       const int dg = __builtin_amdgcn_readlane(deg, src);
       const uint64_t un = rl64(tgt, src);
-      const CT* sgw = (const CT*)p.sigma + (int64_t)wi * p.n * 64 + ln;
+      const CT* sgw = (const CT*)p.sigma + (int64_t)wi * p.n * 64;  // uniform
       const uint64_t* Fcw = Fc + (int64_t)wi * p.n;
       const uint64_t* ovw = p.ovfw + (int64_t)wi * p.n;
-      CT acc = 0;
-      bool hit = false, ovb = false;
+      // 32-bit counts are summed in 64 bits (at most kBatchSlice terms below
+      // 2^32: no wrap) and checked once; 64-bit counts check each carry
+      uint64_t acc = 0;
+      uint64_t hm = 0, om = 0;  // uniform: sources with a parent / an overflowed parent
+      bool ovb = false;
       for (int c0 = ka; c0 < kb; c0 += 64) {
         const int k = c0 + ln;
         int u = 0;
@@ -535,35 +547,48 @@ __device__ __forceinline__ void batch_pull_items(const BatchParams& p, int32_t d
           mm = Fcw[u] & un;  // sources for which u -> v is a parent edge
         }
         uint64_t bal = __ballot(mm != 0);
-        // parent edges in groups of G: all count loads of a group are issued
-        // before any is added
+        // parent edges in groups of G, their vertex and mask read into
+        // scalar registers from the lane that found them: all count loads of
+        // a group are issued before any is added.  An edge's mask is the
+        // lane mask of its load and of its add (inverse ballot: no per-lane
+        // bit test), so sparse masks touch one line of the segment
         constexpr int G = 16;
         while (bal) {
           CT x[G];
+          uint64_t me[G];
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            x[g] = 0;
-            if (bal) {
+            me[g] = 0;
+            if (bal) {  // wave-uniform
               const int pe = __builtin_ctzll(bal);
               bal &= bal - 1;
               const int ue = __builtin_amdgcn_readlane(u, pe);
-              const uint64_t me = rl64(mm, pe);
-              if ((me >> ln) & 1ull) {
-                x[g] = sgw[(int64_t)ue * 64];
-                hit = true;
-                if (pov && ((ovw[ue] >> ln) & 1ull)) ovb = true;
-              }
+              me[g] = rl64(mm, pe);
+              if (__builtin_amdgcn_inverse_ballot_w64(me[g])) x[g] = sgw[(int64_t)ue * 64 + ln];
+              if (pov) om |= me[g] & ovw[ue];
             }
           }
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            const CT y = acc + x[g];
-            if (y < acc) ovb = true;
-            acc = y;
+            if (__builtin_amdgcn_inverse_ballot_w64(me[g])) {
+              if constexpr (sizeof(CT) == 8) {
+                const uint64_t y = acc + (uint64_t)x[g];
+                if (y < acc) ovb = true;
+                acc = y;
+              } else {
+                acc += (uint64_t)x[g];
+              }
+            }
+            hm |= me[g];
           }
         }
       }
-      pull_flush<W, CT>(p, d, Fn, wi, vi, dg, acc, hit, ovb, a);
+      const bool hit = (hm >> ln) & 1ull;
+      if ((om >> ln) & 1ull) ovb = true;
+      if constexpr (sizeof(CT) == 4) {
+        if (acc >> 32) ovb = true;
+      }
+      pull_flush<W, CT>(p, d, Fn, wi, vi, dg, (CT)acc, hit, ovb, a);
     }
   }
 }
@@ -612,14 +637,6 @@ __device__ __forceinline__ bool push_edge(const BatchParams& p, const uint64_t* 
   }
   return old == 0;
 }
-
-struct PushLds {
-  int32_t excl[64];
-  int32_t beg[64];
-  int32_t u[64];
-  int32_t w[64];
-  uint64_t f[64];
-};
 
 // Top-down pass over the frontier words (u, w): u's out-list edges.  The
 // words come from the frontier's word list when it is complete (`nlist` >= 0
