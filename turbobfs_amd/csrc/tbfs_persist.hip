@@ -1494,10 +1494,15 @@ __device__ __forceinline__ void fin_run(const PersistParams& p, LoopState& s, in
 // levels stage the frontier bitmap in dynamic LDS (one workgroup per CU).
 // `step` only selects the state slot (its parity equals the depth's): the
 // depth itself comes from the state, so one captured chunk of level kernels
-// serves every stretch of levels starting at an even depth.
-template <int BLOCK, bool FBL>
-__global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t step,
-                                                       int32_t want) {
+// serves every stretch of levels starting at an even depth.  FIN: the
+// instantiation that can hand the BFS to the one-workgroup finisher (taken
+// only when the option asks for it): compiled into every level kernel, the
+// finisher's registers made the staging launch spill 120 bytes per lane and
+// left the 256-thread one at 148 VGPRs (three waves per SIMD); without it
+// both fit four waves per SIMD.
+template <int BLOCK, bool FBL, bool FIN>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+void k_chain_level(PersistParams p, int32_t step, int32_t want) {
   __shared__ WaveLds lds[BLOCK / 64];
   extern __shared__ uint64_t fb_dyn[];
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1520,7 +1525,8 @@ __global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t 
     return;
   }
   const bool heavy_open = s.heavy_reached < (unsigned long long)p.nheavy;
-  const bool fin = p.fin_mu && s.mu <= (unsigned long long)p.fin_mu && p.mode == TBFS_MODE_AUTO;
+  const bool fin = FIN && p.fin_mu && s.mu <= (unsigned long long)p.fin_mu &&
+                   p.mode == TBFS_MODE_AUTO;
   const bool stage = !fin && wants_stage(p, s.pushing != 0, s.mu);
   // one of a pair of launches for this step: the other one runs the level
   if (want == kLevelIfStaged && !stage) return;
@@ -1529,22 +1535,24 @@ __global__ __launch_bounds__(BLOCK) void k_chain_level(PersistParams p, int32_t 
     const unsigned long long w = s.pushing ? c.mout : s.mu;
     p.stats->work_lg[d] = (uint8_t)(64 - __clzll((long long)w));
   }
-  if (fin) {
-    // the rest of the BFS in the last workgroup to list its columns
-    static_assert(sizeof(FinLds<BLOCK>) <= sizeof(lds), "the finisher's LDS fits the level's");
-    FinLds<BLOCK>& F = *reinterpret_cast<FinLds<BLOCK>*>(lds);
-    fin_list_columns(p, d, heavy_open, gtid, gstride);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __threadfence();
-      const unsigned prev = atomicAdd(&p.fin_ctr[1], 1u);
-      F.last = prev == gridDim.x - 1 ? 1u : 0u;
-      if (F.last) __threadfence();
+  if constexpr (FIN) {
+    if (fin) {
+      // the rest of the BFS in the last workgroup to list its columns
+      static_assert(sizeof(FinLds<BLOCK>) <= sizeof(lds), "the finisher's LDS fits the level's");
+      FinLds<BLOCK>& F = *reinterpret_cast<FinLds<BLOCK>*>(lds);
+      fin_list_columns(p, d, heavy_open, gtid, gstride);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned prev = atomicAdd(&p.fin_ctr[1], 1u);
+        F.last = prev == gridDim.x - 1 ? 1u : 0u;
+        if (F.last) __threadfence();
+      }
+      __syncthreads();
+      if (!F.last) return;
+      fin_run<BLOCK>(p, s, step, F);
+      return;
     }
-    __syncthreads();
-    if (!F.last) return;
-    fin_run<BLOCK>(p, s, step, F);
-    return;
   }
   LoopState nx = s;
   nx.d = d + 1;
@@ -1617,26 +1625,44 @@ hipError_t chain_seed(hipStream_t s, const PersistParams& p) {
 
 hipError_t chain_level(hipStream_t s, int grid, const PersistParams& p, int32_t d,
                        size_t fbl_bytes, int32_t want) {
-  if (fbl_bytes)
-    hipLaunchKernelGGL((k_chain_level<1024, true>), dim3(grid), dim3(1024), fbl_bytes, s, p, d,
-                       want);
-  else
-    hipLaunchKernelGGL((k_chain_level<kLbBlock, false>), dim3(grid), dim3(kLbBlock), 0, s, p, d,
-                       want);
+  // the finisher's instantiations only when the run may hand over to it
+  if (fbl_bytes) {
+    if (p.fin_mu)
+      hipLaunchKernelGGL((k_chain_level<1024, true, true>), dim3(grid), dim3(1024), fbl_bytes, s,
+                         p, d, want);
+    else
+      hipLaunchKernelGGL((k_chain_level<1024, true, false>), dim3(grid), dim3(1024), fbl_bytes, s,
+                         p, d, want);
+  } else {
+    if (p.fin_mu)
+      hipLaunchKernelGGL((k_chain_level<kLbBlock, false, true>), dim3(grid), dim3(kLbBlock), 0, s,
+                         p, d, want);
+    else
+      hipLaunchKernelGGL((k_chain_level<kLbBlock, false, false>), dim3(grid), dim3(kLbBlock), 0, s,
+                         p, d, want);
+  }
   return hipGetLastError();
 }
 
 hipError_t chain_fbl_limit(size_t* bytes) {
-  // LDS left for the staged bitmap next to the kernel's static LDS
-  hipFuncAttributes fa;
-  hipError_t e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_chain_level<1024, true>));
-  if (e != hipSuccess) return e;
+  // LDS left for the staged bitmap next to the kernels' static LDS (both
+  // staging instantiations take the same limit)
+  const void* fns[2] = {reinterpret_cast<const void*>(k_chain_level<1024, true, false>),
+                        reinterpret_cast<const void*>(k_chain_level<1024, true, true>)};
   const size_t lds_cu = 160 * 1024;
-  size_t avail = lds_cu > fa.sharedSizeBytes ? lds_cu - fa.sharedSizeBytes : 0;
+  size_t avail = lds_cu;
+  for (const void* fn : fns) {
+    hipFuncAttributes fa;
+    const hipError_t e = hipFuncGetAttributes(&fa, fn);
+    if (e != hipSuccess) return e;
+    avail = std::min(avail, lds_cu > fa.sharedSizeBytes ? lds_cu - fa.sharedSizeBytes : 0);
+  }
   avail &= ~(size_t)255;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_chain_level<1024, true>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail);
-  if (e != hipSuccess) return e;
+  for (const void* fn : fns) {
+    const hipError_t e =
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail);
+    if (e != hipSuccess) return e;
+  }
   *bytes = avail;
   return hipSuccess;
 }
