@@ -138,6 +138,7 @@ def bench_dropin(tb, g, sources, nsrc: int = 4):
         out[f"repetition_{rep}"] = {
             "ms_per_call": ms, "ms_per_bfs": ms / rep,
             "gteps_pcie_inclusive": edges / 2 * rep / (sum(t)) / 1e9}
+    tb.dropin_release()  # the kept contexts and buffers go before the next section
     return out
 
 
@@ -255,6 +256,7 @@ def bench_er4096(tb, graphs, reps: int = 256, cpu: bool = True):
     for _ in range(k):
         tb.bfs_gpu_td_csc_sc(g.IC, g.CP, S, sig, 0, g.nz, g.n, 1)
     out["dropin_bfs_gpu_td_csc_sc_ms_per_call"] = 1e3 * (time.perf_counter() - t0) / k
+    tb.dropin_release()
     if cpu:  # source 0 repeated until ~2 s of CPU time, every BFS timed in full
         out["cpu_baseline"] = cpu_baseline(g, [0] * 1000, 2.0)
     return out
@@ -413,6 +415,22 @@ def _bench_grid(tb, g, gr, ctx, b, src, nsrc, names, cpu_passes):
             "sample": f"first {made} of the {levels0 + 1} passes of the restated bfs_seq_td_csc "
                       f"(each pass sweeps every column) from the first source, single thread, "
                       f"{per_pass * made:.1f} s; the whole BFS = passes x time per pass"}
+    return out
+
+
+def bench_serving_process():
+    """bench_serving in a process of its own, as a server runs in deployment.
+    Measured in the process that ran the other sections, the same server
+    loses ~13% of its saturated rate (~2070 -> ~1800 queries/s) and its
+    latency at 80% load grows several-fold: after work has run on other HIP
+    streams of the process (even streams since destroyed), the four slot
+    streams no longer get the hardware queues to themselves.  The child's
+    JSON object is returned as it printed it."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--only", "serve"],
+                       stdout=subprocess.PIPE, text=True, check=True)
+    out = json.loads(r.stdout.strip().splitlines()[-1])["serve"]
+    out["process"] = "measured in a child process of its own (bench.py --only serve)"
     return out
 
 
@@ -678,7 +696,7 @@ def main():
         line["multi_graph"] = bench_multigraph(tb, graphs)
         line["high_diameter"] = bench_grid(tb, graphs, cpu_passes=0 if args.no_cpu else 24)
     if not args.no_serve:
-        line["serving"] = bench_serving(tb, graphs)
+        line["serving"] = bench_serving_process()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(g, sources, args.cpu_budget)
     print(json.dumps(line))
