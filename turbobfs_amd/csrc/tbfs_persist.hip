@@ -36,7 +36,11 @@ namespace {
 using namespace dev;
 
 constexpr int kUnroll = 4;  // independent edges in flight per lane
-constexpr int kPullRegs = 8;  // in-list entries per lane per light-column pass
+// in-list entries per lane per light-column pass.  A group's last pass is
+// partly empty and its idle slots still issue their (clamped) loads, so
+// narrow passes waste fewer requests: on R-MAT 20, 8 / 6 / 4 / 3 / 2 / 1
+// entries gave ~99 / 101 / 103 / 104.3 / 104.2 / 99.9 GTEPS.
+constexpr int kPullRegs = 2;
 constexpr int kSliceRegs = kPullChunk / kWave;  // slice entries per lane
 constexpr int kSlicesPerPass = 2;  // pull slices taken together by a wavefront
 
@@ -514,7 +518,7 @@ __device__ __forceinline__ void pull_passes(const GraphView& g, const StateView&
 
 // One group of up to 64 columns of a pull level, column v with in-list
 // [beg, beg + deg) on its lane (deg 0: none; deg <= kBigIn): the in-lists are
-// walked in passes of 4 or 8 entries per lane (parents' counts summed per
+// walked in passes of kPullRegs entries per lane (parents' counts summed per
 // column in LDS), and the columns that found a parent get their depth, count
 // and overflow flag and are tallied.  Returns the wavefront's mask of those
 // columns (lane bits).  All lanes must take part.
@@ -540,9 +544,9 @@ __device__ __forceinline__ uint64_t pull_group(const GraphView& g, const StateVi
   L->x[lane()] = 0;
   if (SIGMA) L->xh[lane()] = 0;
   wave_lds_sync();
-  // one pass of 4 entries per lane for short groups, else passes of 8
-  if (total <= 64 * 4) {
-    pull_passes<4, SIGMA, false>(g, st, L, fpar, ovf_any, total, xj, hasd, kmax);
+  // one pass for short groups, else pipelined passes of kPullRegs entries
+  if (total <= 64 * kPullRegs) {
+    pull_passes<kPullRegs, SIGMA, false>(g, st, L, fpar, ovf_any, total, xj, hasd, kmax);
   } else {
     pull_passes<kPullRegs, SIGMA, true>(g, st, L, fpar, ovf_any, total, xj, hasd, kmax);
   }
